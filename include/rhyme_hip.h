@@ -1,0 +1,181 @@
+/*
+ * rhyme_hip.h -- C ABI of the MI355X (gfx950) hydro hot path of Rhyme.
+ *
+ * The reference's hot path is the body of its main loop (src/main.f90:106-177):
+ *   dt = rhyme_cfl_time_step(...)            (src/main.f90:109)
+ *   call rhyme_samr_bc_set_boundaries(...)   (src/main.f90:115)
+ *   call rhyme_muscl_hancock_solve(...)      (src/main.f90:132-134)
+ * Each entry point below replaces one reference module procedure (cited
+ * file:line, relative to the reference's src/).  The reference keeps every
+ * array in Fortran allocatables and never returns status; here device memory
+ * is owned by the library behind an opaque box handle, host arrays are
+ * borrowed mirrors, and every call returns an int status (RHY_OK = 0).
+ *
+ * Host array layout (identical to samr_box_t%cells, rhyme_samr/src/
+ * rhyme_samr.f90:34-42 and rhyme_samr_init_box.f90:29-33): cells(i,j,k,c)
+ * column-major with 2 ghost layers on every active axis, i.e. extents
+ * (n1+4)[*(n2+4)[*(n3+4)]] * ncmp doubles; components rho, rho_u, [rho_v,
+ * [rho_w,]] e_tot, then ncmp-(ndim+2) non-hydro components
+ * (rhyme_units/src/rhyme_units.f90:32-57).
+ *
+ * Threading: calls on one box are serialised on that box's HIP stream; calls
+ * on different boxes may come from different host threads.
+ */
+#ifndef RHYME_HIP_H
+#define RHYME_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define RHY_OK 0
+#define RHY_ERR_ARG (-1)      /* bad argument (NULL, unknown enum value)  */
+#define RHY_ERR_HIP (-2)      /* a HIP runtime call failed                 */
+#define RHY_ERR_OOM (-3)      /* device allocation failed                  */
+#define RHY_ERR_SHAPE (-4)    /* unsupported box shape                     */
+#define RHY_ERR_NODEV (-5)    /* no usable gfx950 device                   */
+#define RHY_ERR_SOLVER (-6)   /* unknown solver_type (logger%err in ref)   */
+
+/* ---- enumerations (same integer values as the reference) ---- */
+/* slope limiter ids: slid, rhyme_slope_limiter/src/rhyme_slope_limiter.f90:6-8 */
+#define RHY_VAN_LEER 1
+#define RHY_MINMOD 2
+#define RHY_VAN_ALBADA 3
+#define RHY_SUPERBEE 4
+/* solver types: mhid, rhyme_muscl_hancock/src/rhyme_muscl_hancock.f90:10-15 */
+#define RHY_CPU_INTENSIVE 1
+#define RHY_MEMORY_INTENSIVE 2
+/* boundary types: bcid, rhyme_samr_bc/src/rhyme_samr_bc.f90:9-22 */
+#define RHY_BC_REFLECTIVE 1
+#define RHY_BC_OUTFLOW 2
+#define RHY_BC_PERIODIC 3
+#define RHY_BC_INFLOW 4
+/* box storage precision */
+#define RHY_FP64 0   /* state, arithmetic and dt in FP64 (the drop-in)       */
+#define RHY_MIXED 1  /* FP32 state storage and bulk arithmetic; FP64 p* Newton
+                        solve, wave-speed sampling and dt                     */
+
+#define RHY_MAX_CMP 16
+
+/*
+ * Every parameter the reference keeps in process globals or derived-type
+ * components, passed explicitly:
+ *   gamma, kb_amu           rhyme_thermo_base.f90:21-26,130
+ *   n_iteration, tolerance  irs_t (rhyme_irs/src/rhyme_irs.f90:16-19)
+ *   w_vacuum_rho/_p         riemann_problem_t%w_vacuum (…riemann_problem.f90:24-31)
+ *   limiter, omega          slope_limiter_t (rhyme_slope_limiter.f90:12-15)
+ *   solver_type             muscl_hancock_t (rhyme_muscl_hancock.f90:17-19)
+ *   courant                 cfl_t (rhyme_cfl/src/rhyme_cfl.f90:7-9)
+ *   bc_types, cons_inflows  samr_bc_t (rhyme_samr_bc.f90:24-28); faces in the
+ *                           order left, right, bottom, top, back, front.
+ */
+typedef struct rhy_params {
+    int32_t ndim;
+    int32_t limiter;
+    int32_t solver_type;
+    int32_t n_iteration;
+    double gamma;
+    double kb_amu;
+    double omega;
+    double tolerance;
+    double w_vacuum_rho;
+    double w_vacuum_p;
+    double courant;
+    int32_t bc_types[6];
+    int32_t ncmp;
+    int32_t reserved;
+    double cons_inflows[6][RHY_MAX_CMP];
+} rhy_params_t;
+
+typedef struct rhy_box rhy_box_t; /* opaque device-resident box */
+
+/* Select the device and check it is usable; idempotent. */
+int rhy_init(int32_t device);
+/* Text of the last error on the calling thread ("" if none). */
+const char *rhy_last_error(void);
+/* Number of kernels / variants compiled in (for "library loads" checks). */
+int32_t rhy_version(void);
+
+/* ---- box lifetime and host mirror transfers ----
+ * Replaces the allocation side of rhyme_samr_init_box (rhyme_samr/src/
+ * rhyme_samr_init_box.f90:29-33) and rhyme_mh_workspace_check
+ * (rhyme_muscl_hancock/src/rhyme_mh_workspace/src/rhyme_mh_workspace_check.f90:3-104):
+ * the device copy and all scratch are created here and resized by
+ * rhy_box_resize. dims[d] for d >= ndim must be 1; active dims must be >= 2. */
+int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp,
+                   int32_t precision, rhy_box_t **out);
+int rhy_box_resize(rhy_box_t *box, const int32_t dims[3]);
+int rhy_box_destroy(rhy_box_t *box);
+/* H2D / D2H of the whole ghosted host array (all ncmp components). */
+int rhy_box_upload(rhy_box_t *box, const double *cells);
+int rhy_box_download(rhy_box_t *box, double *cells);
+/* Wait for all work queued on the box. */
+int rhy_box_sync(rhy_box_t *box);
+
+/* ---- the module procedures ---- */
+
+/* rhyme_samr_bc_set_boundaries (rhyme_samr_bc/src/rhyme_samr_bc_set_boundaries.f90:3-22):
+ * ghost fill of the 2*ndim faces (faces only, 2 layers). */
+int rhy_bc_set_boundaries(rhy_box_t *box, const rhy_params_t *p);
+/* rhyme_samr_bc_set_{left,right,bottom,top,back,front}_boundary
+ * (…_set_left_boundary.f90:3-92 etc.); face = 0..5 in that order. */
+int rhy_bc_set_face(rhy_box_t *box, const rhy_params_t *p, int32_t face);
+
+/* rhyme_cfl_time_step (rhyme_cfl/src/rhyme_cfl_time_step.f90:3-60):
+ * dt = courant * minval(dx) / max(|v| + c_s) over the interior. */
+int rhy_cfl_time_step(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
+                      double *dt);
+
+/* rhyme_muscl_hancock_solve (rhyme_muscl_hancock/src/rhyme_muscl_hancock_solve.f90:3-30),
+ * dispatching on p->solver_type to the memory_intensive
+ * (…_solve_memory_intensive.f90:3-181) or cpu_intensive
+ * (…_solve_cpu_intensive.f90:3-159) update order.  Reads the box's current
+ * ghost cells; advances the interior by dt in place. */
+int rhy_mh_solve(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
+                 double dt);
+
+/* The main-loop step of src/main.f90:109-136 repeated nsteps times, all on
+ * the device: dt from the CFL reduction, ghost fill, MUSCL-Hancock.  dts
+ * (host, may be NULL) receives the dt of every step.  Blocks until done. */
+int rhy_run(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
+            int32_t nsteps, double *dts);
+
+/* Same, enqueue only (no host sync); dts are kept on the device and can be
+ * fetched after rhy_box_sync with rhy_box_last_dts. */
+int rhy_run_async(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
+                  int32_t nsteps);
+int rhy_box_last_dts(rhy_box_t *box, int32_t n, double *dts);
+
+/* Per-kernel timing with HIP events recorded on the box's stream around every
+ * launch while enabled.  rhy_box_kernel_stats returns, for kernel index
+ * kidx (0: the fused/flux step kernel, 1: update, 2: bc, 3: cfl, 4: dt), the
+ * number of launches timed and their summed milliseconds. */
+int rhy_box_set_profiling(rhy_box_t *box, int32_t on);
+int rhy_box_kernel_stats(rhy_box_t *box, int32_t kidx, int64_t *launches,
+                         double *total_ms);
+
+/* ---- unit-level batches (host arrays, NVAR = ndim+2 values per record,
+ * record-major) so the reference's known-answer tests can run on the GPU ---- */
+
+/* rhyme_riemann_problem_solve at dx = 0 (…_solve.f90:3-104): sampled
+ * conserved state on each interface; axis[q] in 1..ndim. */
+int rhy_riemann_batch(const rhy_params_t *p, int64_t n, const double *l,
+                      const double *r, const int32_t *axis, double *u);
+/* rhyme_slope_limiter_run (rhyme_slope_limiter/src/rhyme_slope_limiter_run.f90:3-21) */
+int rhy_limiter_batch(const rhy_params_t *p, int64_t n, const double *ul,
+                      const double *u, const double *ur, double *delta);
+/* rhyme_muscl_hancock_half_step_extrapolation (…_half_step_extrapolation.f90:3-24) */
+int rhy_half_step_batch(const rhy_params_t *p, int64_t n, const double *u,
+                        const double *delta, const int32_t *axis, double dx,
+                        double dt, double *l, double *r);
+/* calc_flux -> rhyme_ideal_gas_flux (rhyme_ideal_gas/src/rhyme_ideal_gas_flux.f90:3-24) */
+int rhy_flux_batch(const rhy_params_t *p, int64_t n, const double *u,
+                   const int32_t *axis, double *f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,128 @@
+"""ctypes binding of librhyme_hip.so (the C ABI declared in include/rhyme_hip.h).
+
+The product path: every call goes to the HIP library.  There is no CPU
+fallback; if the library is missing or cannot find a gfx950 device, calls
+raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librhyme_hip.so")
+
+RHY_OK = 0
+MAX_CMP = 16
+
+# Every symbol include/rhyme_hip.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "rhy_init", "rhy_last_error", "rhy_version",
+    "rhy_box_create", "rhy_box_resize", "rhy_box_destroy", "rhy_box_upload",
+    "rhy_box_download", "rhy_box_sync",
+    "rhy_bc_set_boundaries", "rhy_bc_set_face", "rhy_cfl_time_step",
+    "rhy_mh_solve", "rhy_run", "rhy_run_async", "rhy_box_last_dts",
+    "rhy_box_set_profiling", "rhy_box_kernel_stats",
+    "rhy_riemann_batch", "rhy_limiter_batch", "rhy_half_step_batch",
+    "rhy_flux_batch",
+]
+
+
+class RhyParams(C.Structure):
+    _fields_ = [
+        ("ndim", C.c_int32),
+        ("limiter", C.c_int32),
+        ("solver_type", C.c_int32),
+        ("n_iteration", C.c_int32),
+        ("gamma", C.c_double),
+        ("kb_amu", C.c_double),
+        ("omega", C.c_double),
+        ("tolerance", C.c_double),
+        ("w_vacuum_rho", C.c_double),
+        ("w_vacuum_p", C.c_double),
+        ("courant", C.c_double),
+        ("bc_types", C.c_int32 * 6),
+        ("ncmp", C.c_int32),
+        ("reserved", C.c_int32),
+        ("cons_inflows", (C.c_double * MAX_CMP) * 6),
+    ]
+
+
+class RhyError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Load librhyme_hip.so and declare its signatures (no device needed)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RhyError("librhyme_hip.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int32)
+    vp = C.c_void_p
+    pp = C.POINTER(RhyParams)
+    sig = {
+        "rhy_init": (C.c_int, [C.c_int32]),
+        "rhy_last_error": (C.c_char_p, []),
+        "rhy_version": (C.c_int32, []),
+        "rhy_box_create": (C.c_int, [C.c_int32, ip, C.c_int32, C.c_int32, C.POINTER(vp)]),
+        "rhy_box_resize": (C.c_int, [vp, ip]),
+        "rhy_box_destroy": (C.c_int, [vp]),
+        "rhy_box_upload": (C.c_int, [vp, dp]),
+        "rhy_box_download": (C.c_int, [vp, dp]),
+        "rhy_box_sync": (C.c_int, [vp]),
+        "rhy_bc_set_boundaries": (C.c_int, [vp, pp]),
+        "rhy_bc_set_face": (C.c_int, [vp, pp, C.c_int32]),
+        "rhy_cfl_time_step": (C.c_int, [vp, pp, dp, dp]),
+        "rhy_mh_solve": (C.c_int, [vp, pp, dp, C.c_double]),
+        "rhy_run": (C.c_int, [vp, pp, dp, C.c_int32, dp]),
+        "rhy_run_async": (C.c_int, [vp, pp, dp, C.c_int32]),
+        "rhy_box_last_dts": (C.c_int, [vp, C.c_int32, dp]),
+        "rhy_box_set_profiling": (C.c_int, [vp, C.c_int32]),
+        "rhy_box_kernel_stats": (C.c_int, [vp, C.c_int32, C.POINTER(C.c_int64),
+                                           C.POINTER(C.c_double)]),
+        "rhy_riemann_batch": (C.c_int, [pp, C.c_int64, dp, dp, ip, dp]),
+        "rhy_limiter_batch": (C.c_int, [pp, C.c_int64, dp, dp, dp, dp]),
+        "rhy_half_step_batch": (C.c_int, [pp, C.c_int64, dp, dp, ip, C.c_double,
+                                          C.c_double, dp, dp]),
+        "rhy_flux_batch": (C.c_int, [pp, C.c_int64, dp, ip, dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, what):
+    if rc != RHY_OK:
+        msg = load().rhy_last_error()
+        raise RhyError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+def dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+def dx3(dx):
+    out = np.ones(3, dtype=np.float64)
+    dx = np.atleast_1d(np.asarray(dx, dtype=np.float64))
+    out[:len(dx)] = dx
+    return out
+
+
+def dims3(dims):
+    out = np.ones(3, dtype=np.int32)
+    out[:len(dims)] = dims
+    return out

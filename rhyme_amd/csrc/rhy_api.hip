@@ -1,0 +1,725 @@
+// rhy_api.hip -- the C ABI of include/rhyme_hip.h: device selection, the
+// device-resident box (state, flux scratch, CFL slots, dt log, stream),
+// host mirror transfers, and the launch sequences of the module procedures.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/rhyme_hip.h"
+#include "rhy_kernels.hpp"
+
+using namespace rhy;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(RHY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));   \
+  } while (0)
+
+std::once_flag g_init_once;
+int g_init_rc = RHY_ERR_NODEV;
+int g_device = -1;
+
+constexpr int kThreads = 256;
+constexpr int kNumKernelIds = 5;  // flux/step, update, bc, cfl, dt
+
+struct TimedLaunch {
+  int kidx;
+  hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct rhy_box {
+  int32_t ndim = 0;
+  int32_t dims[3] = {1, 1, 1};
+  int32_t ncmp = 0;
+  int32_t precision = RHY_FP64;
+  Geom geom{};
+  hipStream_t stream = nullptr;
+  double* cells = nullptr;        // ncmp planes (hydro first)
+  double* flux = nullptr;         // ndim * (ndim+2) planes
+  unsigned long long* slots = nullptr;  // [0..1] CFL max slots
+  double* dt = nullptr;           // current dt
+  double* dt_log = nullptr;       // per-step dt of the last run
+  int32_t dt_log_cap = 0;
+  int32_t last_nsteps = 0;
+  bool profiling = false;
+  std::vector<TimedLaunch> pending;
+  int64_t launches[kNumKernelIds] = {0, 0, 0, 0, 0};
+  double total_ms[kNumKernelIds] = {0, 0, 0, 0, 0};
+};
+
+namespace {
+
+Geom make_geom(int nd, const int32_t* dims) {
+  Geom G{};
+  int64_t st = 1;
+  for (int d = 0; d < 3; ++d) {
+    G.n[d] = d < nd ? dims[d] : 1;
+    G.g[d] = d < nd ? 2 : 0;
+    G.s[d] = st;
+    st *= (int64_t)(G.n[d] + 2 * G.g[d]);
+  }
+  G.plane = st;
+  return G;
+}
+
+int check_params(const rhy_params_t* p, int nd) {
+  if (!p) return fail(RHY_ERR_ARG, "params is NULL");
+  if (p->ndim != nd) return fail(RHY_ERR_ARG, "params.ndim does not match the box");
+  if (p->limiter < 1 || p->limiter > 4) return fail(RHY_ERR_ARG, "unknown slope limiter");
+  return RHY_OK;
+}
+
+Consts make_consts(const rhy_params_t* p) {
+  Consts k{};
+  const double g = p->gamma;
+  k.g = g;
+  k.gm1 = g - 1.0;
+  k.gp1 = g + 1.0;
+  k.gm1_gp1 = k.gm1 / k.gp1;
+  k.gm1_2g = k.gm1 / (2.0 * g);
+  k.gp1_2g = k.gp1 / (2.0 * g);
+  k.g_inv = 1.0 / g;
+  k.kb_amu = p->kb_amu;
+  k.omega = p->omega;
+  k.tol = p->tolerance;
+  k.w_rho = p->w_vacuum_rho;
+  k.w_p = p->w_vacuum_p;
+  k.e_rho = std::nextafter(p->w_vacuum_rho, INFINITY);
+  k.limiter = p->limiter;
+  k.n_iter = p->n_iteration;
+  return k;
+}
+
+BcTypes make_bc(const rhy_params_t* p) {
+  BcTypes b{};
+  for (int f = 0; f < 6; ++f) b.t[f] = p->bc_types[f];
+  return b;
+}
+
+unsigned grid_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+void timed_begin(rhy_box* b, int kidx, TimedLaunch& tl) {
+  tl.kidx = kidx;
+  tl.a = tl.b = nullptr;
+  if (!b->profiling) return;
+  hipEventCreate(&tl.a);
+  hipEventCreate(&tl.b);
+  hipEventRecord(tl.a, b->stream);
+}
+
+void timed_end(rhy_box* b, TimedLaunch& tl) {
+  if (!b->profiling) return;
+  hipEventRecord(tl.b, b->stream);
+  b->pending.push_back(tl);
+}
+
+void collect_timings(rhy_box* b) {
+  for (auto& tl : b->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, tl.a, tl.b) == hipSuccess) {
+      b->launches[tl.kidx] += 1;
+      b->total_ms[tl.kidx] += ms;
+    }
+    hipEventDestroy(tl.a);
+    hipEventDestroy(tl.b);
+  }
+  b->pending.clear();
+}
+
+template <int ND>
+int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
+  const Geom& G = b->geom;
+  const int64_t P = G.plane;
+  const int nv = ND + 2;
+  for (int a = 0; a < ND; ++a) {
+    int64_t nf = (int64_t)(G.n[0] + (a == 0)) * (G.n[1] + (a == 1)) * (G.n[2] + (a == 2));
+    double* F = b->flux + (int64_t)a * nv * P;
+    TimedLaunch tl;
+    timed_begin(b, 0, tl);
+    if (a == 0)
+      hipLaunchKernelGGL((flux_kernel<ND, 0>), dim3(grid_for(nf)), dim3(kThreads), 0, b->stream,
+                         k, G, b->cells, F, b->dt, dx[0]);
+    else if (a == 1)
+      hipLaunchKernelGGL((flux_kernel<ND, (ND > 1 ? 1 : 0)>), dim3(grid_for(nf)), dim3(kThreads),
+                         0, b->stream, k, G, b->cells, F, b->dt, dx[1]);
+    else
+      hipLaunchKernelGGL((flux_kernel<ND, (ND > 2 ? 2 : 0)>), dim3(grid_for(nf)), dim3(kThreads),
+                         0, b->stream, k, G, b->cells, F, b->dt, dx[2]);
+    timed_end(b, tl);
+  }
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+template <int ND, int ORDER, bool FUSED>
+int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc,
+                    unsigned long long* slot_next) {
+  const Geom& G = b->geom;
+  const int64_t P = G.plane;
+  const int nv = ND + 2;
+  int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  const double* F0 = b->flux;
+  const double* F1 = ND > 1 ? b->flux + (int64_t)nv * P : nullptr;
+  const double* F2 = ND > 2 ? b->flux + (int64_t)2 * nv * P : nullptr;
+  TimedLaunch tl;
+  timed_begin(b, 1, tl);
+  hipLaunchKernelGGL((update_kernel<ND, ORDER, FUSED>), dim3(grid_for(n)), dim3(kThreads), 0,
+                     b->stream, k, G, b->cells, F0, F1, F2, b->dt, dx[0], dx[1], dx[2], bc,
+                     slot_next);
+  timed_end(b, tl);
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+template <int ND>
+int launch_update(rhy_box* b, const Consts& k, const double* dx, int order, bool fused,
+                  BcTypes bc, unsigned long long* slot_next) {
+  if (order == RHY_MEMORY_INTENSIVE)
+    return fused ? launch_update_t<ND, 2, true>(b, k, dx, bc, slot_next)
+                 : launch_update_t<ND, 2, false>(b, k, dx, bc, slot_next);
+  return fused ? launch_update_t<ND, 1, true>(b, k, dx, bc, slot_next)
+               : launch_update_t<ND, 1, false>(b, k, dx, bc, slot_next);
+}
+
+template <int ND>
+int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
+  const Geom& G = b->geom;
+  int64_t maxface = 1;
+  for (int a = 0; a < ND; ++a) {
+    int64_t m = 1;
+    for (int d = 0; d < 3; ++d)
+      if (d != a) m *= G.n[d];
+    maxface = m > maxface ? m : maxface;
+  }
+  // inflow states travel as a kernel argument copy in device memory
+  double* dinf = reinterpret_cast<double*>(b->slots + 2);
+  HIP_TRY(hipMemcpyAsync(dinf, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
+                         hipMemcpyHostToDevice, b->stream));
+  TimedLaunch tl;
+  timed_begin(b, 2, tl);
+  dim3 grid(grid_for(maxface), face_only >= 0 ? 1 : 2 * ND);
+  hipLaunchKernelGGL((bc_kernel<ND>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p), dinf,
+                     b->ncmp, b->cells, face_only);
+  timed_end(b, tl);
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+template <int ND>
+int launch_cfl(rhy_box* b, const Consts& k, unsigned long long* slot) {
+  const Geom& G = b->geom;
+  int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  TimedLaunch tl;
+  timed_begin(b, 3, tl);
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, b->stream, (double*)nullptr, 0.0,
+                     slot);
+  hipLaunchKernelGGL((cfl_kernel<ND>), dim3(grid_for(n)), dim3(kThreads), 0, b->stream, k, G,
+                     b->cells, slot);
+  timed_end(b, tl);
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+double min_dx(int nd, const double* dx) {
+  double m = dx[0];
+  for (int d = 1; d < nd; ++d) m = dx[d] < m ? dx[d] : m;
+  return m;
+}
+
+int ensure_dt_log(rhy_box* b, int32_t n) {
+  if (n <= b->dt_log_cap) return RHY_OK;
+  if (b->dt_log) hipFree(b->dt_log);
+  b->dt_log = nullptr;
+  b->dt_log_cap = 0;
+  if (hipMalloc(&b->dt_log, sizeof(double) * (size_t)n) != hipSuccess)
+    return fail(RHY_ERR_OOM, "dt log allocation failed");
+  b->dt_log_cap = n;
+  return RHY_OK;
+}
+
+template <int ND>
+int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nsteps) {
+  const Consts k = make_consts(p);
+  const BcTypes bc = make_bc(p);
+  int rc;
+  if ((rc = ensure_dt_log(b, nsteps > 0 ? nsteps : 1))) return rc;
+  // prologue: CFL of the current state and the ghost fill it is stepped with
+  if ((rc = launch_cfl<ND>(b, k, b->slots + 0))) return rc;
+  if ((rc = launch_bc<ND>(b, p, -1))) return rc;
+  const double cmdx = p->courant * min_dx(ND, dx);
+  for (int32_t s = 0; s < nsteps; ++s) {
+    unsigned long long* cur = b->slots + (s & 1);
+    unsigned long long* nxt = b->slots + ((s + 1) & 1);
+    TimedLaunch tl;
+    timed_begin(b, 4, tl);
+    hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, b->stream, cmdx, cur, nxt, b->dt,
+                       b->dt_log + s);
+    timed_end(b, tl);
+    if ((rc = launch_flux<ND>(b, k, dx))) return rc;
+    if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc, nxt))) return rc;
+  }
+  b->last_nsteps = nsteps;
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+int check_box(rhy_box* b) {
+  if (!b) return fail(RHY_ERR_ARG, "box is NULL");
+  return RHY_OK;
+}
+
+void free_box_memory(rhy_box* b) {
+  if (b->cells) hipFree(b->cells);
+  if (b->flux) hipFree(b->flux);
+  b->cells = nullptr;
+  b->flux = nullptr;
+}
+
+int alloc_box_memory(rhy_box* b) {
+  b->geom = make_geom(b->ndim, b->dims);
+  const int64_t P = b->geom.plane;
+  const int nv = b->ndim + 2;
+  if (hipMalloc(&b->cells, sizeof(double) * (size_t)P * b->ncmp) != hipSuccess) {
+    free_box_memory(b);
+    return fail(RHY_ERR_OOM, "state allocation failed");
+  }
+  if (hipMalloc(&b->flux, sizeof(double) * (size_t)P * nv * b->ndim) != hipSuccess) {
+    free_box_memory(b);
+    return fail(RHY_ERR_OOM, "flux scratch allocation failed");
+  }
+  // ghosts start at 0 (rhyme_samr_bc_init.f90:61-87)
+  HIP_TRY(hipMemsetAsync(b->cells, 0, sizeof(double) * (size_t)P * b->ncmp, b->stream));
+  return RHY_OK;
+}
+
+int check_dims(int32_t ndim, const int32_t* dims) {
+  if (ndim < 1 || ndim > 3) return fail(RHY_ERR_SHAPE, "ndim must be 1, 2 or 3");
+  if (!dims) return fail(RHY_ERR_ARG, "dims is NULL");
+  for (int d = 0; d < 3; ++d) {
+    if (d < ndim && dims[d] < 2) return fail(RHY_ERR_SHAPE, "active extents must be >= 2");
+    if (d >= ndim && dims[d] != 1) return fail(RHY_ERR_SHAPE, "inactive extents must be 1");
+  }
+  int64_t cells = 1;
+  for (int d = 0; d < 3; ++d) cells *= (int64_t)dims[d] + (d < ndim ? 4 : 0);
+  if (cells >= ((int64_t)1 << 31)) return fail(RHY_ERR_SHAPE, "box exceeds 2^31 cells");
+  return RHY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rhy_init(int32_t device) {
+  std::call_once(g_init_once, [device]() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+      g_init_rc = RHY_ERR_NODEV;
+      g_err = "no HIP device";
+      return;
+    }
+    int d = device >= 0 && device < n ? device : 0;
+    if (hipSetDevice(d) != hipSuccess) {
+      g_init_rc = RHY_ERR_NODEV;
+      g_err = "hipSetDevice failed";
+      return;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d) != hipSuccess) {
+      g_init_rc = RHY_ERR_NODEV;
+      g_err = "hipGetDeviceProperties failed";
+      return;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+      g_init_rc = RHY_ERR_NODEV;
+      g_err = std::string("device is ") + prop.gcnArchName + ", library is built for gfx950";
+      return;
+    }
+    g_device = d;
+    g_init_rc = RHY_OK;
+  });
+  if (g_init_rc != RHY_OK && g_err.empty()) g_err = "device initialisation failed";
+  return g_init_rc;
+}
+
+const char* rhy_last_error(void) { return g_err.c_str(); }
+
+int32_t rhy_version(void) { return 1; }
+
+int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t precision,
+                   rhy_box_t** out) {
+  int rc;
+  if (!out) return fail(RHY_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if ((rc = rhy_init(0))) return rc;
+  if ((rc = check_dims(ndim, dims))) return rc;
+  if (ncmp < ndim + 2 || ncmp > RHY_MAX_CMP) return fail(RHY_ERR_ARG, "ncmp out of range");
+  if (precision != RHY_FP64) return fail(RHY_ERR_ARG, "unsupported precision");
+  rhy_box* b = new rhy_box();
+  b->ndim = ndim;
+  for (int d = 0; d < 3; ++d) b->dims[d] = dims[d];
+  b->ncmp = ncmp;
+  b->precision = precision;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete b;
+    return fail(RHY_ERR_HIP, "stream creation failed");
+  }
+  if (hipMalloc(&b->slots, sizeof(unsigned long long) * (4 + 6 * RHY_MAX_CMP)) != hipSuccess ||
+      hipMalloc(&b->dt, sizeof(double)) != hipSuccess) {
+    rhy_box_destroy(b);
+    return fail(RHY_ERR_OOM, "scalar allocation failed");
+  }
+  if ((rc = alloc_box_memory(b))) {
+    rhy_box_destroy(b);
+    return rc;
+  }
+  *out = b;
+  return RHY_OK;
+}
+
+int rhy_box_resize(rhy_box_t* b, const int32_t dims[3]) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_dims(b->ndim, dims))) return rc;
+  bool same = true;
+  for (int d = 0; d < 3; ++d) same = same && dims[d] == b->dims[d];
+  if (same) return RHY_OK;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  free_box_memory(b);
+  for (int d = 0; d < 3; ++d) b->dims[d] = dims[d];
+  return alloc_box_memory(b);
+}
+
+int rhy_box_destroy(rhy_box_t* b) {
+  if (!b) return RHY_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  collect_timings(b);
+  free_box_memory(b);
+  if (b->slots) hipFree(b->slots);
+  if (b->dt) hipFree(b->dt);
+  if (b->dt_log) hipFree(b->dt_log);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return RHY_OK;
+}
+
+int rhy_box_upload(rhy_box_t* b, const double* cells) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
+  HIP_TRY(hipMemcpyAsync(b->cells, cells, sizeof(double) * (size_t)b->geom.plane * b->ncmp,
+                         hipMemcpyHostToDevice, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_box_download(rhy_box_t* b, double* cells) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
+  HIP_TRY(hipMemcpyAsync(cells, b->cells, sizeof(double) * (size_t)b->geom.plane * b->ncmp,
+                         hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_box_sync(rhy_box_t* b) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  collect_timings(b);
+  return RHY_OK;
+}
+
+#define DISPATCH_ND(nd, call)                   \
+  ((nd) == 1 ? call<1> : (nd) == 2 ? call<2> : call<3>)
+
+int rhy_bc_set_boundaries(rhy_box_t* b, const rhy_params_t* p) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (b->ndim == 1) rc = launch_bc<1>(b, p, -1);
+  else if (b->ndim == 2) rc = launch_bc<2>(b, p, -1);
+  else rc = launch_bc<3>(b, p, -1);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_bc_set_face(rhy_box_t* b, const rhy_params_t* p, int32_t face) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (face < 0 || face >= 6) return fail(RHY_ERR_ARG, "face out of range");
+  if (face >= 2 * b->ndim) return RHY_OK;  // no such face at this NDIM
+  if (b->ndim == 1) rc = launch_bc<1>(b, p, face);
+  else if (b->ndim == 2) rc = launch_bc<2>(b, p, face);
+  else rc = launch_bc<3>(b, p, face);
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_cfl_time_step(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double* dt) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (!dx || !dt) return fail(RHY_ERR_ARG, "dx/dt is NULL");
+  const Consts k = make_consts(p);
+  if (b->ndim == 1) rc = launch_cfl<1>(b, k, b->slots + 0);
+  else if (b->ndim == 2) rc = launch_cfl<2>(b, k, b->slots + 0);
+  else rc = launch_cfl<3>(b, k, b->slots + 0);
+  if (rc) return rc;
+  unsigned long long bits = 0;
+  HIP_TRY(hipMemcpyAsync(&bits, b->slots, sizeof(bits), hipMemcpyDeviceToHost, b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  double maxu;
+  std::memcpy(&maxu, &bits, sizeof(maxu));
+  *dt = p->courant * min_dx(b->ndim, dx) / maxu;
+  return RHY_OK;
+}
+
+int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double dt) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (!dx) return fail(RHY_ERR_ARG, "dx is NULL");
+  if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
+    return fail(RHY_ERR_SOLVER, "Unknown solver type");
+  const Consts k = make_consts(p);
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, b->stream, b->dt, dt,
+                     (unsigned long long*)nullptr);
+  const BcTypes bc = make_bc(p);
+  if (b->ndim == 1) {
+    if ((rc = launch_flux<1>(b, k, dx))) return rc;
+    rc = launch_update<1>(b, k, dx, p->solver_type, false, bc, nullptr);
+  } else if (b->ndim == 2) {
+    if ((rc = launch_flux<2>(b, k, dx))) return rc;
+    rc = launch_update<2>(b, k, dx, p->solver_type, false, bc, nullptr);
+  } else {
+    if ((rc = launch_flux<3>(b, k, dx))) return rc;
+    rc = launch_update<3>(b, k, dx, p->solver_type, false, bc, nullptr);
+  }
+  if (rc) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_run_async(rhy_box_t* b, const rhy_params_t* p, const double dx[3], int32_t nsteps) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (!dx) return fail(RHY_ERR_ARG, "dx is NULL");
+  if (nsteps < 0) return fail(RHY_ERR_ARG, "nsteps < 0");
+  if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
+    return fail(RHY_ERR_SOLVER, "Unknown solver type");
+  if (b->ndim == 1) return run_steps<1>(b, p, dx, nsteps);
+  if (b->ndim == 2) return run_steps<2>(b, p, dx, nsteps);
+  return run_steps<3>(b, p, dx, nsteps);
+}
+
+int rhy_run(rhy_box_t* b, const rhy_params_t* p, const double dx[3], int32_t nsteps,
+            double* dts) {
+  int rc = rhy_run_async(b, p, dx, nsteps);
+  if (rc) return rc;
+  if ((rc = rhy_box_sync(b))) return rc;
+  if (dts && nsteps > 0) return rhy_box_last_dts(b, nsteps, dts);
+  return RHY_OK;
+}
+
+int rhy_box_last_dts(rhy_box_t* b, int32_t n, double* dts) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!dts || n < 0 || n > b->last_nsteps) return fail(RHY_ERR_ARG, "bad dts request");
+  if (n == 0) return RHY_OK;
+  HIP_TRY(hipMemcpyAsync(dts, b->dt_log, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost,
+                         b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_box_set_profiling(rhy_box_t* b, int32_t on) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  collect_timings(b);
+  b->profiling = on != 0;
+  for (int i = 0; i < kNumKernelIds; ++i) {
+    b->launches[i] = 0;
+    b->total_ms[i] = 0.0;
+  }
+  return RHY_OK;
+}
+
+int rhy_box_kernel_stats(rhy_box_t* b, int32_t kidx, int64_t* launches, double* total_ms) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (kidx < 0 || kidx >= kNumKernelIds || !launches || !total_ms)
+    return fail(RHY_ERR_ARG, "bad kernel stats request");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  collect_timings(b);
+  *launches = b->launches[kidx];
+  *total_ms = b->total_ms[kidx];
+  return RHY_OK;
+}
+
+// ------------------------------------------------------------ unit batches
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() {
+    if (p) hipFree(p);
+  }
+};
+
+int to_dev(DevBuf& d, const void* h, size_t bytes) {
+  if (hipMalloc(&d.p, bytes ? bytes : 8) != hipSuccess) return fail(RHY_ERR_OOM, "batch alloc");
+  if (h && bytes) HIP_TRY(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice));
+  return RHY_OK;
+}
+
+int batch_prelude(const rhy_params_t* p, int64_t n) {
+  int rc;
+  if ((rc = rhy_init(0))) return rc;
+  if (!p) return fail(RHY_ERR_ARG, "params is NULL");
+  if (p->ndim < 1 || p->ndim > 3) return fail(RHY_ERR_ARG, "ndim");
+  if (n < 0) return fail(RHY_ERR_ARG, "n < 0");
+  return RHY_OK;
+}
+
+}  // namespace
+
+int rhy_riemann_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                      const int32_t* axis, double* u) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  const int nv = p->ndim + 2;
+  const size_t vb = sizeof(double) * (size_t)n * nv;
+  DevBuf dl, dr, da, du;
+  if ((rc = to_dev(dl, l, vb)) || (rc = to_dev(dr, r, vb)) ||
+      (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)) || (rc = to_dev(du, nullptr, vb)))
+    return rc;
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  const double* L = (const double*)dl.p;
+  const double* R = (const double*)dr.p;
+  const int32_t* A = (const int32_t*)da.p;
+  double* U = (double*)du.p;
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(riemann_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(riemann_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+  else
+    hipLaunchKernelGGL(riemann_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(u, du.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_limiter_batch(const rhy_params_t* p, int64_t n, const double* ul, const double* u,
+                      const double* ur, double* delta) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  const int nv = p->ndim + 2;
+  const size_t vb = sizeof(double) * (size_t)n * nv;
+  DevBuf a, b, c, d;
+  if ((rc = to_dev(a, ul, vb)) || (rc = to_dev(b, u, vb)) || (rc = to_dev(c, ur, vb)) ||
+      (rc = to_dev(d, nullptr, vb)))
+    return rc;
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(limiter_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)a.p, (const double*)b.p, (const double*)c.p, (double*)d.p);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(limiter_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)a.p, (const double*)b.p, (const double*)c.p, (double*)d.p);
+  else
+    hipLaunchKernelGGL(limiter_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)a.p, (const double*)b.p, (const double*)c.p, (double*)d.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(delta, d.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_half_step_batch(const rhy_params_t* p, int64_t n, const double* u, const double* delta,
+                        const int32_t* axis, double dx, double dt, double* l, double* r) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  const int nv = p->ndim + 2;
+  const size_t vb = sizeof(double) * (size_t)n * nv;
+  DevBuf du, dd, da, dl, dr;
+  if ((rc = to_dev(du, u, vb)) || (rc = to_dev(dd, delta, vb)) ||
+      (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)) || (rc = to_dev(dl, nullptr, vb)) ||
+      (rc = to_dev(dr, nullptr, vb)))
+    return rc;
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(half_step_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const double*)dd.p, (const int32_t*)da.p, dx, dt,
+                       (double*)dl.p, (double*)dr.p);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(half_step_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const double*)dd.p, (const int32_t*)da.p, dx, dt,
+                       (double*)dl.p, (double*)dr.p);
+  else
+    hipLaunchKernelGGL(half_step_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const double*)dd.p, (const int32_t*)da.p, dx, dt,
+                       (double*)dl.p, (double*)dr.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(l, dl.p, vb, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(r, dr.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_flux_batch(const rhy_params_t* p, int64_t n, const double* u, const int32_t* axis,
+                   double* f) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  const int nv = p->ndim + 2;
+  const size_t vb = sizeof(double) * (size_t)n * nv;
+  DevBuf du, da, df;
+  if ((rc = to_dev(du, u, vb)) || (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)) ||
+      (rc = to_dev(df, nullptr, vb)))
+    return rc;
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(flux_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const int32_t*)da.p, (double*)df.p);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(flux_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const int32_t*)da.p, (double*)df.p);
+  else
+    hipLaunchKernelGGL(flux_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n,
+                       (const double*)du.p, (const int32_t*)da.p, (double*)df.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(f, df.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,427 @@
+// rhy_kernels.hpp -- gfx950 kernels of the hydro step on a device-resident
+// box: per-axis intercell flux, conservative update (with the ghost fill and
+// the CFL reduction of the next step fused into its epilogue), stand-alone
+// ghost fill, CFL reduction, and the unit-level batch kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "rhy_physics.hpp"
+
+namespace rhy {
+
+// Geometry of a ghosted box: Fortran cell (i, j, k), 1-based interior
+// indices, lives at (i-1+g0)*s0 + (j-1+g1)*s1 + (k-1+g2)*s2 inside each
+// component plane; inactive axes have n = 1 and g = 0.
+struct Geom {
+  int32_t n[3];
+  int32_t g[3];
+  int64_t s[3];
+  int64_t plane;
+};
+
+__device__ __forceinline__ int64_t goff(const Geom& G, int i, int j, int k) {
+  return (int64_t)(i - 1 + G.g[0]) * G.s[0] + (int64_t)(j - 1 + G.g[1]) * G.s[1] +
+         (int64_t)(k - 1 + G.g[2]) * G.s[2];
+}
+
+template <int ND>
+__device__ __forceinline__ void load_state(const double* __restrict__ cells, int64_t plane,
+                                           int64_t o, double* u) {
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) u[c] = cells[o + c * plane];
+}
+
+// Boundary types per face (left, right, bottom, top, back, front).
+struct BcTypes {
+  int32_t t[6];
+};
+
+// Writes the ghost images of an updated interior cell, i.e. what
+// rhyme_samr_bc_set_<face>_boundary (rhyme_samr_bc/src/…_left_boundary.f90:47-90
+// and siblings) would write from this cell on the next step's fill.
+// Reflective/outflow mirror cells 1,2 -> ghosts 0,-1 (n, n-1 -> n+1, n+2),
+// periodic wraps n, n-1 -> 0, -1 and 1, 2 -> n+1, n+2; reflective negates the
+// normal momentum; inflow ghosts are constant and left alone.
+template <int ND>
+__device__ __forceinline__ void write_ghost_images(const Geom& G, const BcTypes& bc,
+                                                   double* __restrict__ cells, int i, int j,
+                                                   int k, const double* u) {
+  int idx[3] = {i, j, k};
+#pragma unroll
+  for (int a = 0; a < ND; ++a) {
+    const int n = G.n[a];
+    const int ia = idx[a];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int type = bc.t[2 * a + side];
+      int ghost = 0;
+      bool hit = false;
+      if (type == 1 || type == 2) {
+        if (side == 0) {
+          if (ia == 1) { ghost = 0; hit = true; }
+          else if (ia == 2) { ghost = -1; hit = true; }
+        } else {
+          if (ia == n) { ghost = n + 1; hit = true; }
+          else if (ia == n - 1) { ghost = n + 2; hit = true; }
+        }
+      } else if (type == 3) {
+        if (side == 0) {
+          if (ia == n) { ghost = 0; hit = true; }
+          else if (ia == n - 1) { ghost = -1; hit = true; }
+        } else {
+          if (ia == 1) { ghost = n + 1; hit = true; }
+          else if (ia == 2) { ghost = n + 2; hit = true; }
+        }
+      }
+      if (hit) {
+        int gi[3] = {i, j, k};
+        gi[a] = ghost;
+        int64_t o = goff(G, gi[0], gi[1], gi[2]);
+#pragma unroll
+        for (int c = 0; c < ND + 2; ++c) {
+          double v = u[c];
+          if (type == 1 && c == 1 + a) v = -v;
+          cells[o + c * G.plane] = v;
+        }
+      }
+    }
+  }
+}
+
+// |v| + c_s of one cell: calc_u of rhyme_cfl_time_step.f90:52-59.
+template <int ND>
+__device__ __forceinline__ double cfl_speed(const Consts& k, const double* u) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
+  return sqrt(s) / u[0] + speed_of_sound<ND>(k, u);
+}
+
+// Block-wide max of non-negative values into a 64-bit slot holding a double.
+// max is exact and order independent, so the result is deterministic; NaN
+// speeds are skipped like the reference's strict `u > max_u` test.
+__device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* slot) {
+  v = v > 0.0 ? v : 0.0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0 && v > 0.0)
+    atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+}
+
+// Intercell flux on every face normal to axis A that the interior update
+// reads: face f_A in [0, n_A] sits between cells f_A and f_A+1, the other
+// indices run over the interior (the faces-only range the update consumes,
+// rhyme_muscl_hancock_solve_memory_intensive.f90:142-176).  Per face: the
+// half-step states of the two adjacent cells (loop A, :49-98) and the
+// Riemann flux (loop B, :100-140).  The flux is stored at the low cell's
+// offset in F (NV planes).
+template <int ND, int A>
+__global__ __launch_bounds__(256) void flux_kernel(Consts k, Geom G,
+                                                   const double* __restrict__ cells,
+                                                   double* __restrict__ F,
+                                                   const double* __restrict__ dtp, double dx) {
+  constexpr int NV = ND + 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e0 = A == 0 ? G.n[0] + 1 : G.n[0];
+  const int e1 = A == 1 ? G.n[1] + 1 : G.n[1];
+  const int e2 = A == 2 ? G.n[2] + 1 : G.n[2];
+  if (t >= (int64_t)e0 * e1 * e2) return;
+  int i = (int)(t % e0);
+  int64_t r = t / e0;
+  int j = (int)(r % e1);
+  int kk = (int)(r / e1);
+  // faces index from 0 along A, cells from 1 elsewhere
+  if (A != 0) i += 1;
+  if (A != 1) j += 1;
+  if (A != 2) kk += 1;
+  const int64_t o = goff(G, i, j, kk);
+  const int64_t s = G.s[A];
+  const double dt = *dtp;
+  const double hdtdx = .5 * dt / dx;
+  double um[NV], u0[NV], u1[NV], up[NV];
+  load_state<ND>(cells, G.plane, o - s, um);
+  load_state<ND>(cells, G.plane, o, u0);
+  load_state<ND>(cells, G.plane, o + s, u1);
+  load_state<ND>(cells, G.plane, o + 2 * s, up);
+  double l0[NV], r0[NV], l1[NV], r1[NV], f[NV];
+  half_step<ND, A>(k, um, u0, u1, hdtdx, l0, r0);
+  half_step<ND, A>(k, u0, u1, up, hdtdx, l1, r1);
+  riemann_flux<ND, A>(k, r0, l1, f);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) F[o + c * G.plane] = f[c];
+}
+
+// Conservative update of every interior cell from the stored fluxes.
+// ORDER 2 (memory_intensive, …_solve_memory_intensive.f90:151-176):
+//   u + ((dt/dx_x*(F_x[i-1]-F_x[i]) + dt/dx_y*(...)) + dt/dx_z*(...))
+// ORDER 1 (cpu_intensive, …_solve_cpu_intensive.f90:117-136):
+//   ((u + dt/dx_x*(...)) + dt/dx_y*(...)) + dt/dx_z*(...)
+// FUSED: also writes the new state's ghost images for the next step and
+// folds its CFL speeds into maxu_next.
+template <int ND, int ORDER, bool FUSED>
+__global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* __restrict__ cells,
+                                                     const double* __restrict__ F0,
+                                                     const double* __restrict__ F1,
+                                                     const double* __restrict__ F2,
+                                                     const double* __restrict__ dtp, double dx0,
+                                                     double dx1, double dx2, BcTypes bc,
+                                                     unsigned long long* maxu_next) {
+  constexpr int NV = ND + 2;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  double speed = 0.0;
+  if (t < total) {
+    const int i = (int)(t % G.n[0]) + 1;
+    const int64_t r = t / G.n[0];
+    const int j = (int)(r % G.n[1]) + 1;
+    const int kk = (int)(r / G.n[1]) + 1;
+    const int64_t o = goff(G, i, j, kk);
+    const double dt = *dtp;
+    const double dtdx0 = dt / dx0;
+    const double dtdx1 = ND > 1 ? dt / dx1 : 0.0;
+    const double dtdx2 = ND > 2 ? dt / dx2 : 0.0;
+    double u[NV];
+    load_state<ND>(cells, G.plane, o, u);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const int64_t oc = o + c * G.plane;
+      double a0 = dtdx0 * (F0[oc - G.s[0]] - F0[oc]);
+      if (ORDER == 2) {
+        double acc = a0;
+        if (ND > 1) acc = acc + dtdx1 * (F1[oc - G.s[1]] - F1[oc]);
+        if (ND > 2) acc = acc + dtdx2 * (F2[oc - G.s[2]] - F2[oc]);
+        u[c] = u[c] + acc;
+      } else {
+        u[c] = u[c] + a0;
+        if (ND > 1) u[c] = u[c] + dtdx1 * (F1[oc - G.s[1]] - F1[oc]);
+        if (ND > 2) u[c] = u[c] + dtdx2 * (F2[oc - G.s[2]] - F2[oc]);
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < NV; ++c) cells[o + c * G.plane] = u[c];
+    if (FUSED) {
+      write_ghost_images<ND>(G, bc, cells, i, j, kk, u);
+      speed = cfl_speed<ND>(k, u);
+    }
+  }
+  if (FUSED) block_max_to_slot(speed, maxu_next);
+}
+
+// Stand-alone ghost fill, one thread per face cell; blockIdx.y = face
+// (rhyme_samr_bc_set_boundaries.f90:9-20 calls the faces in this order; the
+// faces are disjoint and read interior cells only, so they run together).
+template <int ND>
+__global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const double* __restrict__ inflow,
+                                                 int32_t ncmp, double* __restrict__ cells,
+                                                 int32_t face_only) {
+  const int face = face_only >= 0 ? face_only : (int)blockIdx.y;
+  const int a = face / 2, side = face % 2;
+  if (a >= ND) return;
+  const int o1 = a == 0 ? 1 : 0;
+  const int o2 = a == 2 ? 1 : 2;
+  const int n1 = G.n[o1], n2 = G.n[o2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)n1 * n2) return;
+  int idx[3];
+  idx[o1] = (int)(t % n1) + 1;
+  idx[o2] = (int)(t / n1) + 1;
+  const int n = G.n[a];
+  const int type = bc.t[face];
+  const int g0 = side ? n + 1 : 0, g1 = side ? n + 2 : -1;
+  int gi[3] = {idx[0], idx[1], idx[2]};
+  gi[a] = g0;
+  const int64_t og0 = goff(G, gi[0], gi[1], gi[2]);
+  gi[a] = g1;
+  const int64_t og1 = goff(G, gi[0], gi[1], gi[2]);
+  if (type == 4) {
+    for (int c = 0; c < ncmp; ++c) {
+      cells[og0 + c * G.plane] = inflow[face * 16 + c];
+      cells[og1 + c * G.plane] = inflow[face * 16 + c];
+    }
+    return;
+  }
+  if (type < 1 || type > 3) return;
+  int s0, s1;
+  if (type == 3) {
+    s0 = side ? 1 : n;
+    s1 = side ? 2 : n - 1;
+  } else {
+    s0 = side ? n : 1;
+    s1 = side ? n - 1 : 2;
+  }
+  gi[a] = s0;
+  const int64_t os0 = goff(G, gi[0], gi[1], gi[2]);
+  gi[a] = s1;
+  const int64_t os1 = goff(G, gi[0], gi[1], gi[2]);
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) {
+    double v0 = cells[os0 + c * G.plane];
+    double v1 = cells[os1 + c * G.plane];
+    if (type == 1 && c == 1 + a) {
+      v0 = -v0;
+      v1 = -v1;
+    }
+    cells[og0 + c * G.plane] = v0;
+    cells[og1 + c * G.plane] = v1;
+  }
+}
+
+// max over the interior of |v| + c_s (rhyme_cfl_time_step.f90:40-47).
+template <int ND>
+__global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const double* __restrict__ cells,
+                                                  unsigned long long* slot) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  double v = 0.0;
+  if (t < total) {
+    const int i = (int)(t % G.n[0]) + 1;
+    const int64_t r = t / G.n[0];
+    const int j = (int)(r % G.n[1]) + 1;
+    const int kk = (int)(r / G.n[1]) + 1;
+    double u[ND + 2];
+    load_state<ND>(cells, G.plane, goff(G, i, j, kk), u);
+    v = cfl_speed<ND>(k, u);
+  }
+  block_max_to_slot(v, slot);
+}
+
+// dt of the step: courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the
+// product courant*minval(dx) is formed on the host.  Also clears the slot
+// the coming update will fill and logs dt.
+__global__ void dt_kernel(double cmdx, const unsigned long long* slot_cur,
+                          unsigned long long* slot_next, double* dt_out, double* dt_log) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double maxu = __longlong_as_double((long long)*slot_cur);
+    double dt = cmdx / maxu;
+    *dt_out = dt;
+    if (dt_log) *dt_log = dt;
+    if (slot_next) *slot_next = 0ull;
+  }
+}
+
+__global__ void set_scalar_kernel(double* dst, double v, unsigned long long* slot_clear) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (dst) *dst = v;
+    if (slot_clear) *slot_clear = 0ull;
+  }
+}
+
+// ------------------------------------------------------ unit batches ----
+
+template <int ND>
+__global__ __launch_bounds__(256) void riemann_batch_kernel(Consts k, int64_t n,
+                                                            const double* __restrict__ l,
+                                                            const double* __restrict__ r,
+                                                            const int32_t* __restrict__ axis,
+                                                            double* __restrict__ u) {
+  constexpr int NV = ND + 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double L[NV], R[NV], U[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    L[c] = l[q * NV + c];
+    R[c] = r[q * NV + c];
+  }
+  const int a = axis[q];
+  if (a == 1) riemann_state<ND, 0>(k, L, R, U);
+  else if (ND > 1 && a == 2) riemann_state<ND, (ND > 1 ? 1 : 0)>(k, L, R, U);
+  else if (ND > 2 && a == 3) riemann_state<ND, (ND > 2 ? 2 : 0)>(k, L, R, U);
+  else {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) U[c] = __builtin_nan("");
+  }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) u[q * NV + c] = U[c];
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void limiter_batch_kernel(Consts k, int64_t n,
+                                                            const double* __restrict__ ul,
+                                                            const double* __restrict__ u,
+                                                            const double* __restrict__ ur,
+                                                            double* __restrict__ d) {
+  constexpr int NV = ND + 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+#pragma unroll
+  for (int c = 0; c < NV; ++c)
+    d[q * NV + c] = limit_one(k.limiter, k.omega, ul[q * NV + c], u[q * NV + c], ur[q * NV + c]);
+}
+
+// half step from a given slope (the reference routine takes delta as input)
+template <int ND, int A>
+__device__ __forceinline__ void half_step_given(const Consts& k, const double* u,
+                                                const double* delta, double hdtdx, double* l,
+                                                double* r) {
+  constexpr int NV = ND + 2;
+  double fl[NV], fr[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    l[c] = u[c] - .5 * delta[c];
+    r[c] = u[c] + .5 * delta[c];
+  }
+  flux<ND, A>(k, l, fl);
+  flux<ND, A>(k, r, fr);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double df = fl[c] - fr[c];
+    l[c] = l[c] + hdtdx * df;
+    r[c] = r[c] + hdtdx * df;
+  }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void half_step_batch_kernel(Consts k, int64_t n,
+                                                              const double* __restrict__ u,
+                                                              const double* __restrict__ delta,
+                                                              const int32_t* __restrict__ axis,
+                                                              double dx, double dt,
+                                                              double* __restrict__ l,
+                                                              double* __restrict__ r) {
+  constexpr int NV = ND + 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double U[NV], D[NV], Lo[NV], Ro[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    U[c] = u[q * NV + c];
+    D[c] = delta[q * NV + c];
+  }
+  const double hdtdx = .5 * dt / dx;
+  const int a = axis[q];
+  if (a == 1) half_step_given<ND, 0>(k, U, D, hdtdx, Lo, Ro);
+  else if (ND > 1 && a == 2) half_step_given<ND, (ND > 1 ? 1 : 0)>(k, U, D, hdtdx, Lo, Ro);
+  else half_step_given<ND, (ND > 2 ? 2 : 0)>(k, U, D, hdtdx, Lo, Ro);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    l[q * NV + c] = Lo[c];
+    r[q * NV + c] = Ro[c];
+  }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void flux_batch_kernel(Consts k, int64_t n,
+                                                         const double* __restrict__ u,
+                                                         const int32_t* __restrict__ axis,
+                                                         double* __restrict__ f) {
+  constexpr int NV = ND + 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double U[NV], Fo[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) U[c] = u[q * NV + c];
+  const int a = axis[q];
+  if (a == 1) flux<ND, 0>(k, U, Fo);
+  else if (ND > 1 && a == 2) flux<ND, (ND > 1 ? 1 : 0)>(k, U, Fo);
+  else flux<ND, (ND > 2 ? 2 : 0)>(k, U, Fo);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) f[q * NV + c] = Fo[c];
+}
+
+}  // namespace rhy

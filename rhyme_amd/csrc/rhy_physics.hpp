@@ -1,0 +1,444 @@
+// rhy_physics.hpp -- device-side ideal-gas hydro physics of the hot path:
+// EOS and fluxes, slope limiters, half-step extrapolation and the exact
+// iterative Riemann solver, all evaluated per lane in registers.
+//
+// Arithmetic order is part of the contract: the file must be compiled with
+// -ffp-contract=off, and each expression keeps the parse of the reference
+// Fortran routine it restates (cited file:line, relative to the reference's
+// src/), so that FP64 results agree with the reference CPU path to rounding
+// of pow() only.  Division and sqrt of doubles are IEEE correctly rounded on
+// gfx950 under the default (non fast-math) lowering.
+//
+// Template parameter ND is the box dimensionality (1, 2, 3); a state vector
+// holds NV = ND + 2 conserved values: rho, rho*v[0..ND), e_tot.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rhy {
+
+constexpr double kTiny = 2.2250738585072014e-308;  // tiny(0d0)
+
+// Problem constants that the reference keeps in module-private globals
+// (rhyme_riemann_problem.f90:11-16, rhyme_irs.f90:9-14,
+// rhyme_thermo_base.f90:25-26) and derived-type defaults.
+struct Consts {
+  double g, gm1, gp1, gm1_gp1, gm1_2g, gp1_2g, g_inv;  // gamma and derived
+  double kb_amu;
+  double omega;          // sl%w
+  double tol;            // irs%tolerance
+  double w_rho, w_p;     // rp%w_vacuum(rho), rp%w_vacuum(p)
+  double e_rho;          // nearest(w_rho, +1), computed on the host
+  int32_t limiter;       // slid
+  int32_t n_iter;        // irs%n_iteration
+};
+
+template <int ND>
+struct State {
+  double v[ND + 2];
+};
+
+// ---------------------------------------------------------------- EOS ----
+
+// rhyme_hydro_base_specific_kinetic_energy.f90:9
+template <int ND>
+__device__ __forceinline__ double sp_kin_e(const double* u) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
+  return .5 * s / (u[0] * u[0]);
+}
+
+// rhyme_hydro_base_specific_internal_energy.f90:9
+template <int ND>
+__device__ __forceinline__ double sp_int_e(const double* u) {
+  return u[ND + 1] / u[0] - sp_kin_e<ND>(u);
+}
+
+// rhyme_ideal_gas_pressure.f90:9
+template <int ND>
+__device__ __forceinline__ double pressure(const Consts& k, const double* u) {
+  return u[0] * sp_int_e<ND>(u) * (k.g - 1.0);
+}
+
+// rhyme_ideal_gas_speed_of_sound.f90:9 with temperature_per_mu.f90:9-10
+template <int ND>
+__device__ __forceinline__ double speed_of_sound(const Consts& k, const double* u) {
+  double t_mu = sp_int_e<ND>(u) * (k.g - 1.0) / k.kb_amu;
+  return sqrt(k.g * k.kb_amu * t_mu);
+}
+
+// rhyme_ideal_gas_flux.f90:12-23; A is the 0-based axis.
+template <int ND, int A>
+__device__ __forceinline__ void flux(const Consts& k, const double* u, double* f) {
+  if (fabs(u[0]) < kTiny) {
+#pragma unroll
+    for (int c = 0; c < ND + 2; ++c) f[c] = 0.0;
+    return;
+  }
+  double p = pressure<ND>(k, u);
+  f[0] = u[1 + A];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) f[1 + d] = u[1 + d] * f[0] / u[0];
+  f[1 + A] = f[1 + A] + p;
+  f[ND + 1] = f[0] / u[0] * (u[ND + 1] + p);
+}
+
+// rhyme_ideal_gas_primitive_vars_to_conserved.f90:9-11
+template <int ND>
+__device__ __forceinline__ void prim_to_cons(const Consts& k, double rho, const double* v,
+                                             double p, double* u) {
+  double s = 0.0;
+  u[0] = rho;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[1 + d] = rho * v[d];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) s += v[d] * v[d];
+  u[ND + 1] = .5 * rho * s + p / (k.g - 1.0);
+}
+
+// ------------------------------------------------------- slope limiter ----
+
+__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+
+// rhyme_slope_limiter_run.f90:10-19 and the four limiters
+// (…_van_leer.f90:13-27, …_minmod.f90:13-29, …_van_albada.f90:13-27,
+// …_superbee.f90:13-31), r from rhyme_slope_limiter.f90:62-82 and
+// xi_R from …_xi_r.f90:11.
+__device__ __forceinline__ double limit_one(int type, double w, double ul, double u, double ur) {
+  double dL = u - ul;
+  double dR = ur - u;
+  double d = .5 * (1.0 + w) * dL + .5 * (1.0 + w) * dR;
+  double denom = ur - u;
+  double r = fabs(denom) < kTiny ? 0.0 : (u - ul) / denom;
+  if (!(r > 0.0)) return 0.0;  // every limiter gives 0 for r <= 0
+  double xi = 2.0 / (1.0 - w + (1.0 + w) * r);
+  switch (type) {
+    case 1:  // van Leer
+      return dmin(2.0 * r / (1.0 + r), xi) * d;
+    case 2:  // minmod
+      return r > 1.0 ? dmin(1.0, xi) * d : r * d;
+    case 3:  // van Albada
+      return dmin(r * (1.0 + r) / (1.0 + r * r), xi) * d;
+    case 4:  // superbee
+      if (r > 1.0) return dmin(dmin(r, xi), 2.0) * d;
+      if (r > .5) return d;
+      return 2.0 * r * d;
+    default:
+      return 0.0;
+  }
+}
+
+// rhyme_muscl_hancock_half_step_extrapolation.f90:15-21: slope from the
+// three cells um, u, up along axis A, then the half-step states l and r.
+template <int ND, int A>
+__device__ __forceinline__ void half_step(const Consts& k, const double* um, const double* u,
+                                          const double* up, double hdtdx, double* l, double* r) {
+  constexpr int NV = ND + 2;
+  double fl[NV], fr[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double delta = limit_one(k.limiter, k.omega, um[c], u[c], up[c]);
+    l[c] = u[c] - .5 * delta;
+    r[c] = u[c] + .5 * delta;
+  }
+  flux<ND, A>(k, l, fl);
+  flux<ND, A>(k, r, fr);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) {
+    double df = fl[c] - fr[c];
+    l[c] = l[c] + hdtdx * df;  // hdtdx = (.5d0*dt)/dx, evaluated once
+    r[c] = r[c] + hdtdx * df;
+  }
+}
+
+// -------------------------------------------------- Riemann (IRS) ----
+
+// rhyme_irs_nonlinear_wave_function.f90:13-23
+__device__ __forceinline__ void wave_fn(const Consts& k, double rho, double pk, double cs,
+                                        double p, double& f, double& fp) {
+  if (p > pk) {
+    double Ak = 2.0 / (k.gp1 * rho);
+    double Bk = k.gm1 * pk / k.gp1;
+    double factor = sqrt(Ak / (Bk + p));
+    f = (p - pk) * factor;
+    fp = factor * (1.0 - (p - pk) / (2.0 * (Bk + p)));
+  } else {
+    f = 2.0 * cs / k.gm1 * (pow(p / pk, k.gm1_2g) - 1.0);
+    fp = 1.0 / (rho * cs) * pow(p / pk, -k.gp1_2g);
+  }
+}
+
+// rhyme_irs_guess_p_star.f90 guess 1 (:14-39): PVRS / two-rarefaction /
+// two-shock selection.
+__device__ __forceinline__ double guess1(const Consts& k, double lr, double lv, double lp,
+                                         double lc, double rr, double rv, double rp, double rc) {
+  double cup = 0.25 * (lr + rr) * (lc + rc);
+  double ppv = dmax(0.5 * (lp + rp) + 0.5 * (lv - rv) * cup, 0.0);
+  double p_min = dmin(lp, rp);
+  double p_max = dmax(lp, rp);
+  double q_max = p_max / p_min;
+  double g;
+  if (q_max <= 2.0 && (p_min < ppv && ppv < p_max)) {
+    g = ppv;
+  } else if (ppv < p_min) {
+    double pq = pow(lp / rp, k.gm1_2g);
+    double um = (pq * lv / lc + rv / rc + 2.0 / k.gm1 * (pq - 1.0)) / (pq / lc + 1.0 / rc);
+    double ptl = 1.0 + k.gm1 / 2.0 * (lv - um) / lc;
+    double ptr = 1.0 + k.gm1 / 2.0 * (um - rv) / rc;
+    g = 0.5 * (lp * pow(ptl, 1.0 / k.gm1_2g) + rp * pow(ptr, 1.0 / k.gm1_2g));
+  } else {
+    double gel = sqrt(((2.0 / (k.gp1)) / lr) / (k.gm1_gp1 * lp + ppv));
+    double ger = sqrt(((2.0 / (k.gp1)) / rr) / (k.gm1_gp1 * rp + ppv));
+    g = (gel * lp + ger * rp - (rv - lv)) / (gel + ger);
+  }
+  return dmax(g, kTiny);
+}
+
+// Guesses 2..6 (rhyme_irs_guess_p_star.f90:42-71).  The reference always
+// computes all six; they are only read when the Newton iteration from the
+// previous guess ends at p* <= 0 (rhyme_irs_exec.f90:24-42), so computing
+// guess i on demand yields the same values.
+__device__ __noinline__ double guess_n(const Consts& k, int i, double lr, double lv, double lp,
+                                       double lc, double rr, double rv, double rp, double rc) {
+  double g;
+  switch (i) {
+    case 1:
+      g = (rr * rc * lp + lr * lc * rp + rc * lc * (lr * lv - rr * rv)) / (rr * rc + lr * lc);
+      break;
+    case 2:
+      g = pow((lc + rc - .5 * k.gm1 * (rv - lv)) /
+                  (pow(lc / lp, k.gm1_2g) + pow(rc / rp, k.gm1_2g)),
+              1.0 / k.gm1_2g);
+      break;
+    case 3: {
+      double lg = sqrt((2.0 / (k.gp1 * lr)) / (lp + k.gm1_gp1 * lp));
+      double rg = sqrt((2.0 / (k.gp1 * rr)) / (rp + k.gm1_gp1 * rp));
+      g = (lg * lp + rg * rp - (rv - lv)) / (lg + rg);
+      break;
+    }
+    case 4:
+      g = .5 * (lp + rp) - .125 * (rv - lv) * (lr + rr) * (lc + rc);
+      break;
+    default:
+      g = .5 * (lp + rp);
+      break;
+  }
+  return dmax(g, kTiny);
+}
+
+// Newton iteration from one starting guess (rhyme_irs_exec.f90:25-39).
+// Leaves lf/rf at the wave-function values of the last evaluated iterate.
+__device__ __forceinline__ double newton(const Consts& k, double p_star, double lr, double lp,
+                                         double lc, double rr, double rp, double rc, double du,
+                                         double& lf, double& rf) {
+  double p_prev = p_star;
+  for (int i = 1; i <= k.n_iter; ++i) {
+    double lfp, rfp;
+    wave_fn(k, lr, lp, lc, p_star, lf, lfp);
+    wave_fn(k, rr, rp, rc, p_star, rf, rfp);
+    p_star = p_prev - (lf + rf + du) / (lfp + rfp);
+    if (p_star < 0.0) break;
+    if (2.0 * fabs(p_star - p_prev) / (p_star + p_prev) < k.tol) break;
+    p_prev = p_star;
+  }
+  return p_star;
+}
+
+// rhyme_irs_exec.f90:3-45
+__device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv, double lp,
+                                           double lc, double rr, double rv, double rp, double rc,
+                                           double& lf, double& rf) {
+  const double du = rv - lv;
+  double p = newton(k, guess1(k, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp, rc, du,
+                    lf, rf);
+  if (!(p > 0.0)) {
+    for (int gi = 1; gi < 6; ++gi) {
+      p = newton(k, guess_n(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp, rc, du,
+                 lf, rf);
+      if (p > 0.0) break;
+    }
+  }
+  if (p < 0.0) p = 0.0;
+  return p;
+}
+
+// rhyme_riemann_problem_w_kfan.f90:14-31 (is_right selects the sign of cs)
+template <int ND, int A>
+__device__ __forceinline__ void w_kfan(const Consts& k, double srho, const double* sv, double sp,
+                                       double scs, double dxdt, bool is_right, double* u) {
+  double cs = is_right ? -scs : scs;
+  double v[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = sv[d];
+  double rho = srho * pow(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1);
+  v[A] = 2.0 / k.gp1 * (cs + k.gm1 / 2.0 * sv[A] + dxdt);
+  double p = sp * pow(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 1.0 / k.gm1_2g);
+  prim_to_cons<ND>(k, rho, v, p, u);
+}
+
+// conv_prim_to_cons(rp%w_vacuum, u): velocities of w_vacuum are 0.
+template <int ND>
+__device__ __forceinline__ void vacuum_state(const Consts& k, double* u) {
+  double v[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = 0.0;
+  prim_to_cons<ND>(k, k.w_rho, v, k.w_p, u);
+}
+
+// rhyme_riemann_problem_solve.f90:3-104 at dx = 0 (Godunov state, x/t = 0):
+// the conserved state u sampled on the interface.  l is the right half-step
+// state of the cell on the low side of the interface, r the left half-step
+// state of the cell on the high side.
+template <int ND, int A>
+__device__ __forceinline__ void riemann_state(const Consts& k, const double* l, const double* r,
+                                              double* u) {
+  const double dxdt = 0.0;  // dx/dt with dx = 0 and dt > 0
+  double l_rho = dmax(l[0], k.w_rho);
+  double r_rho = dmax(r[0], k.w_rho);
+  const double e_rho = k.e_rho;
+  bool vac_r = r_rho < e_rho && l_rho > e_rho;
+  bool vac_l = l_rho < e_rho && r_rho > e_rho;
+  bool vac_b = l_rho < e_rho && r_rho < e_rho;
+  double lv[ND], rv[ND];
+
+  if (vac_r) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) lv[d] = l[1 + d] / l[0];
+    double lp = dmax(pressure<ND>(k, l), k.w_p);
+    double lc = dmax(speed_of_sound<ND>(k, l), kTiny);
+    double s_starl = lv[A] - 2 * lc / k.gm1;
+    if (dxdt > s_starl)
+      vacuum_state<ND>(k, u);
+    else if (dxdt > lv[A] - lc)
+      w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+    else
+      prim_to_cons<ND>(k, l_rho, lv, lp, u);
+  } else if (vac_l) {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) rv[d] = r[1 + d] / r[0];
+    double rp = dmax(pressure<ND>(k, r), k.w_p);
+    double rc = dmax(speed_of_sound<ND>(k, r), kTiny);
+    double s_starr = rv[A] + 2 * rc / k.gm1;
+    if (dxdt > rv[A] + rc)
+      prim_to_cons<ND>(k, r_rho, rv, rp, u);
+    else if (dxdt > s_starr)
+      w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+    else
+      vacuum_state<ND>(k, u);
+  } else if (vac_b) {
+    vacuum_state<ND>(k, u);
+  } else {
+#pragma unroll
+    for (int d = 0; d < ND; ++d) lv[d] = l[1 + d] / l[0];
+    double lp = dmax(pressure<ND>(k, l), k.w_p);
+    double lc = dmax(speed_of_sound<ND>(k, l), kTiny);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) rv[d] = r[1 + d] / r[0];
+    double rp = dmax(pressure<ND>(k, r), k.w_p);
+    double rc = dmax(speed_of_sound<ND>(k, r), kTiny);
+    double s_starl = lv[A] - 2 * lc / k.gm1;
+    double s_starr = rv[A] + 2 * rc / k.gm1;
+    double du = rv[A] - lv[A];
+    double du_crit = 2 * (lc + rc) / k.gm1;
+    if (du_crit < du) {  // generation of vacuum (:83-98)
+      if (dxdt > s_starr) {
+        if (dxdt > rv[A] + rc)
+          prim_to_cons<ND>(k, r_rho, rv, rp, u);
+        else
+          w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+      } else if (dxdt > s_starl) {
+        vacuum_state<ND>(k, u);
+      } else {
+        if (dxdt > lv[A] - lc)
+          w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+        else
+          prim_to_cons<ND>(k, l_rho, lv, lp, u);
+      }
+    } else {
+      // star state: rhyme_riemann_problem_star.f90:3-66
+      double lf = 0.0, rf = 0.0;
+      double ps = irs_exec(k, l_rho, lv[A], lp, lc, r_rho, rv[A], rp, rc, lf, rf);
+      if (ps < k.w_p) ps = k.w_p;
+      double us = 0.5 * ((rv[A] + lv[A]) + (rf - lf));
+      // sampling at x/t = 0: rhyme_riemann_problem_sampling.f90:13-47
+      if (dxdt > us) {  // right of the contact
+        double ps_pr = ps / rp;
+        if (ps > rp) {
+          double speed = rv[A] + rc * sqrt(k.gp1_2g * ps_pr + k.gm1_2g);
+          if (dxdt < speed) {
+            double rho = r_rho * (k.gm1_gp1 + ps_pr) / (k.gm1_gp1 * ps_pr + 1.0);
+            double v[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) v[d] = rv[d];
+            v[A] = us;
+            prim_to_cons<ND>(k, rho, v, ps, u);
+          } else {
+            prim_to_cons<ND>(k, r_rho, rv, rp, u);
+          }
+        } else {
+          double fan_cs = rc * pow(ps_pr, k.gm1_2g);
+          double speedH = rv[A] + rc;
+          double speedT = us + fan_cs;
+          if (dxdt < speedT) {
+            double rho = r_rho * pow(ps_pr, k.g_inv);
+            double v[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) v[d] = rv[d];
+            v[A] = us;
+            prim_to_cons<ND>(k, rho, v, ps, u);
+          } else if (dxdt < speedH) {
+            w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+          } else {
+            prim_to_cons<ND>(k, r_rho, rv, rp, u);
+          }
+        }
+      } else {  // left of the contact
+        double ps_pl = ps / lp;
+        if (ps > lp) {
+          double speed = lv[A] - lc * sqrt(k.gp1_2g * ps_pl + k.gm1_2g);
+          if (dxdt > speed) {
+            double rho = l_rho * (k.gm1_gp1 + ps_pl) / (k.gm1_gp1 * ps_pl + 1.0);
+            double v[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) v[d] = lv[d];
+            v[A] = us;
+            prim_to_cons<ND>(k, rho, v, ps, u);
+          } else {
+            prim_to_cons<ND>(k, l_rho, lv, lp, u);
+          }
+        } else {
+          double fan_cs = lc * pow(ps_pl, k.gm1_2g);
+          double speedH = lv[A] - lc;
+          double speedT = us - fan_cs;
+          if (dxdt > speedT) {
+            double rho = l_rho * pow(ps_pl, k.g_inv);
+            double v[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) v[d] = lv[d];
+            v[A] = us;
+            prim_to_cons<ND>(k, rho, v, ps, u);
+          } else if (dxdt > speedH) {
+            w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+          } else {
+            prim_to_cons<ND>(k, l_rho, lv, lp, u);
+          }
+        }
+      }
+    }
+  }
+}
+
+// Riemann solve + calc_flux of the sampled state
+// (…_solve_memory_intensive.f90:110-116): the intercell flux.
+template <int ND, int A>
+__device__ __forceinline__ void riemann_flux(const Consts& k, const double* l, const double* r,
+                                             double* F) {
+  double u[ND + 2];
+  riemann_state<ND, A>(k, l, r, u);
+  flux<ND, A>(k, u, F);
+}
+
+}  // namespace rhy

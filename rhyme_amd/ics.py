@@ -1,0 +1,132 @@
+"""Seeded initial conditions of the benchmark workloads, in the reference's
+box layout (host arrays of box_shape(ndim, dims, ncmp), ghosts zero as after
+rhyme_samr_bc_init, rhyme_samr_bc/src/rhyme_samr_bc_init.f90:61-87).
+
+The generator is numpy's counter-based Philox; its arrays are the *inputs*
+to both the device path and the CPU oracle, so the choice of generator cannot
+affect parity.  The non-hydro components (temp, ntr_frac_*) are left zero:
+the hot path never reads them.
+"""
+import numpy as np
+
+from .hydro import box_shape, ncmp_for
+
+
+def rng(seed):
+    return np.random.Generator(np.random.Philox(int(seed)))
+
+
+def prim_to_cons(gamma, rho, v, p):
+    """conv_prim_vars_to_cons (rhyme_ideal_gas_primitive_vars_to_conserved.f90:9-11)
+    on arrays: v has shape (ndim, ...)."""
+    s = np.zeros_like(rho)
+    for d in range(v.shape[0]):
+        s = s + v[d] * v[d]
+    out = [rho]
+    for d in range(v.shape[0]):
+        out.append(rho * v[d])
+    out.append(.5 * rho * s + p / (gamma - 1.0))
+    return out
+
+
+def _fill(ndim, dims, ncmp, gamma, rho, v, p):
+    cells = np.zeros(box_shape(ndim, dims, ncmp), dtype=np.float64)
+    cons = prim_to_cons(gamma, rho, v, p)
+    sl = tuple(slice(2, 2 + n) for n in list(dims)[::-1])
+    for c, arr in enumerate(cons):
+        cells[(c,) + sl] = arr
+    return cells
+
+
+def cell_centres(dims):
+    """x_d = (i_d - 1/2) / n_d on [0, 1] per axis, arrays in C order (z, y, x)."""
+    axes = [(np.arange(n) + 0.5) / n for n in dims]
+    grids = np.meshgrid(*axes[::-1], indexing="ij")
+    return grids[::-1]  # x, y, z
+
+
+def two_state_1d(n=1024, seed=0, gamma=5.0 / 3.0, ncmp=None):
+    """Config 1: discontinuity at x = 0.5 (between cells n/2 and n/2+1), u = 0,
+    rho and p on each side drawn from U[0.1, 1]."""
+    g = rng(seed)
+    rl, pl, rr, pr = g.uniform(0.1, 1.0, size=4)
+    ncmp = ncmp or ncmp_for(1)
+    rho = np.where(np.arange(n) < n // 2, rl, rr).astype(np.float64)
+    p = np.where(np.arange(n) < n // 2, pl, pr).astype(np.float64)
+    v = np.zeros((1, n))
+    return _fill(1, [n], ncmp, gamma, rho, v, p), dict(rho_l=rl, p_l=pl, rho_r=rr, p_r=pr)
+
+
+def spheres_2d(n=1024, seed=0, gamma=5.0 / 3.0, nspheres=16, amp=0.05, lam=32.0, ncmp=None):
+    """Config 2: canvas rho = p = 1 at rest; harmonic perturbation of p along x
+    with amplitude amp and wavelength lam cells; then nspheres sharp discs at
+    seeded centres, radius U[8, 32] cells, rho and p U[0.1, 10]; later discs
+    overwrite earlier ones."""
+    g = rng(seed)
+    ncmp = ncmp or ncmp_for(2)
+    i = np.arange(1, n + 1, dtype=np.float64)
+    x = np.broadcast_to(i[None, :], (n, n))
+    y = np.broadcast_to(i[:, None], (n, n))
+    rho = np.ones((n, n))
+    p = np.ones((n, n)) + amp * np.cos(2.0 * np.pi * (x - 0.5) / lam)
+    for _ in range(nspheres):
+        cx, cy = g.uniform(1, n, size=2)
+        rad = g.uniform(8, 32)
+        r_s, p_s = g.uniform(0.1, 10.0, size=2)
+        m = (x - cx) ** 2 + (y - cy) ** 2 <= rad * rad
+        rho[m] = r_s
+        p[m] = p_s
+    v = np.zeros((2, n, n))
+    return _fill(2, [n, n], ncmp, gamma, rho, v, p)
+
+
+def blast_3d(n=256, seed=0, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, dims=None,
+             ncmp=None):
+    """Config 3: rho = 1 + jitter*U[-1, 1], p = p0, at rest; unit energy
+    deposited as pressure in the cells whose centre lies within `radius`
+    cells of the box centre: p = (gamma-1) * 1 / (N_in * dx^3)."""
+    dims = list(dims) if dims is not None else [n, n, n]
+    g = rng(seed)
+    ncmp = ncmp or ncmp_for(3)
+    nz, ny, nx = dims[2], dims[1], dims[0]
+    rho = 1.0 + jitter * g.uniform(-1.0, 1.0, size=(nz, ny, nx))
+    k = np.arange(nz)[:, None, None] + 0.5 - nz / 2.0
+    j = np.arange(ny)[None, :, None] + 0.5 - ny / 2.0
+    i = np.arange(nx)[None, None, :] + 0.5 - nx / 2.0
+    inside = (i * i + j * j + k * k) <= radius * radius
+    n_in = int(inside.sum())
+    dx = 1.0 / max(dims)
+    p = np.full((nz, ny, nx), p0)
+    p[inside] = (gamma - 1.0) * 1.0 / (n_in * dx ** 3)
+    v = np.zeros((3, nz, ny, nx))
+    return _fill(3, dims, ncmp, gamma, rho, v, p)
+
+
+def random_two_state(ndim, dims, seed=0, gamma=5.0 / 3.0, ncmp=None, vmax=0.0):
+    """Seeded two-state problem along a random axis: rho, p ~ U[0.1, 1] per side,
+    optional velocities U[-vmax, vmax] per side."""
+    g = rng(seed)
+    ncmp = ncmp or ncmp_for(ndim)
+    axis = int(g.integers(0, ndim))
+    rl, pl, rr, pr = g.uniform(0.1, 1.0, size=4)
+    vl = g.uniform(-vmax, vmax, size=ndim) if vmax else np.zeros(ndim)
+    vr = g.uniform(-vmax, vmax, size=ndim) if vmax else np.zeros(ndim)
+    shape = list(dims)[::-1]
+    idx = np.indices(shape)[::-1][axis]  # index along `axis`
+    left = idx < dims[axis] // 2
+    rho = np.where(left, rl, rr).astype(np.float64)
+    p = np.where(left, pl, pr).astype(np.float64)
+    v = np.stack([np.where(left, vl[d], vr[d]) for d in range(ndim)]).astype(np.float64)
+    return _fill(ndim, dims, ncmp, gamma, rho, v, p)
+
+
+def random_smooth(ndim, dims, seed=0, gamma=5.0 / 3.0, ncmp=None, amp=0.3, vamp=0.5):
+    """Seeded smooth-plus-noise state exercising every branch of the limiters
+    and both shock and rarefaction waves (used by the parity tests)."""
+    g = rng(seed)
+    ncmp = ncmp or ncmp_for(ndim)
+    shape = list(dims)[::-1]
+    rho = 1.0 + amp * g.uniform(-1, 1, size=shape)
+    p = 1.0 + amp * g.uniform(-1, 1, size=shape)
+    v = vamp * g.uniform(-1, 1, size=[ndim] + shape)
+    return _fill(ndim, dims, ncmp, gamma, rho, v, p)
