@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the hydro step at 3D 256^3 on one MI355X.
+
+Workload (the 3D point blast): 256^3 cells on [0,1]^3, reflective faces,
+superbee limiter, gamma = 5/3, rho = 1 +- 1% seeded jitter, p = 1e-5 at
+rest, unit energy deposited as pressure in the cells within 3 cells of the
+centre; exact Riemann solver with tol 1e-6 and 100 iterations, Courant
+number 0.2 (the reference's own 3D example value; at 0.8 the unsplit 3D
+update turns pressures negative on the first step in the reference
+algorithm itself).
+
+A step is one main-loop step (dt from the CFL reduction, ghost fill,
+MUSCL-Hancock update) of the whole box, state resident in HBM.  W warmup
+steps run untimed from the initial condition, then K steps are timed between
+two stream synchronisations; zone-updates/s = 256^3 * K / elapsed.
+
+Also reported:
+  roofline      algorithmic HBM bytes of the step (2 * 5 * 8 = 80 B per
+                zone-update: one read and one write of the FP64 state) over
+                the measured time of one step's kernels, against 8 TB/s.
+  cpu_baseline  the CPU oracle (a restatement of the reference Fortran path,
+                both loop structures) on the host cores, on a bounded sample
+                of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "zone-updates/s at 3D 256^3 (FP64 vs mixed), % HBM roofline; serve p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def blast_params(hydro, solver):
+    return hydro.make_params(
+        3, thermo=hydro.Thermo(state_of_matter=hydro.MONATOMIC),
+        rp=hydro.RiemannProblem(irs=hydro.Irs(n_iteration=100, tolerance=1e-6)),
+        sl=hydro.SlopeLimiter(type=hydro.SUPERBEE, w=0.0),
+        mh=hydro.MusclHancock(solver_type=solver),
+        cfl=hydro.Cfl(0.2), bc=hydro.SamrBc(types=[hydro.REFLECTIVE] * 6))
+
+
+def gpu_leg(n, steps, warmup, seed):
+    import numpy as np
+    from rhyme_amd import hydro, ics
+
+    cells = ics.blast_3d(n=n, seed=seed)
+    params = blast_params(hydro, hydro.MEMORY_INTENSIVE)
+    dx = [1.0 / n] * 3
+    box = hydro.SamrBox(3, [n, n, n], cells=cells)
+    box.upload()
+    if warmup:
+        hydro.run(params, box, dx, warmup)
+    # timed region: K steps enqueued, bracketed by stream syncs; HIP events
+    # recorded around every launch on the box's stream
+    box.set_profiling(True)
+    box.sync()
+    t0 = time.perf_counter()
+    hydro.run_async(params, box, dx, steps)
+    box.sync()
+    t1 = time.perf_counter()
+    stats = {k: box.kernel_stats(k) for k in range(5)}
+    dts = hydro.last_dts(box, steps)
+    out = box.download()
+    interior = out[:5, 2:-2, 2:-2, 2:-2]
+    finite = bool(np.isfinite(interior).all())
+    box.close()
+    return dict(elapsed=t1 - t0, stats=stats, dts=dts, finite=finite)
+
+
+def cpu_leg(n_sample, steps_sample, seed, threads_all):
+    """The oracle on the host cores: both reference loop structures with all
+    threads, plus memory_intensive on one thread (at a quarter of the edge),
+    on the first steps of the same blast workload.  Each configuration runs
+    in its own process so that OMP_NUM_THREADS takes effect."""
+    from rhyme_amd import hydro
+    return {
+        "memory_intensive": _cpu_subprocess(n_sample, steps_sample, seed,
+                                            hydro.MEMORY_INTENSIVE, threads_all),
+        "cpu_intensive": _cpu_subprocess(n_sample, steps_sample, seed,
+                                         hydro.CPU_INTENSIVE, threads_all),
+        "memory_intensive_1thread": _cpu_subprocess(max(32, n_sample // 4), steps_sample, seed,
+                                                    hydro.MEMORY_INTENSIVE, 1),
+    }
+
+
+def _cpu_subprocess(n, steps, seed, solver, threads):
+    import subprocess
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    code = (
+        "import sys, time, json; sys.path.insert(0, %r)\n"
+        "import oracle\n"
+        "from rhyme_amd import hydro, ics\n"
+        "import bench\n"
+        "cells = ics.blast_3d(n=%d, seed=%d)\n"
+        "p = oracle.params_from(bench.blast_params(hydro, %d))\n"
+        "dx = [1.0 / %d] * 3\n"
+        "t0 = time.perf_counter()\n"
+        "oracle.run(p, [%d] * 3, cells, dx, %d)\n"
+        "t1 = time.perf_counter()\n"
+        "print(json.dumps(dict(seconds=t1 - t0, threads=oracle.num_threads())))\n"
+    ) % (ROOT, n, seed, solver, n, n, steps)
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed:\n" + r.stderr)
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    zu = n ** 3 * steps / d["seconds"]
+    return dict(zone_updates_per_s=zu, seconds=d["seconds"], threads=d["threads"],
+                sample="3D blast %d^3, steps 0..%d from the initial condition" % (n, steps))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--skip-cpu", action="store_true",
+                    help="omit the CPU baseline leg (profiling runs)")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run one plain device copy of 1 GiB (torch), the "
+                         "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(--gpus %d requested)" % a.gpus, file=sys.stderr)
+        return 2
+
+    if a.copy_probe:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    g = gpu_leg(a.n, a.steps, a.warmup, a.seed)
+    zones = a.n ** 3
+    zu = zones * a.steps / g["elapsed"]
+    ms_step = 1e3 * g["elapsed"] / a.steps
+    st = g["stats"]
+    # kernels of one step: dt (4), flux x/y/z (0), update (1)
+    kern_ms = sum(st[k][1] for k in (0, 1, 4)) / max(a.steps, 1)
+    flux_ms = st[0][1] / max(st[0][0], 1)
+    bytes_alg = 80.0 * zones
+    achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
+    roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    kernel="one step: dt_kernel + 3 x flux_kernel + update_kernel",
+                    step_kernel_ms=kern_ms, flux_kernel_avg_ms=flux_ms,
+                    update_kernel_avg_ms=st[1][1] / max(st[1][0], 1),
+                    bytes_per_zone_update=80)
+    line = dict(metric=METRIC, value=zu, unit="zone-updates/s", n_gpus=1, steps=a.steps,
+                warmup=a.warmup, ms_per_step=ms_step, higher_is_better=True, scaling="weak",
+                vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload="3D %d^3 point blast, FP64 drop-in path "
+                            "(memory_intensive order), superbee, reflective, CFL 0.2, "
+                            "exact RS tol 1e-6 / 100 it" % a.n,
+                            grid=[a.n] * 3, precision="fp64", state_finite=g["finite"],
+                            dt_first=float(g["dts"][0]), dt_last=float(g["dts"][-1])),
+                roofline=roofline)
+    if not a.skip_cpu:
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)
+        m = c["memory_intensive"]
+        line["cpu_baseline"] = dict(
+            value=m["zone_updates_per_s"], unit="zone-updates/s", cores=m["threads"],
+            kind="port", sample=m["sample"] + " (memory_intensive, all threads)",
+            cpu_intensive=c["cpu_intensive"],
+            one_thread=c["memory_intensive_1thread"])
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -54,7 +54,9 @@ struct rhy_box {
   hipStream_t stream = nullptr;
   double* cells = nullptr;        // ncmp planes (hydro first)
   double* flux = nullptr;         // ndim * (ndim+2) planes
-  unsigned long long* slots = nullptr;  // [0..1] CFL max slots
+  double* partial = nullptr;      // per-block CFL maxima of the last reduction
+  int32_t partial_n = 0;
+  double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
   double* dt = nullptr;           // current dt
   double* dt_log = nullptr;       // per-step dt of the last run
   int32_t dt_log_cap = 0;
@@ -170,8 +172,7 @@ int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
 }
 
 template <int ND, int ORDER, bool FUSED>
-int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc,
-                    unsigned long long* slot_next) {
+int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc) {
   const Geom& G = b->geom;
   const int64_t P = G.plane;
   const int nv = ND + 2;
@@ -183,7 +184,7 @@ int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc,
   timed_begin(b, 1, tl);
   hipLaunchKernelGGL((update_kernel<ND, ORDER, FUSED>), dim3(grid_for(n)), dim3(kThreads), 0,
                      b->stream, k, G, b->cells, F0, F1, F2, b->dt, dx[0], dx[1], dx[2], bc,
-                     slot_next);
+                     b->partial);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -191,12 +192,12 @@ int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc,
 
 template <int ND>
 int launch_update(rhy_box* b, const Consts& k, const double* dx, int order, bool fused,
-                  BcTypes bc, unsigned long long* slot_next) {
+                  BcTypes bc) {
   if (order == RHY_MEMORY_INTENSIVE)
-    return fused ? launch_update_t<ND, 2, true>(b, k, dx, bc, slot_next)
-                 : launch_update_t<ND, 2, false>(b, k, dx, bc, slot_next);
-  return fused ? launch_update_t<ND, 1, true>(b, k, dx, bc, slot_next)
-               : launch_update_t<ND, 1, false>(b, k, dx, bc, slot_next);
+    return fused ? launch_update_t<ND, 2, true>(b, k, dx, bc)
+                 : launch_update_t<ND, 2, false>(b, k, dx, bc);
+  return fused ? launch_update_t<ND, 1, true>(b, k, dx, bc)
+               : launch_update_t<ND, 1, false>(b, k, dx, bc);
 }
 
 template <int ND>
@@ -210,7 +211,7 @@ int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
     maxface = m > maxface ? m : maxface;
   }
   // inflow states travel as a kernel argument copy in device memory
-  double* dinf = reinterpret_cast<double*>(b->slots + 2);
+  double* dinf = b->scalars + 16;
   HIP_TRY(hipMemcpyAsync(dinf, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
                          hipMemcpyHostToDevice, b->stream));
   TimedLaunch tl;
@@ -224,15 +225,13 @@ int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
 }
 
 template <int ND>
-int launch_cfl(rhy_box* b, const Consts& k, unsigned long long* slot) {
+int launch_cfl(rhy_box* b, const Consts& k) {
   const Geom& G = b->geom;
   int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
   TimedLaunch tl;
   timed_begin(b, 3, tl);
-  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, b->stream, (double*)nullptr, 0.0,
-                     slot);
   hipLaunchKernelGGL((cfl_kernel<ND>), dim3(grid_for(n)), dim3(kThreads), 0, b->stream, k, G,
-                     b->cells, slot);
+                     b->cells, b->partial);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -262,19 +261,17 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
   int rc;
   if ((rc = ensure_dt_log(b, nsteps > 0 ? nsteps : 1))) return rc;
   // prologue: CFL of the current state and the ghost fill it is stepped with
-  if ((rc = launch_cfl<ND>(b, k, b->slots + 0))) return rc;
+  if ((rc = launch_cfl<ND>(b, k))) return rc;
   if ((rc = launch_bc<ND>(b, p, -1))) return rc;
   const double cmdx = p->courant * min_dx(ND, dx);
   for (int32_t s = 0; s < nsteps; ++s) {
-    unsigned long long* cur = b->slots + (s & 1);
-    unsigned long long* nxt = b->slots + ((s + 1) & 1);
     TimedLaunch tl;
     timed_begin(b, 4, tl);
-    hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(64), 0, b->stream, cmdx, cur, nxt, b->dt,
-                       b->dt_log + s);
+    hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1024), 0, b->stream, b->partial, b->partial_n,
+                       cmdx, b->dt, b->dt_log + s, (double*)nullptr);
     timed_end(b, tl);
     if ((rc = launch_flux<ND>(b, k, dx))) return rc;
-    if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc, nxt))) return rc;
+    if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc))) return rc;
   }
   b->last_nsteps = nsteps;
   HIP_TRY(hipGetLastError());
@@ -289,8 +286,10 @@ int check_box(rhy_box* b) {
 void free_box_memory(rhy_box* b) {
   if (b->cells) hipFree(b->cells);
   if (b->flux) hipFree(b->flux);
+  if (b->partial) hipFree(b->partial);
   b->cells = nullptr;
   b->flux = nullptr;
+  b->partial = nullptr;
 }
 
 int alloc_box_memory(rhy_box* b) {
@@ -304,6 +303,12 @@ int alloc_box_memory(rhy_box* b) {
   if (hipMalloc(&b->flux, sizeof(double) * (size_t)P * nv * b->ndim) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "flux scratch allocation failed");
+  }
+  const int64_t ncell = (int64_t)b->geom.n[0] * b->geom.n[1] * b->geom.n[2];
+  b->partial_n = (int32_t)grid_for(ncell);
+  if (hipMalloc(&b->partial, sizeof(double) * (size_t)b->partial_n) != hipSuccess) {
+    free_box_memory(b);
+    return fail(RHY_ERR_OOM, "reduction scratch allocation failed");
   }
   // ghosts start at 0 (rhyme_samr_bc_init.f90:61-87)
   HIP_TRY(hipMemsetAsync(b->cells, 0, sizeof(double) * (size_t)P * b->ncmp, b->stream));
@@ -381,7 +386,7 @@ int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t pr
     delete b;
     return fail(RHY_ERR_HIP, "stream creation failed");
   }
-  if (hipMalloc(&b->slots, sizeof(unsigned long long) * (4 + 6 * RHY_MAX_CMP)) != hipSuccess ||
+  if (hipMalloc(&b->scalars, sizeof(double) * (16 + 6 * RHY_MAX_CMP)) != hipSuccess ||
       hipMalloc(&b->dt, sizeof(double)) != hipSuccess) {
     rhy_box_destroy(b);
     return fail(RHY_ERR_OOM, "scalar allocation failed");
@@ -412,7 +417,7 @@ int rhy_box_destroy(rhy_box_t* b) {
   if (b->stream) hipStreamSynchronize(b->stream);
   collect_timings(b);
   free_box_memory(b);
-  if (b->slots) hipFree(b->slots);
+  if (b->scalars) hipFree(b->scalars);
   if (b->dt) hipFree(b->dt);
   if (b->dt_log) hipFree(b->dt_log);
   if (b->stream) hipStreamDestroy(b->stream);
@@ -448,9 +453,6 @@ int rhy_box_sync(rhy_box_t* b) {
   return RHY_OK;
 }
 
-#define DISPATCH_ND(nd, call)                   \
-  ((nd) == 1 ? call<1> : (nd) == 2 ? call<2> : call<3>)
-
 int rhy_bc_set_boundaries(rhy_box_t* b, const rhy_params_t* p) {
   int rc;
   if ((rc = check_box(b))) return rc;
@@ -483,15 +485,16 @@ int rhy_cfl_time_step(rhy_box_t* b, const rhy_params_t* p, const double dx[3], d
   if ((rc = check_params(p, b->ndim))) return rc;
   if (!dx || !dt) return fail(RHY_ERR_ARG, "dx/dt is NULL");
   const Consts k = make_consts(p);
-  if (b->ndim == 1) rc = launch_cfl<1>(b, k, b->slots + 0);
-  else if (b->ndim == 2) rc = launch_cfl<2>(b, k, b->slots + 0);
-  else rc = launch_cfl<3>(b, k, b->slots + 0);
+  if (b->ndim == 1) rc = launch_cfl<1>(b, k);
+  else if (b->ndim == 2) rc = launch_cfl<2>(b, k);
+  else rc = launch_cfl<3>(b, k);
   if (rc) return rc;
-  unsigned long long bits = 0;
-  HIP_TRY(hipMemcpyAsync(&bits, b->slots, sizeof(bits), hipMemcpyDeviceToHost, b->stream));
+  hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1024), 0, b->stream, b->partial, b->partial_n, 0.0,
+                     (double*)nullptr, (double*)nullptr, b->scalars);
+  HIP_TRY(hipGetLastError());
+  double maxu = 0.0;
+  HIP_TRY(hipMemcpyAsync(&maxu, b->scalars, sizeof(maxu), hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
-  double maxu;
-  std::memcpy(&maxu, &bits, sizeof(maxu));
   *dt = p->courant * min_dx(b->ndim, dx) / maxu;
   return RHY_OK;
 }
@@ -504,18 +507,17 @@ int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double
   if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
     return fail(RHY_ERR_SOLVER, "Unknown solver type");
   const Consts k = make_consts(p);
-  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, b->stream, b->dt, dt,
-                     (unsigned long long*)nullptr);
+  hipLaunchKernelGGL(set_scalar_kernel, dim3(1), dim3(64), 0, b->stream, b->dt, dt);
   const BcTypes bc = make_bc(p);
   if (b->ndim == 1) {
     if ((rc = launch_flux<1>(b, k, dx))) return rc;
-    rc = launch_update<1>(b, k, dx, p->solver_type, false, bc, nullptr);
+    rc = launch_update<1>(b, k, dx, p->solver_type, false, bc);
   } else if (b->ndim == 2) {
     if ((rc = launch_flux<2>(b, k, dx))) return rc;
-    rc = launch_update<2>(b, k, dx, p->solver_type, false, bc, nullptr);
+    rc = launch_update<2>(b, k, dx, p->solver_type, false, bc);
   } else {
     if ((rc = launch_flux<3>(b, k, dx))) return rc;
-    rc = launch_update<3>(b, k, dx, p->solver_type, false, bc, nullptr);
+    rc = launch_update<3>(b, k, dx, p->solver_type, false, bc);
   }
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));

@@ -99,18 +99,28 @@ __device__ __forceinline__ double cfl_speed(const Consts& k, const double* u) {
   return sqrt(s) / u[0] + speed_of_sound<ND>(k, u);
 }
 
-// Block-wide max of non-negative values into a 64-bit slot holding a double.
-// max is exact and order independent, so the result is deterministic; NaN
-// speeds are skipped like the reference's strict `u > max_u` test.
-__device__ __forceinline__ void block_max_to_slot(double v, unsigned long long* slot) {
+// Block-wide max of non-negative values; thread 0 of the block writes the
+// block's maximum to partial[blockIdx.x].  max is exact and order
+// independent, so the reduction is deterministic; NaN speeds are skipped like
+// the reference's strict `u > max_u` test (rhyme_cfl_time_step.f90:44).
+// Every thread of the block must call it (blockDim.x a multiple of 64, <= 1024).
+__device__ __forceinline__ void block_max_to_partial(double v, double* partial) {
+  __shared__ double wave_max[16];
   v = v > 0.0 ? v : 0.0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);
     v = o > v ? o : v;
   }
-  if ((threadIdx.x & 63) == 0 && v > 0.0)
-    atomicMax(slot, (unsigned long long)__double_as_longlong(v));
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wave_max[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) m = wave_max[i] > m ? wave_max[i] : m;
+    partial[blockIdx.x] = m;
+  }
 }
 
 // Intercell flux on every face normal to axis A that the interior update
@@ -170,7 +180,7 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* _
                                                      const double* __restrict__ F2,
                                                      const double* __restrict__ dtp, double dx0,
                                                      double dx1, double dx2, BcTypes bc,
-                                                     unsigned long long* maxu_next) {
+                                                     double* __restrict__ partial) {
   constexpr int NV = ND + 2;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
@@ -209,7 +219,7 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* _
       speed = cfl_speed<ND>(k, u);
     }
   }
-  if (FUSED) block_max_to_slot(speed, maxu_next);
+  if (FUSED) block_max_to_partial(speed, partial);
 }
 
 // Stand-alone ghost fill, one thread per face cell; blockIdx.y = face
@@ -274,7 +284,7 @@ __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const doubl
 // max over the interior of |v| + c_s (rhyme_cfl_time_step.f90:40-47).
 template <int ND>
 __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const double* __restrict__ cells,
-                                                  unsigned long long* slot) {
+                                                  double* __restrict__ partial) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
   double v = 0.0;
@@ -287,28 +297,40 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const double
     load_state<ND>(cells, G.plane, goff(G, i, j, kk), u);
     v = cfl_speed<ND>(k, u);
   }
-  block_max_to_slot(v, slot);
+  block_max_to_partial(v, partial);
 }
 
-// dt of the step: courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the
-// product courant*minval(dx) is formed on the host.  Also clears the slot
-// the coming update will fill and logs dt.
-__global__ void dt_kernel(double cmdx, const unsigned long long* slot_cur,
-                          unsigned long long* slot_next, double* dt_out, double* dt_log) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double maxu = __longlong_as_double((long long)*slot_cur);
-    double dt = cmdx / maxu;
-    *dt_out = dt;
-    if (dt_log) *dt_log = dt;
-    if (slot_next) *slot_next = 0ull;
+// Final max over the per-block partials (one block of 1024 threads), then
+// dt = courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the product
+// courant*minval(dx) is formed on the host.  Writes dt (and logs it) and the
+// raw max_u bits to maxu_out when given.
+__global__ __launch_bounds__(1024) void dt_kernel(const double* __restrict__ partial, int32_t n,
+                                                  double cmdx, double* dt_out, double* dt_log,
+                                                  double* maxu_out) {
+  __shared__ double wave_max[16];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) v = partial[i] > v ? partial[i] : v;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double o = __shfl_xor(v, off, 64);
+    v = o > v ? o : v;
+  }
+  if ((threadIdx.x & 63) == 0) wave_max[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) m = wave_max[i] > m ? wave_max[i] : m;
+    if (maxu_out) *maxu_out = m;
+    if (dt_out) {
+      double dt = cmdx / m;
+      *dt_out = dt;
+      if (dt_log) *dt_log = dt;
+    }
   }
 }
 
-__global__ void set_scalar_kernel(double* dst, double v, unsigned long long* slot_clear) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    if (dst) *dst = v;
-    if (slot_clear) *slot_clear = 0ull;
-  }
+__global__ void set_scalar_kernel(double* dst, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 
 // ------------------------------------------------------ unit batches ----
