@@ -44,14 +44,14 @@ def blast_params(hydro, solver):
         cfl=hydro.Cfl(0.2), bc=hydro.SamrBc(types=[hydro.REFLECTIVE] * 6))
 
 
-def gpu_leg(n, steps, warmup, seed):
+def gpu_leg(n, steps, warmup, seed, precision=0):
     import numpy as np
     from rhyme_amd import hydro, ics
 
     cells = ics.blast_3d(n=n, seed=seed)
     params = blast_params(hydro, hydro.MEMORY_INTENSIVE)
     dx = [1.0 / n] * 3
-    box = hydro.SamrBox(3, [n, n, n], cells=cells)
+    box = hydro.SamrBox(3, [n, n, n], cells=cells, precision=precision)
     box.upload()
     if warmup:
         hydro.run(params, box, dx, warmup)
@@ -66,10 +66,10 @@ def gpu_leg(n, steps, warmup, seed):
     stats = {k: box.kernel_stats(k) for k in range(5)}
     dts = hydro.last_dts(box, steps)
     out = box.download()
-    interior = out[:5, 2:-2, 2:-2, 2:-2]
+    interior = out[:5, 2:-2, 2:-2, 2:-2].copy()
     finite = bool(np.isfinite(interior).all())
     box.close()
-    return dict(elapsed=t1 - t0, stats=stats, dts=dts, finite=finite)
+    return dict(elapsed=t1 - t0, stats=stats, dts=dts, finite=finite, state=interior)
 
 
 def cpu_leg(n_sample, steps_sample, seed, threads_all):
@@ -125,6 +125,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--skip-cpu", action="store_true",
                     help="omit the CPU baseline leg (profiling runs)")
+    ap.add_argument("--skip-mixed", action="store_true",
+                    help="omit the mixed-precision leg")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain device copy of 1 GiB (torch), the "
                          "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
@@ -167,6 +169,24 @@ def main():
                             grid=[a.n] * 3, precision="fp64", state_finite=g["finite"],
                             dt_first=float(g["dts"][0]), dt_last=float(g["dts"][-1])),
                 roofline=roofline)
+    if not a.skip_mixed:
+        import numpy as np
+        m = gpu_leg(a.n, a.steps, a.warmup, a.seed, precision=1)
+        zu_m = zones * a.steps / m["elapsed"]
+        ref = g["state"]
+        rel = [float(np.abs(m["state"][c] - ref[c]).sum() / np.abs(ref[c]).sum())
+               for c in range(5)]
+        line["mixed"] = dict(
+            value=zu_m, unit="zone-updates/s", ms_per_step=1e3 * m["elapsed"] / a.steps,
+            speedup_vs_fp64=zu_m / zu, dtype="f32 state/bulk, f64 Riemann p*/sampling/dt",
+            rel_l1_vs_fp64=dict(zip(["rho", "rho_u", "rho_v", "rho_w", "e_tot"], rel)),
+            rel_l1_note="against the FP64 device path after the same warmup+steps; the FP64 "
+                        "path matches the FP64 oracle to <= 1e-12 (tests/test_gpu_parity.py)",
+            flux_kernel_avg_ms=m["stats"][0][1] / max(m["stats"][0][0], 1),
+            update_kernel_avg_ms=m["stats"][1][1] / max(m["stats"][1][0], 1),
+            roofline_achieved_gbs=40.0 * zones / (sum(m["stats"][k][1] for k in (0, 1, 4))
+                                                   / max(a.steps, 1) * 1e-3) / 1e9,
+            bytes_per_zone_update=40, state_finite=m["finite"])
     if not a.skip_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)

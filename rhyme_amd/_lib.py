@@ -10,7 +10,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "librhyme_hip.so")
+# RHYME_HIP_LIB selects another build of the same library (e.g. one compiled
+# with different tile constants); the default is the in-tree build.
+LIB_PATH = os.environ.get("RHYME_HIP_LIB") or os.path.join(HERE, "librhyme_hip.so")
 
 RHY_OK = 0
 MAX_CMP = 16

@@ -30,9 +30,10 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=None):
+def build(verbose=False, extra=None, out=None):
     srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
+    lib = out or LIB
+    tmp = lib + ".tmp"
     cmd = [HIPCC] + FLAGS + (extra or []) + srcs + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
@@ -40,8 +41,8 @@ def build(verbose=False, extra=None):
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building librhyme_hip.so")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

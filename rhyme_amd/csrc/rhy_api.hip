@@ -52,8 +52,9 @@ struct rhy_box {
   int32_t precision = RHY_FP64;
   Geom geom{};
   hipStream_t stream = nullptr;
-  double* cells = nullptr;        // ncmp planes (hydro first)
-  double* flux = nullptr;         // ndim * (ndim+2) planes
+  void* cells = nullptr;          // ncmp planes (hydro first), double or float
+  void* flux = nullptr;           // ndim * (ndim+2) planes, same type
+  size_t esz = sizeof(double);    // bytes per stored value
   double* partial = nullptr;      // per-block CFL maxima of the last reduction
   int32_t partial_n = 0;
   double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
@@ -146,58 +147,95 @@ void collect_timings(rhy_box* b) {
   b->pending.clear();
 }
 
+// Faces walked per thread along each axis (x walks with a stride of NF
+// doubles between lanes, so it takes fewer; y and z lanes stay coalesced).
+#ifndef RHY_NF_X
+#define RHY_NF_X 4
+#endif
+#ifndef RHY_NF_YZ
+#define RHY_NF_YZ 8
+#endif
+constexpr int kNfX = RHY_NF_X;
+constexpr int kNfYZ = RHY_NF_YZ;
+
+template <int ND, int A, int LIM, typename R, class PW>
+void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx) {
+  const Geom& G = b->geom;
+  constexpr int NF = A == 0 ? kNfX : kNfYZ;
+  const int64_t nchunk = (G.n[A] + 1 + NF - 1) / NF;
+  int64_t nthreads = nchunk;
+  for (int d = 0; d < 3; ++d)
+    if (d != A) nthreads *= G.n[d];
+  R* F = static_cast<R*>(b->flux) + (int64_t)A * (ND + 2) * G.plane;
+  TimedLaunch tl;
+  timed_begin(b, 0, tl);
+  hipLaunchKernelGGL((flux_kernel<ND, A, LIM, R, PW, NF>), dim3(grid_for(nthreads)),
+                     dim3(kThreads), 0, b->stream, k, G, static_cast<const R*>(b->cells), F, b->dt,
+                     dx[A]);
+  timed_end(b, tl);
+}
+
+template <int ND, int LIM, typename R, class PW>
+void launch_flux_lim(rhy_box* b, const Consts& k, const double* dx) {
+  launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx);
+  if constexpr (ND > 1) launch_flux_axis<ND, 1, LIM, R, PW>(b, k, dx);
+  if constexpr (ND > 2) launch_flux_axis<ND, 2, LIM, R, PW>(b, k, dx);
+}
+
+template <int ND, typename R, class PW>
+void launch_flux_r(rhy_box* b, const Consts& k, const double* dx) {
+  switch (k.limiter) {
+    case 1: launch_flux_lim<ND, 1, R, PW>(b, k, dx); break;
+    case 2: launch_flux_lim<ND, 2, R, PW>(b, k, dx); break;
+    case 3: launch_flux_lim<ND, 3, R, PW>(b, k, dx); break;
+    default: launch_flux_lim<ND, 4, R, PW>(b, k, dx); break;
+  }
+}
+
 template <int ND>
 int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
-  const Geom& G = b->geom;
-  const int64_t P = G.plane;
-  const int nv = ND + 2;
-  for (int a = 0; a < ND; ++a) {
-    int64_t nf = (int64_t)(G.n[0] + (a == 0)) * (G.n[1] + (a == 1)) * (G.n[2] + (a == 2));
-    double* F = b->flux + (int64_t)a * nv * P;
-    TimedLaunch tl;
-    timed_begin(b, 0, tl);
-    if (a == 0)
-      hipLaunchKernelGGL((flux_kernel<ND, 0>), dim3(grid_for(nf)), dim3(kThreads), 0, b->stream,
-                         k, G, b->cells, F, b->dt, dx[0]);
-    else if (a == 1)
-      hipLaunchKernelGGL((flux_kernel<ND, (ND > 1 ? 1 : 0)>), dim3(grid_for(nf)), dim3(kThreads),
-                         0, b->stream, k, G, b->cells, F, b->dt, dx[1]);
-    else
-      hipLaunchKernelGGL((flux_kernel<ND, (ND > 2 ? 2 : 0)>), dim3(grid_for(nf)), dim3(kThreads),
-                         0, b->stream, k, G, b->cells, F, b->dt, dx[2]);
-    timed_end(b, tl);
-  }
+  if (b->precision == RHY_MIXED)
+    launch_flux_r<ND, float, PowFast>(b, k, dx);
+  else
+    launch_flux_r<ND, double, PowLib>(b, k, dx);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
 }
 
-template <int ND, int ORDER, bool FUSED>
+template <int ND, int ORDER, bool FUSED, typename R>
 int launch_update_t(rhy_box* b, const Consts& k, const double* dx, BcTypes bc) {
   const Geom& G = b->geom;
   const int64_t P = G.plane;
   const int nv = ND + 2;
   int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
-  const double* F0 = b->flux;
-  const double* F1 = ND > 1 ? b->flux + (int64_t)nv * P : nullptr;
-  const double* F2 = ND > 2 ? b->flux + (int64_t)2 * nv * P : nullptr;
+  const R* F0 = static_cast<const R*>(b->flux);
+  const R* F1 = ND > 1 ? F0 + (int64_t)nv * P : nullptr;
+  const R* F2 = ND > 2 ? F0 + (int64_t)2 * nv * P : nullptr;
   TimedLaunch tl;
   timed_begin(b, 1, tl);
-  hipLaunchKernelGGL((update_kernel<ND, ORDER, FUSED>), dim3(grid_for(n)), dim3(kThreads), 0,
-                     b->stream, k, G, b->cells, F0, F1, F2, b->dt, dx[0], dx[1], dx[2], bc,
-                     b->partial);
+  hipLaunchKernelGGL((update_kernel<ND, ORDER, FUSED, R>), dim3(grid_for(n)), dim3(kThreads), 0,
+                     b->stream, k, G, static_cast<R*>(b->cells), F0, F1, F2, b->dt, dx[0], dx[1],
+                     dx[2], bc, b->partial);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
 }
 
+template <int ND, typename R>
+int launch_update_r(rhy_box* b, const Consts& k, const double* dx, int order, bool fused,
+                    BcTypes bc) {
+  if (order == RHY_MEMORY_INTENSIVE)
+    return fused ? launch_update_t<ND, 2, true, R>(b, k, dx, bc)
+                 : launch_update_t<ND, 2, false, R>(b, k, dx, bc);
+  return fused ? launch_update_t<ND, 1, true, R>(b, k, dx, bc)
+               : launch_update_t<ND, 1, false, R>(b, k, dx, bc);
+}
+
 template <int ND>
 int launch_update(rhy_box* b, const Consts& k, const double* dx, int order, bool fused,
                   BcTypes bc) {
-  if (order == RHY_MEMORY_INTENSIVE)
-    return fused ? launch_update_t<ND, 2, true>(b, k, dx, bc)
-                 : launch_update_t<ND, 2, false>(b, k, dx, bc);
-  return fused ? launch_update_t<ND, 1, true>(b, k, dx, bc)
-               : launch_update_t<ND, 1, false>(b, k, dx, bc);
+  if (b->precision == RHY_MIXED) return launch_update_r<ND, float>(b, k, dx, order, fused, bc);
+  return launch_update_r<ND, double>(b, k, dx, order, fused, bc);
 }
 
 template <int ND>
@@ -210,15 +248,19 @@ int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
       if (d != a) m *= G.n[d];
     maxface = m > maxface ? m : maxface;
   }
-  // inflow states travel as a kernel argument copy in device memory
+  // inflow states travel in device memory
   double* dinf = b->scalars + 16;
   HIP_TRY(hipMemcpyAsync(dinf, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
                          hipMemcpyHostToDevice, b->stream));
   TimedLaunch tl;
   timed_begin(b, 2, tl);
   dim3 grid(grid_for(maxface), face_only >= 0 ? 1 : 2 * ND);
-  hipLaunchKernelGGL((bc_kernel<ND>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p), dinf,
-                     b->ncmp, b->cells, face_only);
+  if (b->precision == RHY_MIXED)
+    hipLaunchKernelGGL((bc_kernel<ND, float>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p),
+                       dinf, b->ncmp, static_cast<float*>(b->cells), face_only);
+  else
+    hipLaunchKernelGGL((bc_kernel<ND, double>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p),
+                       dinf, b->ncmp, static_cast<double*>(b->cells), face_only);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -230,8 +272,12 @@ int launch_cfl(rhy_box* b, const Consts& k) {
   int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
   TimedLaunch tl;
   timed_begin(b, 3, tl);
-  hipLaunchKernelGGL((cfl_kernel<ND>), dim3(grid_for(n)), dim3(kThreads), 0, b->stream, k, G,
-                     b->cells, b->partial);
+  if (b->precision == RHY_MIXED)
+    hipLaunchKernelGGL((cfl_kernel<ND, float>), dim3(grid_for(n)), dim3(kThreads), 0, b->stream,
+                       k, G, static_cast<const float*>(b->cells), b->partial);
+  else
+    hipLaunchKernelGGL((cfl_kernel<ND, double>), dim3(grid_for(n)), dim3(kThreads), 0, b->stream,
+                       k, G, static_cast<const double*>(b->cells), b->partial);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -296,11 +342,12 @@ int alloc_box_memory(rhy_box* b) {
   b->geom = make_geom(b->ndim, b->dims);
   const int64_t P = b->geom.plane;
   const int nv = b->ndim + 2;
-  if (hipMalloc(&b->cells, sizeof(double) * (size_t)P * b->ncmp) != hipSuccess) {
+  b->esz = b->precision == RHY_MIXED ? sizeof(float) : sizeof(double);
+  if (hipMalloc(&b->cells, b->esz * (size_t)P * b->ncmp) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "state allocation failed");
   }
-  if (hipMalloc(&b->flux, sizeof(double) * (size_t)P * nv * b->ndim) != hipSuccess) {
+  if (hipMalloc(&b->flux, b->esz * (size_t)P * nv * b->ndim) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "flux scratch allocation failed");
   }
@@ -311,7 +358,7 @@ int alloc_box_memory(rhy_box* b) {
     return fail(RHY_ERR_OOM, "reduction scratch allocation failed");
   }
   // ghosts start at 0 (rhyme_samr_bc_init.f90:61-87)
-  HIP_TRY(hipMemsetAsync(b->cells, 0, sizeof(double) * (size_t)P * b->ncmp, b->stream));
+  HIP_TRY(hipMemsetAsync(b->cells, 0, b->esz * (size_t)P * b->ncmp, b->stream));
   return RHY_OK;
 }
 
@@ -376,7 +423,8 @@ int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t pr
   if ((rc = rhy_init(0))) return rc;
   if ((rc = check_dims(ndim, dims))) return rc;
   if (ncmp < ndim + 2 || ncmp > RHY_MAX_CMP) return fail(RHY_ERR_ARG, "ncmp out of range");
-  if (precision != RHY_FP64) return fail(RHY_ERR_ARG, "unsupported precision");
+  if (precision != RHY_FP64 && precision != RHY_MIXED)
+    return fail(RHY_ERR_ARG, "unsupported precision");
   rhy_box* b = new rhy_box();
   b->ndim = ndim;
   for (int d = 0; d < 3; ++d) b->dims[d] = dims[d];
@@ -429,8 +477,16 @@ int rhy_box_upload(rhy_box_t* b, const double* cells) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
-  HIP_TRY(hipMemcpyAsync(b->cells, cells, sizeof(double) * (size_t)b->geom.plane * b->ncmp,
-                         hipMemcpyHostToDevice, b->stream));
+  const size_t n = (size_t)b->geom.plane * b->ncmp;
+  if (b->precision == RHY_MIXED) {
+    std::vector<float> tmp(n);
+    for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
+    HIP_TRY(hipMemcpyAsync(b->cells, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RHY_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(b->cells, cells, sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
 }
@@ -439,8 +495,16 @@ int rhy_box_download(rhy_box_t* b, double* cells) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
-  HIP_TRY(hipMemcpyAsync(cells, b->cells, sizeof(double) * (size_t)b->geom.plane * b->ncmp,
-                         hipMemcpyDeviceToHost, b->stream));
+  const size_t n = (size_t)b->geom.plane * b->ncmp;
+  if (b->precision == RHY_MIXED) {
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpyAsync(tmp.data(), b->cells, sizeof(float) * n, hipMemcpyDeviceToHost,
+                           b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < n; ++i) cells[i] = (double)tmp[i];
+    return RHY_OK;
+  }
+  HIP_TRY(hipMemcpyAsync(cells, b->cells, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
   HIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
 }

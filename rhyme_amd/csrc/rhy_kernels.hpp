@@ -9,6 +9,17 @@
 
 #include "rhy_physics.hpp"
 
+// Minimum waves per SIMD requested for the flux kernels (caps their VGPRs:
+// 4 -> 128, 3 -> 168, 2 -> 256); 0 leaves the choice to the compiler.
+#ifndef RHY_FLUX_WPE
+#define RHY_FLUX_WPE 0
+#endif
+#if RHY_FLUX_WPE > 0
+#define RHY_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RHY_FLUX_WPE)))
+#else
+#define RHY_FLUX_OCCUPANCY
+#endif
+
 namespace rhy {
 
 // Geometry of a ghosted box: Fortran cell (i, j, k), 1-based interior
@@ -26,9 +37,9 @@ __device__ __forceinline__ int64_t goff(const Geom& G, int i, int j, int k) {
          (int64_t)(k - 1 + G.g[2]) * G.s[2];
 }
 
-template <int ND>
-__device__ __forceinline__ void load_state(const double* __restrict__ cells, int64_t plane,
-                                           int64_t o, double* u) {
+template <int ND, typename R = double>
+__device__ __forceinline__ void load_state(const R* __restrict__ cells, int64_t plane, int64_t o,
+                                           R* u) {
 #pragma unroll
   for (int c = 0; c < ND + 2; ++c) u[c] = cells[o + c * plane];
 }
@@ -44,10 +55,10 @@ struct BcTypes {
 // Reflective/outflow mirror cells 1,2 -> ghosts 0,-1 (n, n-1 -> n+1, n+2),
 // periodic wraps n, n-1 -> 0, -1 and 1, 2 -> n+1, n+2; reflective negates the
 // normal momentum; inflow ghosts are constant and left alone.
-template <int ND>
+template <int ND, typename R = double>
 __device__ __forceinline__ void write_ghost_images(const Geom& G, const BcTypes& bc,
-                                                   double* __restrict__ cells, int i, int j,
-                                                   int k, const double* u) {
+                                                   R* __restrict__ cells, int i, int j, int k,
+                                                   const R* u) {
   int idx[3] = {i, j, k};
 #pragma unroll
   for (int a = 0; a < ND; ++a) {
@@ -81,7 +92,7 @@ __device__ __forceinline__ void write_ghost_images(const Geom& G, const BcTypes&
         int64_t o = goff(G, gi[0], gi[1], gi[2]);
 #pragma unroll
         for (int c = 0; c < ND + 2; ++c) {
-          double v = u[c];
+          R v = u[c];
           if (type == 1 && c == 1 + a) v = -v;
           cells[o + c * G.plane] = v;
         }
@@ -124,46 +135,72 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 }
 
 // Intercell flux on every face normal to axis A that the interior update
-// reads: face f_A in [0, n_A] sits between cells f_A and f_A+1, the other
-// indices run over the interior (the faces-only range the update consumes,
-// rhyme_muscl_hancock_solve_memory_intensive.f90:142-176).  Per face: the
-// half-step states of the two adjacent cells (loop A, :49-98) and the
-// Riemann flux (loop B, :100-140).  The flux is stored at the low cell's
-// offset in F (NV planes).
-template <int ND, int A>
-__global__ __launch_bounds__(256) void flux_kernel(Consts k, Geom G,
-                                                   const double* __restrict__ cells,
-                                                   double* __restrict__ F,
+// reads: face f in [0, n_A] sits between cells f and f+1, the other indices
+// run over the interior (the faces-only range the update consumes,
+// rhyme_muscl_hancock_solve_memory_intensive.f90:142-176).  Each thread
+// walks NF consecutive faces along A with a sliding window of cells: the
+// half-step states of every cell on the walk (loop A, :49-98) are formed
+// once and shared by its two faces, then each face gets its Riemann flux
+// (loop B, :100-140), stored at the low cell's offset in F (NV planes).
+//
+// R is the storage/bulk type (double: the FP64 drop-in; float: the mixed
+// form, whose half step runs in FP32 and whose Riemann solve is promoted to
+// FP64 with the PowFast exponentials).  LIM is the slope limiter (slid).
+template <int ND, int A, int LIM, typename R, class PW, int NF>
+__global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(Consts k, Geom G,
+                                                   const R* __restrict__ cells,
+                                                   R* __restrict__ F,
                                                    const double* __restrict__ dtp, double dx) {
   constexpr int NV = ND + 2;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int e0 = A == 0 ? G.n[0] + 1 : G.n[0];
-  const int e1 = A == 1 ? G.n[1] + 1 : G.n[1];
-  const int e2 = A == 2 ? G.n[2] + 1 : G.n[2];
+  const int nA = G.n[A];
+  const int nchunk = (nA + 1 + NF - 1) / NF;
+  const int e0 = A == 0 ? nchunk : G.n[0];
+  const int e1 = A == 1 ? nchunk : G.n[1];
+  const int e2 = A == 2 ? nchunk : G.n[2];
   if (t >= (int64_t)e0 * e1 * e2) return;
-  int i = (int)(t % e0);
-  int64_t r = t / e0;
-  int j = (int)(r % e1);
-  int kk = (int)(r / e1);
-  // faces index from 0 along A, cells from 1 elsewhere
-  if (A != 0) i += 1;
-  if (A != 1) j += 1;
-  if (A != 2) kk += 1;
-  const int64_t o = goff(G, i, j, kk);
+  int idx[3];
+  idx[0] = (int)(t % e0);
+  const int64_t rr = t / e0;
+  idx[1] = (int)(rr % e1);
+  idx[2] = (int)(rr / e1);
+  const int f0 = idx[A] * NF;
+  const int nf = nA + 1 - f0 < NF ? nA + 1 - f0 : NF;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) idx[d] = d == A ? f0 : idx[d] + 1;
+  const int64_t o = goff(G, idx[0], idx[1], idx[2]);
   const int64_t s = G.s[A];
   const double dt = *dtp;
-  const double hdtdx = .5 * dt / dx;
-  double um[NV], u0[NV], u1[NV], up[NV];
-  load_state<ND>(cells, G.plane, o - s, um);
-  load_state<ND>(cells, G.plane, o, u0);
-  load_state<ND>(cells, G.plane, o + s, u1);
-  load_state<ND>(cells, G.plane, o + 2 * s, up);
-  double l0[NV], r0[NV], l1[NV], r1[NV], f[NV];
-  half_step<ND, A>(k, um, u0, u1, hdtdx, l0, r0);
-  half_step<ND, A>(k, u0, u1, up, hdtdx, l1, r1);
-  riemann_flux<ND, A>(k, r0, l1, f);
+  const R hdtdx = (R)(.5 * dt / dx);
+  R um[NV], u0[NV], up[NV], lw[NV], rp[NV], rn[NV];
+  load_state<ND, R>(cells, G.plane, o - s, um);
+  load_state<ND, R>(cells, G.plane, o, u0);
+  load_state<ND, R>(cells, G.plane, o + s, up);
+  half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rp);
+  // Only rp (the right state of the low cell) is carried across the
+  // Riemann solve; the three cells of the next half step are re-read
+  // (cache hits), which keeps the register count near the single-face one.
+#pragma unroll 1
+  for (int m = 0; m < nf; ++m) {
+    const int64_t oc = o + (m + 1) * s;
+    load_state<ND, R>(cells, G.plane, oc - s, um);
+    load_state<ND, R>(cells, G.plane, oc, u0);
+    load_state<ND, R>(cells, G.plane, oc + s, up);
+    half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+    double Ld[NV], Rd[NV], Fd[NV];
 #pragma unroll
-  for (int c = 0; c < NV; ++c) F[o + c * G.plane] = f[c];
+    for (int c = 0; c < NV; ++c) {
+      Ld[c] = (double)rp[c];
+      Rd[c] = (double)lw[c];
+    }
+    riemann_flux<ND, A, PW>(k, Ld, Rd, Fd);
+    const int64_t of = o + m * s;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      F[of + c * G.plane] = (R)Fd[c];
+      rp[c] = rn[c];
+    }
+  }
 }
 
 // Conservative update of every interior cell from the stored fluxes.
@@ -173,11 +210,11 @@ __global__ __launch_bounds__(256) void flux_kernel(Consts k, Geom G,
 //   ((u + dt/dx_x*(...)) + dt/dx_y*(...)) + dt/dx_z*(...)
 // FUSED: also writes the new state's ghost images for the next step and
 // folds its CFL speeds into maxu_next.
-template <int ND, int ORDER, bool FUSED>
-__global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* __restrict__ cells,
-                                                     const double* __restrict__ F0,
-                                                     const double* __restrict__ F1,
-                                                     const double* __restrict__ F2,
+template <int ND, int ORDER, bool FUSED, typename R = double>
+__global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, R* __restrict__ cells,
+                                                     const R* __restrict__ F0,
+                                                     const R* __restrict__ F1,
+                                                     const R* __restrict__ F2,
                                                      const double* __restrict__ dtp, double dx0,
                                                      double dx1, double dx2, BcTypes bc,
                                                      double* __restrict__ partial) {
@@ -192,17 +229,17 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* _
     const int kk = (int)(r / G.n[1]) + 1;
     const int64_t o = goff(G, i, j, kk);
     const double dt = *dtp;
-    const double dtdx0 = dt / dx0;
-    const double dtdx1 = ND > 1 ? dt / dx1 : 0.0;
-    const double dtdx2 = ND > 2 ? dt / dx2 : 0.0;
-    double u[NV];
-    load_state<ND>(cells, G.plane, o, u);
+    const R dtdx0 = (R)(dt / dx0);
+    const R dtdx1 = ND > 1 ? (R)(dt / dx1) : R(0);
+    const R dtdx2 = ND > 2 ? (R)(dt / dx2) : R(0);
+    R u[NV];
+    load_state<ND, R>(cells, G.plane, o, u);
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
       const int64_t oc = o + c * G.plane;
-      double a0 = dtdx0 * (F0[oc - G.s[0]] - F0[oc]);
+      R a0 = dtdx0 * (F0[oc - G.s[0]] - F0[oc]);
       if (ORDER == 2) {
-        double acc = a0;
+        R acc = a0;
         if (ND > 1) acc = acc + dtdx1 * (F1[oc - G.s[1]] - F1[oc]);
         if (ND > 2) acc = acc + dtdx2 * (F2[oc - G.s[2]] - F2[oc]);
         u[c] = u[c] + acc;
@@ -215,8 +252,11 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* _
 #pragma unroll
     for (int c = 0; c < NV; ++c) cells[o + c * G.plane] = u[c];
     if (FUSED) {
-      write_ghost_images<ND>(G, bc, cells, i, j, kk, u);
-      speed = cfl_speed<ND>(k, u);
+      write_ghost_images<ND, R>(G, bc, cells, i, j, kk, u);
+      double ud[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) ud[c] = (double)u[c];
+      speed = cfl_speed<ND>(k, ud);
     }
   }
   if (FUSED) block_max_to_partial(speed, partial);
@@ -225,9 +265,9 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, double* _
 // Stand-alone ghost fill, one thread per face cell; blockIdx.y = face
 // (rhyme_samr_bc_set_boundaries.f90:9-20 calls the faces in this order; the
 // faces are disjoint and read interior cells only, so they run together).
-template <int ND>
+template <int ND, typename R = double>
 __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const double* __restrict__ inflow,
-                                                 int32_t ncmp, double* __restrict__ cells,
+                                                 int32_t ncmp, R* __restrict__ cells,
                                                  int32_t face_only) {
   const int face = face_only >= 0 ? face_only : (int)blockIdx.y;
   const int a = face / 2, side = face % 2;
@@ -250,8 +290,8 @@ __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const doubl
   const int64_t og1 = goff(G, gi[0], gi[1], gi[2]);
   if (type == 4) {
     for (int c = 0; c < ncmp; ++c) {
-      cells[og0 + c * G.plane] = inflow[face * 16 + c];
-      cells[og1 + c * G.plane] = inflow[face * 16 + c];
+      cells[og0 + c * G.plane] = (R)inflow[face * 16 + c];
+      cells[og1 + c * G.plane] = (R)inflow[face * 16 + c];
     }
     return;
   }
@@ -270,8 +310,8 @@ __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const doubl
   const int64_t os1 = goff(G, gi[0], gi[1], gi[2]);
 #pragma unroll
   for (int c = 0; c < ND + 2; ++c) {
-    double v0 = cells[os0 + c * G.plane];
-    double v1 = cells[os1 + c * G.plane];
+    R v0 = cells[os0 + c * G.plane];
+    R v1 = cells[os1 + c * G.plane];
     if (type == 1 && c == 1 + a) {
       v0 = -v0;
       v1 = -v1;
@@ -282,8 +322,8 @@ __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const doubl
 }
 
 // max over the interior of |v| + c_s (rhyme_cfl_time_step.f90:40-47).
-template <int ND>
-__global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const double* __restrict__ cells,
+template <int ND, typename R = double>
+__global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const R* __restrict__ cells,
                                                   double* __restrict__ partial) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
@@ -293,8 +333,11 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const double
     const int64_t r = t / G.n[0];
     const int j = (int)(r % G.n[1]) + 1;
     const int kk = (int)(r / G.n[1]) + 1;
+    R ur[ND + 2];
+    load_state<ND, R>(cells, G.plane, goff(G, i, j, kk), ur);
     double u[ND + 2];
-    load_state<ND>(cells, G.plane, goff(G, i, j, kk), u);
+#pragma unroll
+    for (int c = 0; c < ND + 2; ++c) u[c] = (double)ur[c];
     v = cfl_speed<ND>(k, u);
   }
   block_max_to_partial(v, partial);
@@ -373,7 +416,7 @@ __global__ __launch_bounds__(256) void limiter_batch_kernel(Consts k, int64_t n,
   if (q >= n) return;
 #pragma unroll
   for (int c = 0; c < NV; ++c)
-    d[q * NV + c] = limit_one(k.limiter, k.omega, ul[q * NV + c], u[q * NV + c], ur[q * NV + c]);
+    d[q * NV + c] = limit_one<0>(k.limiter, k.omega, ul[q * NV + c], u[q * NV + c], ur[q * NV + c]);
 }
 
 // half step from a given slope (the reference routine takes delta as input)

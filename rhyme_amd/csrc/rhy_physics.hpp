@@ -34,32 +34,35 @@ struct Consts {
   int32_t n_iter;        // irs%n_iteration
 };
 
-template <int ND>
-struct State {
-  double v[ND + 2];
-};
+// tiny() of the arithmetic type (tiny(0d0) / tiny(0e0))
+template <typename R>
+__device__ __forceinline__ R tiny_of();
+template <>
+__device__ __forceinline__ double tiny_of<double>() { return kTiny; }
+template <>
+__device__ __forceinline__ float tiny_of<float>() { return 1.17549435e-38f; }
 
 // ---------------------------------------------------------------- EOS ----
 
 // rhyme_hydro_base_specific_kinetic_energy.f90:9
-template <int ND>
-__device__ __forceinline__ double sp_kin_e(const double* u) {
-  double s = 0.0;
+template <int ND, typename R = double>
+__device__ __forceinline__ R sp_kin_e(const R* u) {
+  R s = R(0);
 #pragma unroll
   for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
-  return .5 * s / (u[0] * u[0]);
+  return R(.5) * s / (u[0] * u[0]);
 }
 
 // rhyme_hydro_base_specific_internal_energy.f90:9
-template <int ND>
-__device__ __forceinline__ double sp_int_e(const double* u) {
-  return u[ND + 1] / u[0] - sp_kin_e<ND>(u);
+template <int ND, typename R = double>
+__device__ __forceinline__ R sp_int_e(const R* u) {
+  return u[ND + 1] / u[0] - sp_kin_e<ND, R>(u);
 }
 
 // rhyme_ideal_gas_pressure.f90:9
-template <int ND>
-__device__ __forceinline__ double pressure(const Consts& k, const double* u) {
-  return u[0] * sp_int_e<ND>(u) * (k.g - 1.0);
+template <int ND, typename R = double>
+__device__ __forceinline__ R pressure(const Consts& k, const R* u) {
+  return u[0] * sp_int_e<ND, R>(u) * R(k.g - 1.0);
 }
 
 // rhyme_ideal_gas_speed_of_sound.f90:9 with temperature_per_mu.f90:9-10
@@ -70,14 +73,14 @@ __device__ __forceinline__ double speed_of_sound(const Consts& k, const double* 
 }
 
 // rhyme_ideal_gas_flux.f90:12-23; A is the 0-based axis.
-template <int ND, int A>
-__device__ __forceinline__ void flux(const Consts& k, const double* u, double* f) {
-  if (fabs(u[0]) < kTiny) {
+template <int ND, int A, typename R = double>
+__device__ __forceinline__ void flux(const Consts& k, const R* u, R* f) {
+  if (fabs(u[0]) < tiny_of<R>()) {
 #pragma unroll
-    for (int c = 0; c < ND + 2; ++c) f[c] = 0.0;
+    for (int c = 0; c < ND + 2; ++c) f[c] = R(0);
     return;
   }
-  double p = pressure<ND>(k, u);
+  R p = pressure<ND, R>(k, u);
   f[0] = u[1 + A];
 #pragma unroll
   for (int d = 0; d < ND; ++d) f[1 + d] = u[1 + d] * f[0] / u[0];
@@ -102,53 +105,59 @@ __device__ __forceinline__ void prim_to_cons(const Consts& k, double rho, const 
 
 __device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
+template <typename R>
+__device__ __forceinline__ R rmin(R a, R b) { return a < b ? a : b; }
 
 // rhyme_slope_limiter_run.f90:10-19 and the four limiters
 // (…_van_leer.f90:13-27, …_minmod.f90:13-29, …_van_albada.f90:13-27,
 // …_superbee.f90:13-31), r from rhyme_slope_limiter.f90:62-82 and
-// xi_R from …_xi_r.f90:11.
-__device__ __forceinline__ double limit_one(int type, double w, double ul, double u, double ur) {
-  double dL = u - ul;
-  double dR = ur - u;
-  double d = .5 * (1.0 + w) * dL + .5 * (1.0 + w) * dR;
-  double denom = ur - u;
-  double r = fabs(denom) < kTiny ? 0.0 : (u - ul) / denom;
-  if (!(r > 0.0)) return 0.0;  // every limiter gives 0 for r <= 0
-  double xi = 2.0 / (1.0 - w + (1.0 + w) * r);
-  switch (type) {
+// xi_R from …_xi_r.f90:11.  LIM selects the limiter at compile time
+// (1..4, slid values); LIM = 0 dispatches on `type` at run time.
+template <int LIM, typename R = double>
+__device__ __forceinline__ R limit_one(int type, R w, R ul, R u, R ur) {
+  const int t = LIM ? LIM : type;
+  R dL = u - ul;
+  R dR = ur - u;
+  R d = R(.5) * (R(1) + w) * dL + R(.5) * (R(1) + w) * dR;
+  R denom = ur - u;
+  R r = fabs(denom) < tiny_of<R>() ? R(0) : (u - ul) / denom;
+  if (!(r > R(0))) return R(0);  // every limiter gives 0 for r <= 0
+  R xi = R(2) / (R(1) - w + (R(1) + w) * r);
+  switch (t) {
     case 1:  // van Leer
-      return dmin(2.0 * r / (1.0 + r), xi) * d;
+      return rmin(R(2) * r / (R(1) + r), xi) * d;
     case 2:  // minmod
-      return r > 1.0 ? dmin(1.0, xi) * d : r * d;
+      return r > R(1) ? rmin(R(1), xi) * d : r * d;
     case 3:  // van Albada
-      return dmin(r * (1.0 + r) / (1.0 + r * r), xi) * d;
+      return rmin(r * (R(1) + r) / (R(1) + r * r), xi) * d;
     case 4:  // superbee
-      if (r > 1.0) return dmin(dmin(r, xi), 2.0) * d;
-      if (r > .5) return d;
-      return 2.0 * r * d;
+      if (r > R(1)) return rmin(rmin(r, xi), R(2)) * d;
+      if (r > R(.5)) return d;
+      return R(2) * r * d;
     default:
-      return 0.0;
+      return R(0);
   }
 }
 
 // rhyme_muscl_hancock_half_step_extrapolation.f90:15-21: slope from the
 // three cells um, u, up along axis A, then the half-step states l and r.
-template <int ND, int A>
-__device__ __forceinline__ void half_step(const Consts& k, const double* um, const double* u,
-                                          const double* up, double hdtdx, double* l, double* r) {
+template <int ND, int A, int LIM = 0, typename R = double>
+__device__ __forceinline__ void half_step(const Consts& k, const R* um, const R* u, const R* up,
+                                          R hdtdx, R* l, R* r) {
   constexpr int NV = ND + 2;
-  double fl[NV], fr[NV];
+  R fl[NV], fr[NV];
+  const R w = R(k.omega);
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
-    double delta = limit_one(k.limiter, k.omega, um[c], u[c], up[c]);
-    l[c] = u[c] - .5 * delta;
-    r[c] = u[c] + .5 * delta;
+    R delta = limit_one<LIM, R>(k.limiter, w, um[c], u[c], up[c]);
+    l[c] = u[c] - R(.5) * delta;
+    r[c] = u[c] + R(.5) * delta;
   }
-  flux<ND, A>(k, l, fl);
-  flux<ND, A>(k, r, fr);
+  flux<ND, A, R>(k, l, fl);
+  flux<ND, A, R>(k, r, fr);
 #pragma unroll
   for (int c = 0; c < NV; ++c) {
-    double df = fl[c] - fr[c];
+    R df = fl[c] - fr[c];
     l[c] = l[c] + hdtdx * df;  // hdtdx = (.5d0*dt)/dx, evaluated once
     r[c] = r[c] + hdtdx * df;
   }
@@ -156,7 +165,64 @@ __device__ __forceinline__ void half_step(const Consts& k, const double* um, con
 
 // -------------------------------------------------- Riemann (IRS) ----
 
+// x**y as the reference evaluates it (libm pow; ocml on the device).
+struct PowLib {
+  __device__ __forceinline__ static double pw(double x, double y) { return pow(x, y); }
+};
+
+// x**y = exp(y log x) in plain FP64 for x > 0 (relative error a few 1e-16,
+// not correctly rounded), for the mixed-precision form: log via
+// 2 atanh((m-1)/(m+1)) on m in [1/sqrt2, sqrt2), exp via a Cody-Waite
+// reduction and a degree-13 Taylor polynomial; other x go to pow().
+struct PowFast {
+  __device__ __forceinline__ static double pw(double x, double y) {
+    if (!(x > 0.0) || !(x < INFINITY)) return pow(x, y);
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0.70710678118654752440) {
+      m = m * 2.0;
+      e -= 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double z = s * s;
+    double q = 2.0 / 23.0;
+    q = fma(q, z, 2.0 / 21.0);
+    q = fma(q, z, 2.0 / 19.0);
+    q = fma(q, z, 2.0 / 17.0);
+    q = fma(q, z, 2.0 / 15.0);
+    q = fma(q, z, 2.0 / 13.0);
+    q = fma(q, z, 2.0 / 11.0);
+    q = fma(q, z, 2.0 / 9.0);
+    q = fma(q, z, 2.0 / 7.0);
+    q = fma(q, z, 2.0 / 5.0);
+    q = fma(q, z, 2.0 / 3.0);
+    const double LN2_HI = 6.93147180369123816490e-01;
+    const double LN2_LO = 1.90821492927058770002e-10;
+    const double lm = fma(s * z, q, 2.0 * s);
+    const double de = (double)e;
+    const double t = y * (de * LN2_HI + (lm + de * LN2_LO));
+    if (!(fabs(t) < 700.0)) return pow(x, y);
+    const double kd = rint(t * 1.44269504088896338700e+00);
+    const double r = (t - kd * LN2_HI) - kd * LN2_LO;
+    double p = 1.0 / 6227020800.0;  // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r * r, r);
+    return ldexp(1.0 + p, (int)kd);
+  }
+};
+
 // rhyme_irs_nonlinear_wave_function.f90:13-23
+template <class PW = PowLib>
 __device__ __forceinline__ void wave_fn(const Consts& k, double rho, double pk, double cs,
                                         double p, double& f, double& fp) {
   if (p > pk) {
@@ -166,13 +232,14 @@ __device__ __forceinline__ void wave_fn(const Consts& k, double rho, double pk, 
     f = (p - pk) * factor;
     fp = factor * (1.0 - (p - pk) / (2.0 * (Bk + p)));
   } else {
-    f = 2.0 * cs / k.gm1 * (pow(p / pk, k.gm1_2g) - 1.0);
-    fp = 1.0 / (rho * cs) * pow(p / pk, -k.gp1_2g);
+    f = 2.0 * cs / k.gm1 * (PW::pw(p / pk, k.gm1_2g) - 1.0);
+    fp = 1.0 / (rho * cs) * PW::pw(p / pk, -k.gp1_2g);
   }
 }
 
 // rhyme_irs_guess_p_star.f90 guess 1 (:14-39): PVRS / two-rarefaction /
 // two-shock selection.
+template <class PW = PowLib>
 __device__ __forceinline__ double guess1(const Consts& k, double lr, double lv, double lp,
                                          double lc, double rr, double rv, double rp, double rc) {
   double cup = 0.25 * (lr + rr) * (lc + rc);
@@ -184,11 +251,11 @@ __device__ __forceinline__ double guess1(const Consts& k, double lr, double lv, 
   if (q_max <= 2.0 && (p_min < ppv && ppv < p_max)) {
     g = ppv;
   } else if (ppv < p_min) {
-    double pq = pow(lp / rp, k.gm1_2g);
+    double pq = PW::pw(lp / rp, k.gm1_2g);
     double um = (pq * lv / lc + rv / rc + 2.0 / k.gm1 * (pq - 1.0)) / (pq / lc + 1.0 / rc);
     double ptl = 1.0 + k.gm1 / 2.0 * (lv - um) / lc;
     double ptr = 1.0 + k.gm1 / 2.0 * (um - rv) / rc;
-    g = 0.5 * (lp * pow(ptl, 1.0 / k.gm1_2g) + rp * pow(ptr, 1.0 / k.gm1_2g));
+    g = 0.5 * (lp * PW::pw(ptl, 1.0 / k.gm1_2g) + rp * PW::pw(ptr, 1.0 / k.gm1_2g));
   } else {
     double gel = sqrt(((2.0 / (k.gp1)) / lr) / (k.gm1_gp1 * lp + ppv));
     double ger = sqrt(((2.0 / (k.gp1)) / rr) / (k.gm1_gp1 * rp + ppv));
@@ -201,6 +268,7 @@ __device__ __forceinline__ double guess1(const Consts& k, double lr, double lv, 
 // computes all six; they are only read when the Newton iteration from the
 // previous guess ends at p* <= 0 (rhyme_irs_exec.f90:24-42), so computing
 // guess i on demand yields the same values.
+template <class PW = PowLib>
 __device__ __noinline__ double guess_n(const Consts& k, int i, double lr, double lv, double lp,
                                        double lc, double rr, double rv, double rp, double rc) {
   double g;
@@ -209,9 +277,9 @@ __device__ __noinline__ double guess_n(const Consts& k, int i, double lr, double
       g = (rr * rc * lp + lr * lc * rp + rc * lc * (lr * lv - rr * rv)) / (rr * rc + lr * lc);
       break;
     case 2:
-      g = pow((lc + rc - .5 * k.gm1 * (rv - lv)) /
-                  (pow(lc / lp, k.gm1_2g) + pow(rc / rp, k.gm1_2g)),
-              1.0 / k.gm1_2g);
+      g = PW::pw((lc + rc - .5 * k.gm1 * (rv - lv)) /
+                     (PW::pw(lc / lp, k.gm1_2g) + PW::pw(rc / rp, k.gm1_2g)),
+                 1.0 / k.gm1_2g);
       break;
     case 3: {
       double lg = sqrt((2.0 / (k.gp1 * lr)) / (lp + k.gm1_gp1 * lp));
@@ -231,14 +299,15 @@ __device__ __noinline__ double guess_n(const Consts& k, int i, double lr, double
 
 // Newton iteration from one starting guess (rhyme_irs_exec.f90:25-39).
 // Leaves lf/rf at the wave-function values of the last evaluated iterate.
+template <class PW = PowLib>
 __device__ __forceinline__ double newton(const Consts& k, double p_star, double lr, double lp,
                                          double lc, double rr, double rp, double rc, double du,
                                          double& lf, double& rf) {
   double p_prev = p_star;
   for (int i = 1; i <= k.n_iter; ++i) {
     double lfp, rfp;
-    wave_fn(k, lr, lp, lc, p_star, lf, lfp);
-    wave_fn(k, rr, rp, rc, p_star, rf, rfp);
+    wave_fn<PW>(k, lr, lp, lc, p_star, lf, lfp);
+    wave_fn<PW>(k, rr, rp, rc, p_star, rf, rfp);
     p_star = p_prev - (lf + rf + du) / (lfp + rfp);
     if (p_star < 0.0) break;
     if (2.0 * fabs(p_star - p_prev) / (p_star + p_prev) < k.tol) break;
@@ -248,16 +317,17 @@ __device__ __forceinline__ double newton(const Consts& k, double p_star, double 
 }
 
 // rhyme_irs_exec.f90:3-45
+template <class PW = PowLib>
 __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv, double lp,
                                            double lc, double rr, double rv, double rp, double rc,
                                            double& lf, double& rf) {
   const double du = rv - lv;
-  double p = newton(k, guess1(k, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp, rc, du,
-                    lf, rf);
+  double p = newton<PW>(k, guess1<PW>(k, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp,
+                        rc, du, lf, rf);
   if (!(p > 0.0)) {
     for (int gi = 1; gi < 6; ++gi) {
-      p = newton(k, guess_n(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp, rc, du,
-                 lf, rf);
+      p = newton<PW>(k, guess_n<PW>(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp,
+                     rc, du, lf, rf);
       if (p > 0.0) break;
     }
   }
@@ -266,16 +336,16 @@ __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv
 }
 
 // rhyme_riemann_problem_w_kfan.f90:14-31 (is_right selects the sign of cs)
-template <int ND, int A>
+template <int ND, int A, class PW = PowLib>
 __device__ __forceinline__ void w_kfan(const Consts& k, double srho, const double* sv, double sp,
                                        double scs, double dxdt, bool is_right, double* u) {
   double cs = is_right ? -scs : scs;
   double v[ND];
 #pragma unroll
   for (int d = 0; d < ND; ++d) v[d] = sv[d];
-  double rho = srho * pow(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1);
+  double rho = srho * PW::pw(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1);
   v[A] = 2.0 / k.gp1 * (cs + k.gm1 / 2.0 * sv[A] + dxdt);
-  double p = sp * pow(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 1.0 / k.gm1_2g);
+  double p = sp * PW::pw(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 1.0 / k.gm1_2g);
   prim_to_cons<ND>(k, rho, v, p, u);
 }
 
@@ -292,7 +362,7 @@ __device__ __forceinline__ void vacuum_state(const Consts& k, double* u) {
 // the conserved state u sampled on the interface.  l is the right half-step
 // state of the cell on the low side of the interface, r the left half-step
 // state of the cell on the high side.
-template <int ND, int A>
+template <int ND, int A, class PW = PowLib>
 __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, const double* r,
                                               double* u) {
   const double dxdt = 0.0;  // dx/dt with dx = 0 and dt > 0
@@ -313,7 +383,7 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
     if (dxdt > s_starl)
       vacuum_state<ND>(k, u);
     else if (dxdt > lv[A] - lc)
-      w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+      w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
     else
       prim_to_cons<ND>(k, l_rho, lv, lp, u);
   } else if (vac_l) {
@@ -325,7 +395,7 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
     if (dxdt > rv[A] + rc)
       prim_to_cons<ND>(k, r_rho, rv, rp, u);
     else if (dxdt > s_starr)
-      w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+      w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
     else
       vacuum_state<ND>(k, u);
   } else if (vac_b) {
@@ -348,19 +418,19 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
         if (dxdt > rv[A] + rc)
           prim_to_cons<ND>(k, r_rho, rv, rp, u);
         else
-          w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+          w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
       } else if (dxdt > s_starl) {
         vacuum_state<ND>(k, u);
       } else {
         if (dxdt > lv[A] - lc)
-          w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+          w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
         else
           prim_to_cons<ND>(k, l_rho, lv, lp, u);
       }
     } else {
       // star state: rhyme_riemann_problem_star.f90:3-66
       double lf = 0.0, rf = 0.0;
-      double ps = irs_exec(k, l_rho, lv[A], lp, lc, r_rho, rv[A], rp, rc, lf, rf);
+      double ps = irs_exec<PW>(k, l_rho, lv[A], lp, lc, r_rho, rv[A], rp, rc, lf, rf);
       if (ps < k.w_p) ps = k.w_p;
       double us = 0.5 * ((rv[A] + lv[A]) + (rf - lf));
       // sampling at x/t = 0: rhyme_riemann_problem_sampling.f90:13-47
@@ -379,18 +449,18 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
             prim_to_cons<ND>(k, r_rho, rv, rp, u);
           }
         } else {
-          double fan_cs = rc * pow(ps_pr, k.gm1_2g);
+          double fan_cs = rc * PW::pw(ps_pr, k.gm1_2g);
           double speedH = rv[A] + rc;
           double speedT = us + fan_cs;
           if (dxdt < speedT) {
-            double rho = r_rho * pow(ps_pr, k.g_inv);
+            double rho = r_rho * PW::pw(ps_pr, k.g_inv);
             double v[ND];
 #pragma unroll
             for (int d = 0; d < ND; ++d) v[d] = rv[d];
             v[A] = us;
             prim_to_cons<ND>(k, rho, v, ps, u);
           } else if (dxdt < speedH) {
-            w_kfan<ND, A>(k, r_rho, rv, rp, rc, dxdt, true, u);
+            w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
           } else {
             prim_to_cons<ND>(k, r_rho, rv, rp, u);
           }
@@ -410,18 +480,18 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
             prim_to_cons<ND>(k, l_rho, lv, lp, u);
           }
         } else {
-          double fan_cs = lc * pow(ps_pl, k.gm1_2g);
+          double fan_cs = lc * PW::pw(ps_pl, k.gm1_2g);
           double speedH = lv[A] - lc;
           double speedT = us - fan_cs;
           if (dxdt > speedT) {
-            double rho = l_rho * pow(ps_pl, k.g_inv);
+            double rho = l_rho * PW::pw(ps_pl, k.g_inv);
             double v[ND];
 #pragma unroll
             for (int d = 0; d < ND; ++d) v[d] = lv[d];
             v[A] = us;
             prim_to_cons<ND>(k, rho, v, ps, u);
           } else if (dxdt > speedH) {
-            w_kfan<ND, A>(k, l_rho, lv, lp, lc, dxdt, false, u);
+            w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
           } else {
             prim_to_cons<ND>(k, l_rho, lv, lp, u);
           }
@@ -433,11 +503,11 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
 
 // Riemann solve + calc_flux of the sampled state
 // (…_solve_memory_intensive.f90:110-116): the intercell flux.
-template <int ND, int A>
+template <int ND, int A, class PW = PowLib>
 __device__ __forceinline__ void riemann_flux(const Consts& k, const double* l, const double* r,
                                              double* F) {
   double u[ND + 2];
-  riemann_state<ND, A>(k, l, r, u);
+  riemann_state<ND, A, PW>(k, l, r, u);
   flux<ND, A>(k, u, F);
 }
 

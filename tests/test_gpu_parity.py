@@ -244,3 +244,61 @@ def test_blast_3d_trajectory():
         say("  var", c, "rel L1", e)
         assert e <= REL_L1
     assert np.max(np.abs(dts - rdts) / rdts) <= 1e-12
+
+
+# ------------------------------------------------------- mixed precision ----
+
+MIXED_REL_L1 = 1e-5
+
+
+def run_mixed(ndim, dims, cells, p, dx, nsteps):
+    box = hydro.SamrBox(ndim, dims, cells=cells.copy(), precision=hydro.MIXED)
+    box.upload()
+    dts = hydro.run(p, box, dx, nsteps)
+    return box.download(), dts
+
+
+def test_mixed_blast_3d_vs_oracle():
+    """Mixed form (FP32 state, half step and flux differencing; FP64 Riemann
+    p*, sampling and dt) against the FP64 oracle: <= 1e-5 relative L1."""
+    say("mixed 3D blast")
+    n = 32
+    cells = ics.blast_3d(dims=[n, n, n], seed=3)
+    p = params(3, limiter=hydro.SUPERBEE, bc=[hydro.REFLECTIVE] * 6, courant=0.2)
+    out, dts = run_mixed(3, [n] * 3, cells, p, [1.0 / n] * 3, 20)
+    ref = cells.copy()
+    rdts = oracle.run(oracle.params_from(p), [n] * 3, ref, [1.0 / n] * 3, 20)
+    a, b = interior(3, out), interior(3, ref)
+    for c in range(5):
+        e = rel_l1(a[c], b[c])
+        say("  var", c, "rel L1", e)
+        assert e <= MIXED_REL_L1
+    assert np.max(np.abs(dts - rdts) / rdts) <= 1e-5
+
+
+@pytest.mark.parametrize("limiter", [1, 2, 3, 4])
+def test_mixed_smooth_3d_vs_oracle(limiter):
+    say("mixed smooth 3D limiter", limiter)
+    dims = [12, 10, 14]
+    cells = ics.random_smooth(3, dims, seed=31 + limiter)
+    p = params(3, limiter=limiter, bc=[1, 2, 3, 3, 2, 1], courant=0.2)
+    out, _ = run_mixed(3, dims, cells, p, [1.0 / 16] * 3, 10)
+    ref = cells.copy()
+    oracle.run(oracle.params_from(p), dims, ref, [1.0 / 16] * 3, 10)
+    for c in range(5):
+        assert rel_l1(interior(3, out)[c], interior(3, ref)[c]) <= MIXED_REL_L1
+
+
+def test_mixed_2d_conservation():
+    say("mixed 2D periodic conservation")
+    n = 64
+    cells = ics.spheres_2d(n, seed=2, nspheres=4)
+    p = params(2, limiter=hydro.MINMOD, bc=[hydro.PERIODIC] * 6, courant=0.2)
+    out, _ = run_mixed(2, [n, n], cells, p, [1.0 / n] * 2, 40)
+    a = interior(2, out)
+    c0 = interior(2, cells)
+    for c in (0, 3):
+        s0 = np.sum(c0[c], dtype=np.float64)
+        s1 = np.sum(a[c], dtype=np.float64)
+        say("  var", c, "drift", (s1 - s0) / s0)
+        assert abs(s1 - s0) / s0 < 1e-5
