@@ -1,0 +1,865 @@
+! rhyme_host.f90 -- Fortran host modules of the hydro hot path.
+!
+! These modules keep the names, derived types, defaults and procedure
+! argument lists of the reference's hot-path interfaces so that a Rhyme
+! driver (src/main.f90:106-177) links against them unchanged; every
+! procedure body calls the gfx950 library through rhy_capi (ISO_C_BINDING).
+! File:line citations are relative to the reference's src/.
+!
+! Residency: the first call that touches a box creates its device copy from
+! box%cells; from then on the device copy is authoritative.  After writing
+! box%cells on the host call rhyme_samr_upload(samr); before reading them
+! call rhyme_samr_download(samr) (the reference's host readers -- chombo
+! output, report, sanity check, stabilizer, src/main.f90:122-155 -- are the
+! places that need it).
+!
+! Compile with -cpp -DNDIM=<1|2|3> [-DNSPE=3], as the reference is
+! (config.cmake:8-18).
+
+#ifndef NSPE
+#define NSPE 3
+#endif
+#define NCMP (1 + NDIM + 1 + 1 + NSPE)
+
+module rhyme_units
+   ! component indices of rhyme_units/src/rhyme_units.f90:32-57
+   implicit none
+
+   type, private :: component_indices_t
+      integer :: rho = 1
+      integer :: rho_u = 2, u = 2
+#if NDIM > 1
+      integer :: rho_v = 3, v = 3
+#endif
+#if NDIM > 2
+      integer :: rho_w = 4, w = 4
+#endif
+      integer :: e_tot = 1 + NDIM + 1, p = 1 + NDIM + 1
+      integer :: temp = 1 + NDIM + 1 + 1
+      integer :: ntr_frac_0 = 1 + NDIM + 1 + 1 + 1
+   end type component_indices_t
+
+   type(component_indices_t), parameter :: cid = component_indices_t()
+
+   ! only the values of kB and amu in code units cross into the hot path
+   ! (rhyme_thermo_base.f90:130)
+   type nombre_t
+      real(kind=8) :: v = 0d0
+   end type nombre_t
+
+   type units_t
+      type(nombre_t) :: kb, amu
+   end type units_t
+end module rhyme_units
+
+module rhyme_logger
+   ! Minimal logger with the calls the hot path makes (logger_t,
+   ! rhyme_logger/src/rhyme_logger.f90:24-68): begin/end_section, log, warn, err.
+   implicit none
+
+   type logger_t
+      integer :: secid = 0
+      logical :: quiet = .true.
+      integer :: nerr = 0
+      character(len=1024) :: last_err = ''
+   contains
+      procedure :: begin_section => logger_begin_section
+      procedure :: end_section => logger_end_section
+      procedure :: log => logger_log
+      procedure :: warn => logger_warn
+      procedure :: err => logger_err
+   end type logger_t
+
+contains
+
+   subroutine logger_begin_section(logger, name)
+      class(logger_t), intent(inout) :: logger
+      class(*), intent(in) :: name
+      logger%secid = logger%secid + 1
+   end subroutine logger_begin_section
+
+   subroutine logger_end_section(logger, print_duration)
+      class(logger_t), intent(inout) :: logger
+      logical, intent(in), optional :: print_duration
+      logger%secid = max(0, logger%secid - 1)
+   end subroutine logger_end_section
+
+   subroutine logger_log(logger, message)
+      class(logger_t), intent(inout) :: logger
+      character(len=*), intent(in) :: message
+      if (.not. logger%quiet) print '(A)', trim(message)
+   end subroutine logger_log
+
+   subroutine logger_warn(logger, message)
+      class(logger_t), intent(inout) :: logger
+      character(len=*), intent(in) :: message
+      print '(A,A)', 'warning: ', trim(message)
+   end subroutine logger_warn
+
+   subroutine logger_err(logger, message)
+      class(logger_t), intent(inout) :: logger
+      character(len=*), intent(in) :: message
+      logger%nerr = logger%nerr + 1
+      logger%last_err = message
+      write (0, '(A,A)') 'error: ', trim(message)
+   end subroutine logger_err
+end module rhyme_logger
+
+module rhyme_thermo_base
+   ! rhyme_thermo_base/src/rhyme_thermo_base.f90:8-26,118-134
+   use rhyme_units
+   use rhyme_logger
+   implicit none
+
+   type, private :: thermo_base_indices_t
+      integer :: unset = -1
+      integer :: monatomic = 1
+      integer :: diatomic = 2
+      integer :: polyatomic = 3
+   end type thermo_base_indices_t
+
+   type(thermo_base_indices_t), parameter :: thid = thermo_base_indices_t()
+
+   real(kind=8), parameter :: ig_gamma(thid%monatomic:thid%polyatomic) = &
+                              [5.d0/3.d0, 7.d0/5.d0, 4.d0/3.d0]
+
+   integer, private :: state_of_matter = -1
+   real(kind=8), private :: kb_amu = 0.d0
+
+   type thermo_base_t
+      integer :: state_of_matter = -1
+   end type thermo_base_t
+
+contains
+
+   subroutine rhyme_thermo_base_init(thermo, units, logger)
+      type(thermo_base_t), intent(in) :: thermo
+      type(units_t), intent(in) :: units
+      type(logger_t), intent(inout) :: logger
+      state_of_matter = thermo%state_of_matter
+      kb_amu = units%kb%v/units%amu%v
+      call logger%log('thermo_base initialised')
+   end subroutine rhyme_thermo_base_init
+
+   pure function get_gamma() result(g)
+      real(kind=8) :: g
+      if (state_of_matter >= thid%monatomic .and. state_of_matter <= thid%polyatomic) then
+         g = ig_gamma(state_of_matter)
+      else
+         g = -huge(0d0)
+      end if
+   end function get_gamma
+
+   pure function get_kb_amu() result(v)
+      real(kind=8) :: v
+      v = kb_amu
+   end function get_kb_amu
+end module rhyme_thermo_base
+
+module rhyme_samr
+   ! rhyme_samr/src/rhyme_samr.f90:23-65 and rhyme_samr_init_box.f90:29-33
+   use rhyme_units
+   implicit none
+
+   type, private :: samr_indices_t
+      integer :: ghost = -1
+      integer :: unset = -10
+      integer :: max_nlevels = 23
+      integer :: x = 1, left = 1, right = 2
+#if NDIM > 1
+      integer :: y = 2, bottom = 3, top = 4
+#endif
+#if NDIM > 2
+      integer :: z = 3, back = 5, front = 6
+#endif
+   end type samr_indices_t
+
+   type(samr_indices_t), parameter :: samrid = samr_indices_t()
+
+   type samr_box_t
+      integer :: dims(NDIM) = samrid%unset
+      integer :: level = samrid%unset
+      integer :: number = samrid%unset
+      integer :: left_edge(NDIM) = samrid%unset
+      integer :: right_edge(NDIM) = samrid%unset
+#if NDIM == 1
+      integer, allocatable :: flags(:)
+      real(kind=8), allocatable :: cells(:, :)
+#elif NDIM == 2
+      integer, allocatable :: flags(:, :)
+      real(kind=8), allocatable :: cells(:, :, :)
+#else
+      integer, allocatable :: flags(:, :, :)
+      real(kind=8), allocatable :: cells(:, :, :, :)
+#endif
+   end type samr_box_t
+
+   type samr_level_t
+      integer :: level = samrid%unset
+      integer :: nboxes = 0
+      integer :: max_nboxes = 0
+      integer :: iteration = 0
+      real(kind=8) :: refine_factor = 2d0
+      real(kind=8) :: t = 0.d0
+      real(kind=8) :: dt = 0.d0, dx(NDIM) = 0.d0
+      type(samr_box_t), allocatable :: boxes(:)
+   end type samr_level_t
+
+   type samr_t
+      logical :: initialized = .false.
+      integer :: nlevels = 0
+      integer :: base_grid(NDIM) = 0
+      integer :: ghost_cells(NDIM) = 2
+      integer :: max_nboxes(0:23) = 0
+      real(kind=8) :: box_lengths(NDIM) = 1d0
+      type(samr_level_t) :: levels(0:23)
+   end type samr_t
+
+contains
+
+   subroutine rhyme_samr_init_box(samr, l, b, dims, ledges, redges)
+      type(samr_t), intent(inout) :: samr
+      integer, intent(in) :: l, b, dims(NDIM), ledges(NDIM), redges(NDIM)
+      integer :: lb(NDIM), ub(NDIM)
+
+      samr%levels(l)%boxes(b)%level = l
+      samr%levels(l)%boxes(b)%number = b
+      lb = -samr%ghost_cells + 1
+      ub = dims + samr%ghost_cells
+#if NDIM == 1
+      allocate (samr%levels(l)%boxes(b)%flags(lb(1):ub(1)))
+      allocate (samr%levels(l)%boxes(b)%cells(lb(1):ub(1), NCMP))
+#elif NDIM == 2
+      allocate (samr%levels(l)%boxes(b)%flags(lb(1):ub(1), lb(2):ub(2)))
+      allocate (samr%levels(l)%boxes(b)%cells(lb(1):ub(1), lb(2):ub(2), NCMP))
+#else
+      allocate (samr%levels(l)%boxes(b)%flags(lb(1):ub(1), lb(2):ub(2), lb(3):ub(3)))
+      allocate (samr%levels(l)%boxes(b)%cells(lb(1):ub(1), lb(2):ub(2), lb(3):ub(3), NCMP))
+#endif
+      samr%levels(l)%boxes(b)%cells = 0d0
+      samr%levels(l)%nboxes = samr%levels(l)%nboxes + 1
+      samr%levels(l)%boxes(b)%dims = dims
+      samr%levels(l)%boxes(b)%left_edge = ledges
+      samr%levels(l)%boxes(b)%right_edge = redges
+   end subroutine rhyme_samr_init_box
+end module rhyme_samr
+
+module rhy_device
+   ! Registry of device copies, one per (level, box).  Host mirror sync is
+   ! explicit: rhyme_samr_upload / rhyme_samr_download.
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhyme_samr
+   implicit none
+
+   type rhy_slot_t
+      type(c_ptr) :: h = c_null_ptr
+      integer :: dims(NDIM) = -1
+   end type rhy_slot_t
+
+   type(rhy_slot_t), save, private :: slots(0:23, 1:1024)
+   integer(c_int32_t), save :: rhy_precision = RHY_FP64
+
+contains
+
+   ! Device handle of a box, creating (and uploading) it on first use or
+   ! after a shape change.  Returns c_null_ptr and a message on failure.
+   function rhy_box_handle(box, msg) result(h)
+      type(samr_box_t), intent(inout), target :: box
+      character(len=:), allocatable, intent(out) :: msg
+      type(c_ptr) :: h
+      integer(c_int32_t) :: d3(3)
+      integer(c_int) :: rc
+      integer :: l, b
+
+      msg = ''
+      h = c_null_ptr
+      l = box%level
+      b = box%number
+      if (l < 0 .or. l > 23 .or. b < 1 .or. b > 1024) then
+         msg = 'box level/number out of range'
+         return
+      end if
+      d3 = 1
+      d3(1:NDIM) = box%dims
+      if (c_associated(slots(l, b)%h)) then
+         if (all(slots(l, b)%dims == box%dims)) then
+            h = slots(l, b)%h
+            return
+         end if
+         rc = rhy_box_resize(slots(l, b)%h, d3)
+         if (rc /= RHY_OK) then
+            msg = rhy_error_text()
+            return
+         end if
+      else
+         rc = rhy_box_create(int(NDIM, c_int32_t), d3, int(NCMP, c_int32_t), rhy_precision, &
+                             slots(l, b)%h)
+         if (rc /= RHY_OK) then
+            msg = rhy_error_text()
+            slots(l, b)%h = c_null_ptr
+            return
+         end if
+      end if
+      slots(l, b)%dims = box%dims
+      rc = rhy_box_upload(slots(l, b)%h, c_loc(box%cells))
+      if (rc /= RHY_OK) then
+         msg = rhy_error_text()
+         return
+      end if
+      h = slots(l, b)%h
+   end function rhy_box_handle
+
+   subroutine rhyme_samr_box_upload(box)
+      type(samr_box_t), intent(inout), target :: box
+      type(c_ptr) :: h
+      character(len=:), allocatable :: msg
+      integer(c_int) :: rc
+      h = rhy_box_handle(box, msg)
+      if (c_associated(h)) rc = rhy_box_upload(h, c_loc(box%cells))
+   end subroutine rhyme_samr_box_upload
+
+   subroutine rhyme_samr_box_download(box)
+      type(samr_box_t), intent(inout), target :: box
+      type(c_ptr) :: h
+      character(len=:), allocatable :: msg
+      integer(c_int) :: rc
+      h = rhy_box_handle(box, msg)
+      if (c_associated(h)) rc = rhy_box_download(h, c_loc(box%cells))
+   end subroutine rhyme_samr_box_download
+
+   subroutine rhyme_samr_upload(samr)
+      type(samr_t), intent(inout) :: samr
+      integer :: l, b
+      do l = 0, samr%nlevels - 1
+         do b = 1, samr%levels(l)%nboxes
+            call rhyme_samr_box_upload(samr%levels(l)%boxes(b))
+         end do
+      end do
+   end subroutine rhyme_samr_upload
+
+   subroutine rhyme_samr_download(samr)
+      type(samr_t), intent(inout) :: samr
+      integer :: l, b
+      do l = 0, samr%nlevels - 1
+         do b = 1, samr%levels(l)%nboxes
+            call rhyme_samr_box_download(samr%levels(l)%boxes(b))
+         end do
+      end do
+   end subroutine rhyme_samr_download
+
+   subroutine rhy_device_release_all()
+      integer :: l, b
+      integer(c_int) :: rc
+      do l = 0, 23
+         do b = 1, 1024
+            if (c_associated(slots(l, b)%h)) then
+               rc = rhy_box_destroy(slots(l, b)%h)
+               slots(l, b)%h = c_null_ptr
+               slots(l, b)%dims = -1
+            end if
+         end do
+      end do
+   end subroutine rhy_device_release_all
+end module rhy_device
+
+module rhyme_slope_limiter
+   ! rhyme_slope_limiter/src/rhyme_slope_limiter.f90:6-23
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhyme_units
+   use rhyme_thermo_base
+   implicit none
+
+   type, private :: slope_limiter_indices_t
+      integer :: van_Leer = 1, minmod = 2, van_albada = 3, superbee = 4
+   end type slope_limiter_indices_t
+
+   type(slope_limiter_indices_t), parameter :: slid = slope_limiter_indices_t()
+
+   type slope_limiter_t
+      real(kind=8) :: w = 0.d0
+      integer :: type = slid%minmod
+   end type slope_limiter_t
+
+contains
+
+   subroutine rhyme_slope_limiter_run(sl, ul, u, ur, delta)
+      class(slope_limiter_t), intent(in) :: sl
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(in) :: ul, u, ur
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(out) :: delta
+      type(rhy_params_t) :: p
+      integer(c_int) :: rc
+      p%ndim = NDIM
+      p%limiter = sl%type
+      p%omega = sl%w
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%ncmp = NCMP
+      rc = rhy_limiter_batch(p, 1_c_int64_t, ul, u, ur, delta)
+      if (rc /= RHY_OK) delta = 0d0
+   end subroutine rhyme_slope_limiter_run
+end module rhyme_slope_limiter
+
+module rhyme_riemann_problem
+   ! rhyme_riemann_problem/src/rhyme_riemann_problem.f90:18-31 and
+   ! rhyme_irs/src/rhyme_irs.f90:16-19
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhyme_units
+   use rhyme_thermo_base
+   use rhyme_logger
+   implicit none
+
+   type, private :: riemann_problem_indices_t
+      integer :: exact_rs = 1, deep_rs = 2
+   end type riemann_problem_indices_t
+
+   type(riemann_problem_indices_t), parameter :: rpid = riemann_problem_indices_t()
+
+   type irs_t
+      integer :: n_iteration = 10000
+      real(kind=8) :: tolerance = 1d-8
+   end type irs_t
+
+   type riemann_problem_t
+      real(kind=8), dimension(cid%rho:cid%p) :: w_vacuum = 0d0
+      real(kind=8) :: tolerance = 1d-8
+      integer :: n_iteration = 1000, solver = rpid%exact_rs
+      character(len=1024) :: path = ""
+      type(irs_t) :: irs = irs_t()
+   end type riemann_problem_t
+
+contains
+
+   subroutine rhyme_riemann_problem_init(rp, units, thermo, logger)
+      type(riemann_problem_t), intent(inout) :: rp
+      type(units_t), intent(in) :: units
+      type(thermo_base_t), intent(in) :: thermo
+      type(logger_t), intent(inout) :: logger
+      integer(c_int) :: rc
+      if (rp%solver /= rpid%exact_rs) call logger%err('Unknown solver: only ExactRS is supported')
+      rc = rhy_init(0_c_int32_t)
+      if (rc /= RHY_OK) call logger%err('rhy_init: '//rhy_error_text())
+   end subroutine rhyme_riemann_problem_init
+
+   ! Godunov state of the interface problem.  The device samples x/t = 0,
+   ! which is what the MUSCL-Hancock loops request (dx = 0,
+   ! rhyme_muscl_hancock_solve_memory_intensive.f90:110-114); other dx are
+   ! reported through u = NaN.
+   subroutine rhyme_riemann_problem_solve(rp, l, r, dx, dt, axis, u)
+      type(riemann_problem_t), intent(in) :: rp
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(in) :: l, r
+      real(kind=8), intent(in) :: dx, dt
+      integer, intent(in) :: axis
+      real(kind=8), intent(inout) :: u(cid%rho:cid%e_tot)
+      type(rhy_params_t) :: p
+      integer(c_int32_t) :: ax(1)
+      integer(c_int) :: rc
+      if (dx /= 0d0 .or. .not. (dt > 0d0)) then
+         u = ieee_nan()
+         return
+      end if
+      call rhy_rp_params(rp, p)
+      ax(1) = axis
+      rc = rhy_riemann_batch(p, 1_c_int64_t, l, r, ax, u)
+      if (rc /= RHY_OK) u = ieee_nan()
+   end subroutine rhyme_riemann_problem_solve
+
+   subroutine rhy_rp_params(rp, p)
+      type(riemann_problem_t), intent(in) :: rp
+      type(rhy_params_t), intent(inout) :: p
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%n_iteration = rp%irs%n_iteration
+      p%tolerance = rp%irs%tolerance
+      p%w_vacuum_rho = rp%w_vacuum(cid%rho)
+      p%w_vacuum_p = rp%w_vacuum(cid%p)
+      p%ncmp = NCMP
+   end subroutine rhy_rp_params
+
+   function ieee_nan() result(x)
+      use, intrinsic :: ieee_arithmetic
+      real(kind=8) :: x
+      x = ieee_value(x, ieee_quiet_nan)
+   end function ieee_nan
+end module rhyme_riemann_problem
+
+module rhyme_cfl
+   ! rhyme_cfl/src/rhyme_cfl.f90:7-24.  rhyme_cfl_time_step is not pure
+   ! here: it reads the device copy of level-0 box 1.
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhy_device
+   use rhyme_samr
+   use rhyme_thermo_base
+   use rhyme_logger
+   implicit none
+
+   type cfl_t
+      real(kind=8) :: courant_number = .8d0
+   end type cfl_t
+
+contains
+
+   subroutine rhyme_cfl_init(cfl, thermo, samr, logger)
+      type(cfl_t), intent(inout) :: cfl
+      type(thermo_base_t), intent(in) :: thermo
+      type(samr_t), intent(in) :: samr
+      type(logger_t), intent(inout) :: logger
+      call logger%log('cfl initialised')
+   end subroutine rhyme_cfl_init
+
+   function rhyme_cfl_time_step(c, samr) result(dt)
+      real(kind=8), intent(in) :: c
+      type(samr_t), intent(inout) :: samr
+      real(kind=8) :: dt
+      type(rhy_params_t) :: p
+      type(c_ptr) :: h
+      character(len=:), allocatable :: msg
+      real(c_double) :: dx3(3), dtc
+      integer(c_int) :: rc
+      dt = -1d0
+      h = rhy_box_handle(samr%levels(0)%boxes(1), msg)
+      if (.not. c_associated(h)) return
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%courant = c
+      p%ncmp = NCMP
+      dx3 = 1d0
+      dx3(1:NDIM) = samr%levels(0)%dx
+      rc = rhy_cfl_time_step(h, p, dx3, dtc)
+      if (rc == RHY_OK) dt = dtc
+   end function rhyme_cfl_time_step
+end module rhyme_cfl
+
+module rhyme_samr_bc
+   ! rhyme_samr_bc/src/rhyme_samr_bc.f90:9-72; not pure here (device copy).
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhy_device
+   use rhyme_units
+   use rhyme_samr
+   use rhyme_thermo_base
+   use rhyme_logger
+   implicit none
+
+   type, private :: samr_bc_indices_t
+      integer :: unset = -1
+      integer :: reflective = 1, outflow = 2, periodic = 3, inflow = 4
+      integer :: left = 1, right = 2
+#if NDIM > 1
+      integer :: bottom = 3, top = 4
+#endif
+#if NDIM > 2
+      integer :: back = 5, front = 6
+#endif
+   end type samr_bc_indices_t
+
+   type(samr_bc_indices_t), parameter :: bcid = samr_bc_indices_t()
+
+   type samr_bc_t
+      integer :: types(2*NDIM) = bcid%reflective
+      real(kind=8) :: prim_inflows(NCMP, 2*NDIM) = 0d0
+      real(kind=8) :: cons_inflows(NCMP, 2*NDIM) = 0d0
+   end type samr_bc_t
+
+contains
+
+   ! rhyme_samr_bc_init.f90:53-87: inflow primitive -> conserved (non-hydro
+   ! components copied) and every ghost cell of vars 1..NDIM+2 zeroed (on
+   ! the host mirror, before the device copy exists or is re-uploaded).
+   subroutine rhyme_samr_bc_init(bc, thermo, samr, logger)
+      type(samr_bc_t), intent(inout) :: bc
+      type(thermo_base_t), intent(in) :: thermo
+      type(samr_t), intent(inout) :: samr
+      type(logger_t), intent(inout) :: logger
+      integer :: i, b, c
+      real(kind=8) :: g, s
+      g = get_gamma()
+      do i = 1, 2*NDIM
+         if (bc%types(i) == bcid%inflow) then
+            s = 0d0
+            do c = 0, NDIM - 1
+               s = s + bc%prim_inflows(cid%u + c, i)**2
+            end do
+            bc%cons_inflows(cid%rho, i) = bc%prim_inflows(cid%rho, i)
+            bc%cons_inflows(cid%rho_u:cid%rho_u + NDIM - 1, i) = &
+               bc%prim_inflows(cid%rho, i)*bc%prim_inflows(cid%u:cid%u + NDIM - 1, i)
+            bc%cons_inflows(cid%e_tot, i) = .5d0*bc%prim_inflows(cid%rho, i)*s &
+                                            + bc%prim_inflows(cid%p, i)/(g - 1.d0)
+            bc%cons_inflows(cid%e_tot + 1:NCMP, i) = bc%prim_inflows(cid%e_tot + 1:NCMP, i)
+         end if
+      end do
+      do b = 1, samr%levels(0)%nboxes
+         call zero_ghosts(samr%levels(0)%boxes(b))
+      end do
+      call logger%log('samr_bc initialised')
+   end subroutine rhyme_samr_bc_init
+
+   subroutine zero_ghosts(box)
+      type(samr_box_t), intent(inout) :: box
+      integer :: c
+      integer :: lo(NDIM), hi(NDIM)
+      lo = 1
+      hi = box%dims
+      do c = cid%rho, cid%e_tot
+#if NDIM == 1
+         box%cells(:lo(1) - 1, c) = 0d0
+         box%cells(hi(1) + 1:, c) = 0d0
+#elif NDIM == 2
+         box%cells(:lo(1) - 1, :, c) = 0d0
+         box%cells(hi(1) + 1:, :, c) = 0d0
+         box%cells(:, :lo(2) - 1, c) = 0d0
+         box%cells(:, hi(2) + 1:, c) = 0d0
+#else
+         box%cells(:lo(1) - 1, :, :, c) = 0d0
+         box%cells(hi(1) + 1:, :, :, c) = 0d0
+         box%cells(:, :lo(2) - 1, :, c) = 0d0
+         box%cells(:, hi(2) + 1:, :, c) = 0d0
+         box%cells(:, :, :lo(3) - 1, c) = 0d0
+         box%cells(:, :, hi(3) + 1:, c) = 0d0
+#endif
+      end do
+   end subroutine zero_ghosts
+
+   subroutine rhy_bc_params(bc, p)
+      type(samr_bc_t), intent(in) :: bc
+      type(rhy_params_t), intent(inout) :: p
+      integer :: f
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%ncmp = NCMP
+      p%bc_types = -1
+      p%bc_types(1:2*NDIM) = bc%types
+      do f = 1, 2*NDIM
+         p%cons_inflows(1:NCMP, f) = bc%cons_inflows(:, f)
+      end do
+   end subroutine rhy_bc_params
+
+   subroutine rhyme_samr_bc_set_boundaries(bc, samr)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_t), intent(inout) :: samr
+      call set_faces(bc, samr%levels(0)%boxes(1), -1)
+   end subroutine rhyme_samr_bc_set_boundaries
+
+   subroutine set_faces(bc, box, face)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      integer, intent(in) :: face
+      type(rhy_params_t) :: p
+      type(c_ptr) :: h
+      character(len=:), allocatable :: msg
+      integer(c_int) :: rc
+      h = rhy_box_handle(box, msg)
+      if (.not. c_associated(h)) return
+      call rhy_bc_params(bc, p)
+      if (face < 0) then
+         rc = rhy_bc_set_boundaries(h, p)
+      else
+         rc = rhy_bc_set_face(h, p, int(face, c_int32_t))
+      end if
+   end subroutine set_faces
+
+   subroutine rhyme_samr_bc_set_left_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 0)
+   end subroutine rhyme_samr_bc_set_left_boundary
+
+   subroutine rhyme_samr_bc_set_right_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 1)
+   end subroutine rhyme_samr_bc_set_right_boundary
+
+#if NDIM > 1
+   subroutine rhyme_samr_bc_set_bottom_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 2)
+   end subroutine rhyme_samr_bc_set_bottom_boundary
+
+   subroutine rhyme_samr_bc_set_top_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 3)
+   end subroutine rhyme_samr_bc_set_top_boundary
+#endif
+
+#if NDIM > 2
+   subroutine rhyme_samr_bc_set_back_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 4)
+   end subroutine rhyme_samr_bc_set_back_boundary
+
+   subroutine rhyme_samr_bc_set_front_boundary(bc, box)
+      type(samr_bc_t), intent(in) :: bc
+      type(samr_box_t), intent(inout) :: box
+      call set_faces(bc, box, 5)
+   end subroutine rhyme_samr_bc_set_front_boundary
+#endif
+end module rhyme_samr_bc
+
+module rhyme_mh_workspace
+   ! rhyme_mh_workspace/src/rhyme_mh_workspace.f90:18-43.  The scratch of
+   ! the reference (ul, ur, fr or u) lives inside the device box; this type
+   ! keeps the reference's shape and its `type` selector.
+   use rhyme_samr
+   use rhyme_logger
+   implicit none
+
+   type mh_workspace_indices_t
+      integer :: cpu_intensive = 1, memory_intensive = 2
+   end type mh_workspace_indices_t
+
+   type(mh_workspace_indices_t), parameter :: mhwsid = mh_workspace_indices_t()
+
+   type mh_workspace_t
+      integer :: nlevels = 0
+      integer :: type = mhwsid%memory_intensive
+      logical :: initialized = .false.
+   end type mh_workspace_t
+
+contains
+
+   subroutine rhyme_mh_workspace_init(mhws, samr, logger)
+      type(mh_workspace_t), intent(inout) :: mhws
+      type(samr_t), intent(in) :: samr
+      type(logger_t), intent(inout) :: logger
+      if (mhws%initialized) call logger%warn('Trying to re-initialize mh_workspace object')
+      mhws%nlevels = samr%nlevels
+      mhws%initialized = .true.
+   end subroutine rhyme_mh_workspace_init
+
+   subroutine rhyme_mh_workspace_check(mhws, box)
+      type(mh_workspace_t), intent(inout) :: mhws
+      type(samr_box_t), intent(in) :: box
+      ! device scratch is sized with the box (rhy_box_create / rhy_box_resize)
+   end subroutine rhyme_mh_workspace_check
+end module rhyme_mh_workspace
+
+module rhyme_muscl_hancock
+   ! rhyme_muscl_hancock/src/rhyme_muscl_hancock.f90:9-64
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhy_device
+   use rhyme_units
+   use rhyme_samr
+   use rhyme_thermo_base
+   use rhyme_slope_limiter
+   use rhyme_riemann_problem
+   use rhyme_mh_workspace
+   use rhyme_logger
+   implicit none
+
+   type, private :: rhyme_muscl_hancock_indices_t
+      integer :: cpu_intensive = mhwsid%cpu_intensive
+      integer :: memory_intensive = mhwsid%memory_intensive
+   end type rhyme_muscl_hancock_indices_t
+
+   type(rhyme_muscl_hancock_indices_t), parameter :: mhid = rhyme_muscl_hancock_indices_t()
+
+   type muscl_hancock_t
+      integer :: solver_type = mhid%memory_intensive
+   end type muscl_hancock_t
+
+contains
+
+   subroutine rhyme_muscl_hancock_init(mh, samr, ws, logger)
+      type(muscl_hancock_t), intent(inout) :: mh
+      type(samr_t), intent(in) :: samr
+      type(mh_workspace_t), intent(inout) :: ws
+      type(logger_t), intent(inout) :: logger
+      call rhyme_mh_workspace_init(ws, samr, logger)
+   end subroutine rhyme_muscl_hancock_init
+
+   subroutine solve_with(solver_type, box, dx, dt, rp, sl, ws, rc)
+      integer, intent(in) :: solver_type
+      type(samr_box_t), intent(inout) :: box
+      real(kind=8), intent(in) :: dx(NDIM), dt
+      type(riemann_problem_t), intent(in) :: rp
+      type(slope_limiter_t), intent(in) :: sl
+      type(mh_workspace_t), intent(inout) :: ws
+      integer(c_int), intent(out) :: rc
+      type(rhy_params_t) :: p
+      type(c_ptr) :: h
+      character(len=:), allocatable :: msg
+      real(c_double) :: dx3(3)
+
+      rc = -1
+      h = rhy_box_handle(box, msg)
+      if (.not. c_associated(h)) return
+      call rhy_rp_params(rp, p)
+      p%limiter = sl%type
+      p%omega = sl%w
+      p%solver_type = solver_type
+      dx3 = 1d0
+      dx3(1:NDIM) = dx
+      rc = rhy_mh_solve(h, p, dx3, dt)
+   end subroutine solve_with
+
+   subroutine rhyme_muscl_hancock_solve_memory_intensive(box, dx, dt, rp, sl, ws)
+      type(samr_box_t), intent(inout) :: box
+      real(kind=8), intent(in) :: dx(NDIM), dt
+      type(riemann_problem_t), intent(inout) :: rp
+      type(slope_limiter_t), intent(in) :: sl
+      type(mh_workspace_t), intent(inout) :: ws
+      integer(c_int) :: rc
+      call solve_with(mhid%memory_intensive, box, dx, dt, rp, sl, ws, rc)
+   end subroutine rhyme_muscl_hancock_solve_memory_intensive
+
+   subroutine rhyme_muscl_hancock_solve_cpu_intensive(box, dx, dt, rp, sl, ws)
+      type(samr_box_t), intent(inout) :: box
+      real(kind=8), intent(in) :: dx(NDIM), dt
+      type(riemann_problem_t), intent(inout) :: rp
+      type(slope_limiter_t), intent(in) :: sl
+      type(mh_workspace_t), intent(inout) :: ws
+      integer(c_int) :: rc
+      call solve_with(mhid%cpu_intensive, box, dx, dt, rp, sl, ws, rc)
+   end subroutine rhyme_muscl_hancock_solve_cpu_intensive
+
+   ! rhyme_muscl_hancock_solve.f90:17-26
+   subroutine rhyme_muscl_hancock_solve(mh, box, dx, dt, rp, sl, ws, logger)
+      type(muscl_hancock_t), intent(inout) :: mh
+      type(samr_box_t), intent(inout) :: box
+      real(kind=8), intent(in) :: dx(NDIM), dt
+      type(riemann_problem_t), intent(inout) :: rp
+      type(slope_limiter_t), intent(in) :: sl
+      type(mh_workspace_t), intent(inout) :: ws
+      type(logger_t), intent(inout) :: logger
+      integer(c_int) :: rc
+
+      call logger%begin_section('solver')
+      select case (mh%solver_type)
+      case (mhid%cpu_intensive, mhid%memory_intensive)
+         call solve_with(mh%solver_type, box, dx, dt, rp, sl, ws, rc)
+         if (rc /= RHY_OK) call logger%err('rhy_mh_solve: '//rhy_error_text())
+      case DEFAULT
+         call logger%err('Unknown solver type')
+         return
+      end select
+      call logger%end_section
+   end subroutine rhyme_muscl_hancock_solve
+
+   ! rhyme_muscl_hancock_half_step_extrapolation.f90:3-24 (one cell on the device)
+   subroutine rhyme_muscl_hancock_half_step_extrapolation(u, delta, axis, dx, dt, l, r)
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(in) :: u, delta
+      integer, intent(in) :: axis
+      real(kind=8), intent(in) :: dx, dt
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(out) :: l, r
+      type(rhy_params_t) :: p
+      integer(c_int32_t) :: ax(1)
+      integer(c_int) :: rc
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%ncmp = NCMP
+      ax(1) = axis
+      rc = rhy_half_step_batch(p, 1_c_int64_t, u, delta, ax, dx, dt, l, r)
+   end subroutine rhyme_muscl_hancock_half_step_extrapolation
+end module rhyme_muscl_hancock
