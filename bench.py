@@ -72,6 +72,40 @@ def gpu_leg(n, steps, warmup, seed, precision=0):
     return dict(elapsed=t1 - t0, stats=stats, dts=dts, finite=finite, state=interior)
 
 
+def batched_leg(steps, warmup, seed):
+    """Batched AMR boxes: a 128^3 base box plus 200 refined boxes with
+    per-axis extents U{8..64} (dx halved), seeded two-state/blast initial
+    conditions, van Albada, outflow faces, each box with its own dt; every
+    phase of a step is one launch over all boxes, the step is replayed from a
+    hipGraph, tile shape from the autotuner.  FP64 and mixed."""
+    from rhyme_amd import autotune, hydro, ics
+    from rhyme_amd.batch import SamrBatch
+    boxes = ics.amr_boxes(seed=seed)
+    p = hydro.make_params(3, rp=hydro.RiemannProblem(irs=hydro.Irs(100, 1e-6)),
+                          sl=hydro.SlopeLimiter(type=hydro.VAN_ALBADA),
+                          cfl=hydro.Cfl(0.2), bc=hydro.SamrBc(types=[hydro.OUTFLOW] * 6))
+    out = dict(workload="128^3 base + 200 boxes U{8..64}^3, van Albada, outflow, "
+                        "per-box dt, graph-replayed steps", nboxes=len(boxes))
+    for name, prec in (("fp64", hydro.FP64), ("mixed", hydro.MIXED)):
+        b = SamrBatch(3, [d for d, _, _ in boxes], [x for _, x, _ in boxes], precision=prec)
+        init = [c for _, _, c in boxes]
+        best, timings = autotune.tune(b, p, init, steps=3)
+        if warmup:
+            b.run(p, warmup, graph=True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run_async(p, steps, graph=True)
+        b.sync()
+        el = time.perf_counter() - t0
+        zones = b.zones()
+        out[name] = dict(value=zones * steps / el, unit="zone-updates/s",
+                         ms_per_step=1e3 * el / steps, zones_per_step=zones,
+                         tiles=list(best), device_bytes=b.nbytes())
+        b.close()
+    out["mixed_speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    return out
+
+
 def cpu_leg(n_sample, steps_sample, seed, threads_all):
     """The oracle on the host cores: both reference loop structures with all
     threads, plus memory_intensive on one thread (at a quarter of the edge),
@@ -127,6 +161,9 @@ def main():
                     help="omit the CPU baseline leg (profiling runs)")
     ap.add_argument("--skip-mixed", action="store_true",
                     help="omit the mixed-precision leg")
+    ap.add_argument("--skip-batched", action="store_true",
+                    help="omit the batched AMR-boxes leg")
+    ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain device copy of 1 GiB (torch), the "
                          "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
@@ -187,6 +224,8 @@ def main():
             roofline_achieved_gbs=40.0 * zones / (sum(m["stats"][k][1] for k in (0, 1, 4))
                                                    / max(a.steps, 1) * 1e-3) / 1e9,
             bytes_per_zone_update=40, state_finite=m["finite"])
+    if not a.skip_batched:
+        line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
     if not a.skip_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)

@@ -175,6 +175,34 @@ int rhy_half_step_batch(const rhy_params_t *p, int64_t n, const double *u,
 int rhy_flux_batch(const rhy_params_t *p, int64_t n, const double *u,
                    const int32_t *axis, double *f);
 
+/* ---- batched form: many independent boxes of uneven shapes ----
+ * No counterpart in the reference, which advances its SAMR boxes one call
+ * at a time (src/main.f90:130-136) and has no per-box ghost exchange or dt.
+ * Here every box keeps its own ghost fill (p->bc_types on its faces) and its
+ * own CFL dt, so each box evolves exactly as rhy_run would advance it alone.
+ * One launch per phase (dt, all fluxes, all updates) covers every box through
+ * a tile table walked by persistent workgroups; the step is captured once in
+ * a hipGraph and replayed when use_graph is set.  dims and dx are
+ * [nboxes][3] (row-major); host arrays per box as for rhy_box_upload. */
+typedef struct rhy_batch rhy_batch_t;
+int rhy_batch_create(int32_t ndim, int32_t nboxes, const int32_t *dims, const double *dx,
+                     int32_t ncmp, int32_t precision, rhy_batch_t **out);
+int rhy_batch_destroy(rhy_batch_t *batch);
+/* device bytes held (state + flux scratch) */
+int rhy_batch_bytes(rhy_batch_t *batch, int64_t *bytes);
+int rhy_batch_upload(rhy_batch_t *batch, int32_t box, const double *cells);
+int rhy_batch_download(rhy_batch_t *batch, int32_t box, double *cells);
+/* tile shape: threads per workgroup (64..256, multiple of 64), faces walked
+ * per thread along x and along y/z, cap on persistent workgroups (0: auto) */
+int rhy_batch_set_tiles(rhy_batch_t *batch, int32_t block, int32_t nf_x, int32_t nf_yz,
+                        int32_t grid_cap);
+int rhy_batch_run_async(rhy_batch_t *batch, const rhy_params_t *p, int32_t nsteps,
+                        int32_t use_graph);
+int rhy_batch_sync(rhy_batch_t *batch);
+/* dts of the last run, [n][nboxes] */
+int rhy_batch_last_dts(rhy_batch_t *batch, int32_t n, double *dts);
+const char *rhy_batch_last_error(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,126 @@
+"""Many independent boxes of uneven shapes on one device (the batched form).
+
+`SamrBatch` wraps rhy_batch_t (include/rhyme_hip.h): one device arena for
+every box's state and flux scratch, one launch per phase over all boxes,
+the step captured in a hipGraph.  Each box keeps its own ghost fill and its
+own CFL dt, so a box evolves exactly as `hydro.run` would advance it alone.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr, iptr
+from .hydro import FP64, box_shape, ncmp_for
+
+_BATCH_SIGS = {
+    "rhy_batch_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_int32),
+                                   C.POINTER(C.c_double), C.c_int32, C.c_int32,
+                                   C.POINTER(C.c_void_p)]),
+    "rhy_batch_destroy": (C.c_int, [C.c_void_p]),
+    "rhy_batch_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "rhy_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "rhy_batch_download": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "rhy_batch_set_tiles": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                      C.c_int32]),
+    "rhy_batch_run_async": (C.c_int, [C.c_void_p, C.POINTER(_lib.RhyParams), C.c_int32,
+                                      C.c_int32]),
+    "rhy_batch_sync": (C.c_int, [C.c_void_p]),
+    "rhy_batch_last_dts": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    "rhy_batch_last_error": (C.c_char_p, []),
+}
+
+
+def _L():
+    L = _lib.load()
+    if not getattr(L, "_batch_sigs", False):
+        for name, (res, args) in _BATCH_SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        L._batch_sigs = True
+    return L
+
+
+def _check(rc, what):
+    if rc != _lib.RHY_OK:
+        L = _L()
+        msg = L.rhy_batch_last_error() or L.rhy_last_error()
+        raise _lib.RhyError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+
+
+class SamrBatch:
+    """Boxes (ndim, dims[i], dx[i]) advanced together; host arrays per box in
+    the reference layout (hydro.box_shape)."""
+
+    def __init__(self, ndim, dims, dx, ncmp=None, precision=FP64):
+        self.ndim = ndim
+        self.ncmp = ncmp if ncmp is not None else ncmp_for(ndim)
+        self.precision = precision
+        self.dims = [list(d)[:ndim] for d in dims]
+        self.n = len(self.dims)
+        d3 = np.ones((self.n, 3), dtype=np.int32)
+        x3 = np.ones((self.n, 3), dtype=np.float64)
+        for i, d in enumerate(self.dims):
+            d3[i, :ndim] = d
+            x3[i, :ndim] = list(dx[i])[:ndim]
+        self.dx = x3
+        h = C.c_void_p()
+        _check(_L().rhy_batch_create(ndim, self.n, iptr(d3), dptr(x3), self.ncmp, precision,
+                                     C.byref(h)), "rhy_batch_create")
+        self._h = h
+        self.tiles = (256, 4, 8, 0)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _L().rhy_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, i):
+        return box_shape(self.ndim, self.dims[i], self.ncmp)
+
+    def upload(self, i, cells):
+        cells = np.ascontiguousarray(cells, dtype=np.float64)
+        assert cells.shape == self.shape(i)
+        _check(_L().rhy_batch_upload(self._h, i, dptr(cells)), "rhy_batch_upload")
+
+    def download(self, i):
+        out = np.empty(self.shape(i), dtype=np.float64)
+        _check(_L().rhy_batch_download(self._h, i, dptr(out)), "rhy_batch_download")
+        return out
+
+    def nbytes(self):
+        b = C.c_int64(0)
+        _check(_L().rhy_batch_bytes(self._h, C.byref(b)), "rhy_batch_bytes")
+        return b.value
+
+    def set_tiles(self, block=256, nf_x=4, nf_yz=8, grid_cap=0):
+        _check(_L().rhy_batch_set_tiles(self._h, block, nf_x, nf_yz, grid_cap),
+               "rhy_batch_set_tiles")
+        self.tiles = (block, nf_x, nf_yz, grid_cap)
+
+    def run_async(self, params, nsteps, graph=True):
+        _check(_L().rhy_batch_run_async(self._h, C.byref(params), nsteps, 1 if graph else 0),
+               "rhy_batch_run_async")
+
+    def sync(self):
+        _check(_L().rhy_batch_sync(self._h), "rhy_batch_sync")
+
+    def run(self, params, nsteps, graph=True):
+        self.run_async(params, nsteps, graph)
+        self.sync()
+        return self.last_dts(nsteps)
+
+    def last_dts(self, n):
+        out = np.zeros((max(n, 1), self.n))
+        _check(_L().rhy_batch_last_dts(self._h, n, dptr(out)), "rhy_batch_last_dts")
+        return out[:n]
+
+    def zones(self):
+        return int(sum(int(np.prod(d)) for d in self.dims))

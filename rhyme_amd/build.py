@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librhyme_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["rhy_api.hip"]
+SOURCES = ["rhy_api.hip", "rhy_batch.hip"]
 
 FLAGS = [
     "--offload-arch=gfx950",
@@ -30,17 +30,33 @@ FLAGS = [
 ]
 
 
-def build(verbose=False, extra=None, out=None):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    lib = out or LIB
-    tmp = lib + ".tmp"
-    cmd = [HIPCC] + FLAGS + (extra or []) + srcs + ["-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed building librhyme_hip.so")
+
+
+def build(verbose=False, extra=None, out=None):
+    """Compile every translation unit in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+    lib = out or LIB
+    tmp = lib + ".tmp"
+    objdir = os.path.join(HERE, "_build", "obj" + ("" if out is None else
+                                                   "_" + os.path.basename(out)))
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in FLAGS if f != "-shared"]
+
+    def one(src):
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        _run([HIPCC] + compile_flags + (extra or []) + ["-c", os.path.join(CSRC, src),
+                                                        "-o", obj], verbose)
+        return obj
+    with ThreadPoolExecutor(len(SOURCES)) as ex:
+        objs = list(ex.map(one, SOURCES))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp], verbose)
     os.replace(tmp, lib)
     return lib
 

@@ -161,17 +161,14 @@ constexpr int kNfYZ = RHY_NF_YZ;
 template <int ND, int A, int LIM, typename R, class PW>
 void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx) {
   const Geom& G = b->geom;
-  constexpr int NF = A == 0 ? kNfX : kNfYZ;
-  const int64_t nchunk = (G.n[A] + 1 + NF - 1) / NF;
-  int64_t nthreads = nchunk;
-  for (int d = 0; d < 3; ++d)
-    if (d != A) nthreads *= G.n[d];
+  const int nf = A == 0 ? kNfX : kNfYZ;
+  const int64_t nthreads = flux_threads(G, A, nf);
   R* F = static_cast<R*>(b->flux) + (int64_t)A * (ND + 2) * G.plane;
   TimedLaunch tl;
   timed_begin(b, 0, tl);
-  hipLaunchKernelGGL((flux_kernel<ND, A, LIM, R, PW, NF>), dim3(grid_for(nthreads)),
+  hipLaunchKernelGGL((flux_kernel<ND, A, LIM, R, PW>), dim3(grid_for(nthreads)),
                      dim3(kThreads), 0, b->stream, k, G, static_cast<const R*>(b->cells), F, b->dt,
-                     dx[A]);
+                     dx[A], nf);
   timed_end(b, tl);
 }
 

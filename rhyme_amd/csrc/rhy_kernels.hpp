@@ -115,7 +115,7 @@ __device__ __forceinline__ double cfl_speed(const Consts& k, const double* u) {
 // independent, so the reduction is deterministic; NaN speeds are skipped like
 // the reference's strict `u > max_u` test (rhyme_cfl_time_step.f90:44).
 // Every thread of the block must call it (blockDim.x a multiple of 64, <= 1024).
-__device__ __forceinline__ void block_max_to_partial(double v, double* partial) {
+__device__ __forceinline__ void block_max_to_index(double v, double* partial, int64_t index) {
   __shared__ double wave_max[16];
   v = v > 0.0 ? v : 0.0;
 #pragma unroll
@@ -130,8 +130,12 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
     double m = 0.0;
     const int nw = (blockDim.x + 63) >> 6;
     for (int i = 0; i < nw; ++i) m = wave_max[i] > m ? wave_max[i] : m;
-    partial[blockIdx.x] = m;
+    partial[index] = m;
   }
+}
+
+__device__ __forceinline__ void block_max_to_partial(double v, double* partial) {
+  block_max_to_index(v, partial, blockIdx.x);
 }
 
 // Intercell flux on every face normal to axis A that the interior update
@@ -146,15 +150,16 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 // R is the storage/bulk type (double: the FP64 drop-in; float: the mixed
 // form, whose half step runs in FP32 and whose Riemann solve is promoted to
 // FP64 with the PowFast exponentials).  LIM is the slope limiter (slid).
-template <int ND, int A, int LIM, typename R, class PW, int NF>
-__global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(Consts k, Geom G,
-                                                   const R* __restrict__ cells,
-                                                   R* __restrict__ F,
-                                                   const double* __restrict__ dtp, double dx) {
+// One thread's walk: nf_max consecutive faces normal to A starting at face
+// chunk `t` of the box (thread t of the axis' index space); returns false
+// if t is past the end.
+template <int ND, int A, int LIM, typename R, class PW>
+__device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
+                                          const R* __restrict__ cells, R* __restrict__ F,
+                                          double dt, double dx, int nf_max, int64_t t) {
   constexpr int NV = ND + 2;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nA = G.n[A];
-  const int nchunk = (nA + 1 + NF - 1) / NF;
+  const int nchunk = (nA + 1 + nf_max - 1) / nf_max;
   const int e0 = A == 0 ? nchunk : G.n[0];
   const int e1 = A == 1 ? nchunk : G.n[1];
   const int e2 = A == 2 ? nchunk : G.n[2];
@@ -164,13 +169,12 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(Consts k, 
   const int64_t rr = t / e0;
   idx[1] = (int)(rr % e1);
   idx[2] = (int)(rr / e1);
-  const int f0 = idx[A] * NF;
-  const int nf = nA + 1 - f0 < NF ? nA + 1 - f0 : NF;
+  const int f0 = idx[A] * nf_max;
+  const int nf = nA + 1 - f0 < nf_max ? nA + 1 - f0 : nf_max;
 #pragma unroll
   for (int d = 0; d < 3; ++d) idx[d] = d == A ? f0 : idx[d] + 1;
   const int64_t o = goff(G, idx[0], idx[1], idx[2]);
   const int64_t s = G.s[A];
-  const double dt = *dtp;
   const R hdtdx = (R)(.5 * dt / dx);
   R um[NV], u0[NV], up[NV], lw[NV], rp[NV], rn[NV];
   load_state<ND, R>(cells, G.plane, o - s, um);
@@ -201,6 +205,22 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(Consts k, 
       rp[c] = rn[c];
     }
   }
+}
+
+// Threads of the flux index space of axis A for faces walked nf at a time.
+__host__ __device__ inline int64_t flux_threads(const Geom& G, int A, int nf) {
+  int64_t n = (G.n[A] + 1 + nf - 1) / nf;
+  for (int d = 0; d < 3; ++d)
+    if (d != A) n *= G.n[d];
+  return n;
+}
+
+template <int ND, int A, int LIM, typename R, class PW>
+__global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(
+    Consts k, Geom G, const R* __restrict__ cells, R* __restrict__ F,
+    const double* __restrict__ dtp, double dx, int nf) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  flux_walk<ND, A, LIM, R, PW>(k, G, cells, F, *dtp, dx, nf, t);
 }
 
 // Conservative update of every interior cell from the stored fluxes.
@@ -347,7 +367,7 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const R* __r
 // dt = courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the product
 // courant*minval(dx) is formed on the host.  Writes dt (and logs it) and the
 // raw max_u bits to maxu_out when given.
-__global__ __launch_bounds__(1024) void dt_kernel(const double* __restrict__ partial, int32_t n,
+static __global__ __launch_bounds__(1024) void dt_kernel(const double* __restrict__ partial, int32_t n,
                                                   double cmdx, double* dt_out, double* dt_log,
                                                   double* maxu_out) {
   __shared__ double wave_max[16];
@@ -372,7 +392,7 @@ __global__ __launch_bounds__(1024) void dt_kernel(const double* __restrict__ par
   }
 }
 
-__global__ void set_scalar_kernel(double* dst, double v) {
+static __global__ void set_scalar_kernel(double* dst, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 

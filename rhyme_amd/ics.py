@@ -130,3 +130,30 @@ def random_smooth(ndim, dims, seed=0, gamma=5.0 / 3.0, ncmp=None, amp=0.3, vamp=
     p = 1.0 + amp * g.uniform(-1, 1, size=shape)
     v = vamp * g.uniform(-1, 1, size=[ndim] + shape)
     return _fill(ndim, dims, ncmp, gamma, rho, v, p)
+
+
+def amr_boxes(seed=0, nboxes=200, base=128, lo=8, hi=64, gamma=5.0 / 3.0, ndim=3):
+    """Batched-AMR workload: one base box of base^ndim cells (dx0 = 1/base)
+    plus `nboxes` refined boxes (dx = dx0/2) whose per-axis extents are drawn
+    uniformly from [lo, hi]; each refined box gets a seeded two-state or
+    blast initial condition, the base box a blast.  Returns a list of
+    (dims, dx, cells)."""
+    g = rng(seed)
+    out = []
+    dx0 = 1.0 / base
+    if base:
+        dims = [base] * ndim
+        if ndim == 3:
+            cells = blast_3d(dims=dims, seed=int(g.integers(1 << 30)), gamma=gamma)
+        else:
+            cells = random_two_state(ndim, dims, seed=int(g.integers(1 << 30)), gamma=gamma)
+        out.append((dims, [dx0] * ndim, cells))
+    for _ in range(nboxes):
+        dims = [int(x) for x in g.integers(lo, hi + 1, size=ndim)]
+        s = int(g.integers(1 << 30))
+        if ndim == 3 and g.uniform() < 0.5:
+            cells = blast_3d(dims=dims, seed=s, gamma=gamma)
+        else:
+            cells = random_two_state(ndim, dims, seed=s, gamma=gamma)
+        out.append((dims, [dx0 / 2.0] * ndim, cells))
+    return out
