@@ -148,6 +148,31 @@ def _cpu_subprocess(n, steps, seed, solver, threads):
                 sample="3D blast %d^3, steps 0..%d from the initial condition" % (n, steps))
 
 
+def serving_leg(njobs, rate, budget_gib, quantum, seed, precision=0):
+    """Serving stream: `njobs` independent jobs with Poisson arrivals at
+    `rate` jobs/s, half 2D n^2 (n log-uniform in [64, 512]), half 3D n^3 (n
+    log-uniform in [32, 128]), 100 steps each, seeded two-state or blast
+    state, limiter uniform over the four; admitted first come first served
+    under an HBM budget, state resident, continuously batched in quanta.
+    Initial states are generated on host threads ahead of admission and
+    uploaded at admission; finished states are downloaded at retirement
+    (both inside the measured latency)."""
+    from rhyme_amd import serve
+    jobs = serve.serving_jobs(njobs=njobs, rate=rate, seed=seed)
+    srv = serve.HydroServer(int(budget_gib * (1 << 30)), precision=precision, quantum=quantum,
+                            keep_results=False)
+    t0 = time.perf_counter()
+    srv.serve(jobs)
+    wall = time.perf_counter() - t0
+    rep = serve.latency_report(jobs)
+    srv.close()
+    rep.update(rate_jobs_per_s=rate, budget_gib=budget_gib, quantum_steps=quantum,
+               peak_resident_gib=srv.peak_bytes / float(1 << 30), admission_deferrals=srv.deferrals,
+               quanta=srv.quanta, wall_s=wall, unit="zone-updates/s (aggregate), ms latency",
+               precision="mixed" if precision else "fp64")
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -164,6 +189,11 @@ def main():
     ap.add_argument("--skip-batched", action="store_true",
                     help="omit the batched AMR-boxes leg")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--skip-serve", action="store_true", help="omit the serving-stream leg")
+    ap.add_argument("--serve-jobs", type=int, default=1000)
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals, jobs/s")
+    ap.add_argument("--serve-budget-gib", type=float, default=16.0)
+    ap.add_argument("--serve-quantum", type=int, default=10)
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain device copy of 1 GiB (torch), the "
                          "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
@@ -226,6 +256,9 @@ def main():
             bytes_per_zone_update=40, state_finite=m["finite"])
     if not a.skip_batched:
         line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
+    if not a.skip_serve:
+        line["serving"] = serving_leg(a.serve_jobs, a.serve_rate, a.serve_budget_gib,
+                                      a.serve_quantum, a.seed)
     if not a.skip_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)

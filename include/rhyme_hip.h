@@ -199,9 +199,31 @@ int rhy_batch_set_tiles(rhy_batch_t *batch, int32_t block, int32_t nf_x, int32_t
 int rhy_batch_run_async(rhy_batch_t *batch, const rhy_params_t *p, int32_t nsteps,
                         int32_t use_graph);
 int rhy_batch_sync(rhy_batch_t *batch);
-/* dts of the last run, [n][nboxes] */
+/* dts of the last run, [n][nboxes] (for a pool: [n][max_boxes], indexed by
+ * slot; entries of slots that were not active are left as they were) */
 int rhy_batch_last_dts(rhy_batch_t *batch, int32_t n, double *dts);
 const char *rhy_batch_last_error(void);
+
+/* ---- pool: a batch whose boxes come and go (continuous batching) ----
+ * No counterpart in the reference.  A pool holds a device arena of
+ * budget_bytes; rhy_pool_add places a box (state + flux scratch,
+ * rhy_pool_box_bytes of it) first-fit in the arena and returns its slot, or
+ * RHY_ERR_OOM when no free block is large enough or all max_boxes slots are
+ * taken.  Each box carries its own slope limiter and face types (bc_types
+ * NULL: reflective); p->limiter and p->bc_types are ignored by
+ * rhy_batch_run_async on a pool, every other parameter applies to all its
+ * boxes.  Add and remove between runs (both wait for the pool's stream);
+ * rhy_batch_upload/download/last_dts take the slot as the box index. */
+int rhy_pool_create(int32_t ndim, int32_t ncmp, int32_t precision, int64_t budget_bytes,
+                    int32_t max_boxes, rhy_batch_t **out);
+int rhy_pool_box_bytes(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t precision,
+                       int64_t *bytes);
+int rhy_pool_add(rhy_batch_t *pool, const int32_t dims[3], const double dx[3],
+                 const double *cells, int32_t limiter, const int32_t bc_types[6],
+                 int32_t *slot);
+int rhy_pool_remove(rhy_batch_t *pool, int32_t slot);
+/* total free bytes of the arena and the largest free block */
+int rhy_pool_free_bytes(rhy_batch_t *pool, int64_t *free_bytes, int64_t *largest_bytes);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("RHYME_HIP_LIB") or os.path.join(HERE, "librhyme_hip.so")
 
 RHY_OK = 0
+RHY_ERR_ARG, RHY_ERR_HIP, RHY_ERR_OOM, RHY_ERR_SHAPE, RHY_ERR_NODEV, RHY_ERR_SOLVER = \
+    -1, -2, -3, -4, -5, -6
 MAX_CMP = 16
 
 # Every symbol include/rhyme_hip.h declares (checked by the CPU test suite).
@@ -27,6 +29,11 @@ EXPORTS = [
     "rhy_box_set_profiling", "rhy_box_kernel_stats",
     "rhy_riemann_batch", "rhy_limiter_batch", "rhy_half_step_batch",
     "rhy_flux_batch",
+    "rhy_batch_create", "rhy_batch_destroy", "rhy_batch_bytes", "rhy_batch_upload",
+    "rhy_batch_download", "rhy_batch_set_tiles", "rhy_batch_run_async", "rhy_batch_sync",
+    "rhy_batch_last_dts", "rhy_batch_last_error",
+    "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_remove",
+    "rhy_pool_free_bytes",
 ]
 
 

@@ -28,6 +28,16 @@ _BATCH_SIGS = {
     "rhy_batch_sync": (C.c_int, [C.c_void_p]),
     "rhy_batch_last_dts": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "rhy_batch_last_error": (C.c_char_p, []),
+    "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
+                                  C.POINTER(C.c_void_p)]),
+    "rhy_pool_box_bytes": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
+                                     C.POINTER(C.c_int64)]),
+    "rhy_pool_add": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                               C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32),
+                               C.POINTER(C.c_int32)]),
+    "rhy_pool_remove": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rhy_pool_free_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
+                                      C.POINTER(C.c_int64)]),
 }
 
 
@@ -124,3 +134,112 @@ class SamrBatch:
 
     def zones(self):
         return int(sum(int(np.prod(d)) for d in self.dims))
+
+
+def box_bytes(ndim, dims, ncmp=None, precision=FP64):
+    """Arena bytes one box takes in a pool (state planes + flux scratch)."""
+    d3 = np.ones(3, dtype=np.int32)
+    d3[:ndim] = list(dims)[:ndim]
+    b = C.c_int64(0)
+    _check(_L().rhy_pool_box_bytes(ndim, iptr(d3), ncmp if ncmp is not None else ncmp_for(ndim),
+                                   precision, C.byref(b)), "rhy_pool_box_bytes")
+    return b.value
+
+
+class PoolFull(Exception):
+    """The box does not fit in the pool's arena (or every slot is taken)."""
+
+
+class SamrPool:
+    """A batch whose boxes come and go: a device arena of `budget_bytes`,
+    boxes placed first-fit with their own limiter and face types, advanced
+    together by `run_async`.  Boxes are addressed by slot."""
+
+    def __init__(self, ndim, budget_bytes, max_boxes=4096, ncmp=None, precision=FP64):
+        self.ndim = ndim
+        self.ncmp = ncmp if ncmp is not None else ncmp_for(ndim)
+        self.precision = precision
+        self.max_boxes = max_boxes
+        self.budget = int(budget_bytes)
+        h = C.c_void_p()
+        _check(_L().rhy_pool_create(ndim, self.ncmp, precision, self.budget, max_boxes,
+                                    C.byref(h)), "rhy_pool_create")
+        self._h = h
+        self.boxes = {}  # slot -> dims
+        self.tiles = (256, 4, 8, 0)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _L().rhy_batch_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def box_bytes(self, dims):
+        return box_bytes(self.ndim, dims, self.ncmp, self.precision)
+
+    def add(self, dims, dx, cells, limiter, bc_types=None):
+        """Place a box; returns its slot, raises PoolFull when it does not fit."""
+        d3 = np.ones(3, dtype=np.int32)
+        x3 = np.ones(3, dtype=np.float64)
+        d3[:self.ndim] = list(dims)[:self.ndim]
+        x3[:self.ndim] = list(dx)[:self.ndim]
+        cells = np.ascontiguousarray(cells, dtype=np.float64)
+        assert cells.shape == box_shape(self.ndim, list(dims)[:self.ndim], self.ncmp)
+        bc = None
+        if bc_types is not None:
+            bc = (C.c_int32 * 6)(*[int(t) for t in bc_types])
+        slot = C.c_int32(-1)
+        rc = _L().rhy_pool_add(self._h, iptr(d3), dptr(x3), dptr(cells), int(limiter),
+                               C.cast(bc, C.POINTER(C.c_int32)) if bc is not None else None,
+                               C.byref(slot))
+        if rc == _lib.RHY_ERR_OOM:
+            raise PoolFull(_L().rhy_batch_last_error().decode())
+        _check(rc, "rhy_pool_add")
+        self.boxes[slot.value] = list(dims)[:self.ndim]
+        return slot.value
+
+    def remove(self, slot):
+        _check(_L().rhy_pool_remove(self._h, slot), "rhy_pool_remove")
+        del self.boxes[slot]
+
+    def free_bytes(self):
+        tot, big = C.c_int64(0), C.c_int64(0)
+        _check(_L().rhy_pool_free_bytes(self._h, C.byref(tot), C.byref(big)),
+               "rhy_pool_free_bytes")
+        return tot.value, big.value
+
+    def download(self, slot):
+        out = np.empty(box_shape(self.ndim, self.boxes[slot], self.ncmp), dtype=np.float64)
+        _check(_L().rhy_batch_download(self._h, slot, dptr(out)), "rhy_batch_download")
+        return out
+
+    def upload(self, slot, cells):
+        cells = np.ascontiguousarray(cells, dtype=np.float64)
+        assert cells.shape == box_shape(self.ndim, self.boxes[slot], self.ncmp)
+        _check(_L().rhy_batch_upload(self._h, slot, dptr(cells)), "rhy_batch_upload")
+
+    def set_tiles(self, block=256, nf_x=4, nf_yz=8, grid_cap=0):
+        _check(_L().rhy_batch_set_tiles(self._h, block, nf_x, nf_yz, grid_cap),
+               "rhy_batch_set_tiles")
+        self.tiles = (block, nf_x, nf_yz, grid_cap)
+
+    def run_async(self, params, nsteps, graph=True):
+        _check(_L().rhy_batch_run_async(self._h, C.byref(params), nsteps, 1 if graph else 0),
+               "rhy_batch_run_async")
+
+    def sync(self):
+        _check(_L().rhy_batch_sync(self._h), "rhy_batch_sync")
+
+    def last_dts(self, n):
+        """[n][max_boxes] dts of the last run, by slot."""
+        out = np.zeros((max(n, 1), self.max_boxes))
+        _check(_L().rhy_batch_last_dts(self._h, n, dptr(out)), "rhy_batch_last_dts")
+        return out[:n]
+
+    def zones(self):
+        return int(sum(int(np.prod(d)) for d in self.boxes.values()))

@@ -1,15 +1,26 @@
-// rhy_batch.hip -- many independent boxes of uneven shapes advanced
-// together: one launch per phase (dt, fluxes of every axis of every box,
-// update of every box) over a tile table, with persistent workgroups that
-// stride over the tiles, and the step captured once in a hipGraph and
-// replayed.  Each box keeps its own ghost fill (the configured faces) and
-// its own CFL dt, i.e. every box evolves exactly as it would alone
-// (rhy_run on that box), which is what the tests check.
+// rhy_batch.hip -- many independent boxes of uneven shapes on one device.
+//
+// A batch owns one HBM arena; every box (a "slot") gets a block of it for
+// its state planes and its flux scratch.  A step is one launch per phase
+// over all active boxes -- dt of every box, the fluxes of every axis of
+// every box, the update of every box -- driven by a tile table that
+// persistent workgroups stride over; the step can be captured once in a
+// hipGraph and replayed.  Each box keeps its own ghost fill and its own CFL
+// dt, i.e. it evolves exactly as it would alone (rhy_run on that box).
+//
+// A batch made by rhy_batch_create has a fixed set of boxes and takes the
+// limiter and face types from the run's parameters.  A pool (rhy_pool_create)
+// starts empty with a byte budget: boxes are added (first fit in the arena,
+// refused when it does not fit) and removed between runs, each with its own
+// limiter and face types -- the continuous-batching engine of the serving
+// loop admits jobs through it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <iterator>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -34,13 +45,15 @@ int bfail(int code, const std::string& msg) {
       return bfail(RHY_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
   } while (0)
 
-// per-box descriptor, device resident
+// per-box descriptor, device resident (indexed by slot)
 struct BBox {
   Geom G;
-  int64_t cell_off;  // element offset of the box's planes in the cell arena
+  int64_t cell_off;  // element offset of the box's state planes in the arena
   int64_t flux_off;  // element offset of its flux planes
   double dx[3];
   double cmdx;       // courant * minval(dx)
+  BcTypes bc;        // face types of this box
+  int32_t limiter;   // slope limiter of this box
   int32_t part_begin, part_end;  // its update tiles (CFL partials)
 };
 
@@ -55,26 +68,28 @@ struct BTile {
   int32_t pad;
 };
 
+// LIM > 0: every box uses that limiter; LIM == 0: each box's own.
 template <int ND, int LIM, typename R, class PW>
 __global__ __launch_bounds__(256) void batch_flux_kernel(Consts k, const BBox* __restrict__ boxes,
                                                          const BTile* __restrict__ tiles,
-                                                         int32_t ntiles, const R* __restrict__ cells,
-                                                         R* __restrict__ flux,
+                                                         int32_t ntiles, R* __restrict__ arena,
                                                          const double* __restrict__ dt,
                                                          int32_t nf_x, int32_t nf_yz) {
   for (int32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
     const BTile T = tiles[ti];
     if ((int32_t)threadIdx.x >= T.count) continue;
     const BBox& B = boxes[T.box];
+    Consts kb = k;
+    if (LIM == 0) kb.limiter = B.limiter;
     const int64_t t = T.base + threadIdx.x;
-    const R* c = cells + B.cell_off;
-    R* F = flux + B.flux_off + (int64_t)T.kind * (ND + 2) * B.G.plane;
+    const R* c = arena + B.cell_off;
+    R* F = arena + B.flux_off + (int64_t)T.kind * (ND + 2) * B.G.plane;
     const double d = dt[T.box];
-    if (T.kind == 0) flux_walk<ND, 0, LIM, R, PW>(k, B.G, c, F, d, B.dx[0], nf_x, t);
+    if (T.kind == 0) flux_walk<ND, 0, LIM, R, PW>(kb, B.G, c, F, d, B.dx[0], nf_x, t);
     if constexpr (ND > 1)
-      if (T.kind == 1) flux_walk<ND, 1, LIM, R, PW>(k, B.G, c, F, d, B.dx[1], nf_yz, t);
+      if (T.kind == 1) flux_walk<ND, 1, LIM, R, PW>(kb, B.G, c, F, d, B.dx[1], nf_yz, t);
     if constexpr (ND > 2)
-      if (T.kind == 2) flux_walk<ND, 2, LIM, R, PW>(k, B.G, c, F, d, B.dx[2], nf_yz, t);
+      if (T.kind == 2) flux_walk<ND, 2, LIM, R, PW>(kb, B.G, c, F, d, B.dx[2], nf_yz, t);
   }
 }
 
@@ -84,10 +99,9 @@ template <int ND, int ORDER, typename R>
 __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox* __restrict__ boxes,
                                                            const BTile* __restrict__ tiles,
                                                            int32_t first, int32_t ntiles,
-                                                           R* __restrict__ cells,
-                                                           const R* __restrict__ flux,
+                                                           R* __restrict__ arena,
                                                            const double* __restrict__ dt,
-                                                           BcTypes bc, double* __restrict__ partial) {
+                                                           double* __restrict__ partial) {
   constexpr int NV = ND + 2;
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
     const BTile T = tiles[ti];
@@ -101,8 +115,8 @@ __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox*
       const int j = (int)(r % G.n[1]) + 1;
       const int kk = (int)(r / G.n[1]) + 1;
       const int64_t o = goff(G, i, j, kk);
-      R* c = cells + B.cell_off;
-      const R* F0 = flux + B.flux_off;
+      R* c = arena + B.cell_off;
+      const R* F0 = arena + B.flux_off;
       const R* F1 = F0 + (int64_t)NV * G.plane;
       const R* F2 = F0 + (int64_t)2 * NV * G.plane;
       const double d = dt[T.box];
@@ -128,7 +142,7 @@ __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox*
       }
 #pragma unroll
       for (int cc = 0; cc < NV; ++cc) c[o + cc * G.plane] = u[cc];
-      write_ghost_images<ND, R>(G, bc, c, i, j, kk, u);
+      write_ghost_images<ND, R>(G, B.bc, c, i, j, kk, u);
       double ud[NV];
 #pragma unroll
       for (int cc = 0; cc < NV; ++cc) ud[cc] = (double)u[cc];
@@ -144,7 +158,7 @@ template <int ND, typename R>
 __global__ __launch_bounds__(256) void batch_cfl_kernel(Consts k, const BBox* __restrict__ boxes,
                                                         const BTile* __restrict__ tiles,
                                                         int32_t first, int32_t ntiles,
-                                                        const R* __restrict__ cells,
+                                                        const R* __restrict__ arena,
                                                         double* __restrict__ partial) {
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
     const BTile T = tiles[ti];
@@ -158,7 +172,7 @@ __global__ __launch_bounds__(256) void batch_cfl_kernel(Consts k, const BBox* __
       const int j = (int)(r % G.n[1]) + 1;
       const int kk = (int)(r / G.n[1]) + 1;
       R ur[ND + 2];
-      load_state<ND, R>(cells + B.cell_off, G.plane, goff(G, i, j, kk), ur);
+      load_state<ND, R>(arena + B.cell_off, G.plane, goff(G, i, j, kk), ur);
       double u[ND + 2];
 #pragma unroll
       for (int c = 0; c < ND + 2; ++c) u[c] = (double)ur[c];
@@ -169,16 +183,18 @@ __global__ __launch_bounds__(256) void batch_cfl_kernel(Consts k, const BBox* __
   }
 }
 
-// dt of every box (block b reduces box b's partials); logs it at
-// dtlog[*step * nboxes + b] when the step counter is below the log size.
+// dt of every active box (block b reduces the partials of box active[b]);
+// logs it at dtlog[*step * log_stride + box] while the step counter is
+// below the log size.
 __global__ __launch_bounds__(256) void batch_dt_kernel(const BBox* __restrict__ boxes,
+                                                       const int32_t* __restrict__ active,
                                                        const double* __restrict__ partial,
                                                        double* __restrict__ dt,
                                                        double* __restrict__ dtlog,
                                                        const int32_t* __restrict__ step,
-                                                       int32_t log_steps, int32_t nboxes) {
+                                                       int32_t log_steps, int32_t log_stride) {
   __shared__ double wave_max[4];
-  const int b = blockIdx.x;
+  const int b = active[blockIdx.x];
   const BBox& B = boxes[b];
   double v = 0.0;
   for (int i = B.part_begin + threadIdx.x; i < B.part_end; i += blockDim.x)
@@ -196,7 +212,7 @@ __global__ __launch_bounds__(256) void batch_dt_kernel(const BBox* __restrict__ 
     const double d = B.cmdx / m;
     dt[b] = d;
     const int32_t s = *step;
-    if (dtlog && s < log_steps) dtlog[(int64_t)s * nboxes + b] = d;
+    if (dtlog && s < log_steps) dtlog[(int64_t)s * log_stride + b] = d;
   }
 }
 
@@ -210,21 +226,36 @@ __global__ void batch_set_counter_kernel(int32_t* step, int32_t v) {
 
 unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
+constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) boundaries
+
+struct Slot {
+  bool active = false;
+  BBox B{};
+  int64_t off = 0, len = 0;  // arena block (elements)
+};
+
 }  // namespace
 
 struct rhy_batch {
-  int32_t ndim = 0, ncmp = 0, precision = RHY_FP64, nboxes = 0;
+  int32_t ndim = 0, ncmp = 0, precision = RHY_FP64;
+  bool pool = false;
   size_t esz = sizeof(double);
-  std::vector<BBox> hboxes;
+  std::vector<Slot> slots;
+  std::vector<int32_t> active;
+  std::map<int64_t, int64_t> free_blocks;  // offset -> length, elements
+  int64_t arena_elems = 0;
   std::vector<BTile> htiles;
   int32_t nflux_tiles = 0, nupd_tiles = 0;
   int32_t block = 256, nf_x = 4, nf_yz = 8, grid_cap = 0;
+  bool dirty = true;
   hipStream_t stream = nullptr;
-  void* cells = nullptr;
-  void* flux = nullptr;
+  void* arena = nullptr;
   BBox* dboxes = nullptr;
+  int32_t* dactive = nullptr;
   BTile* dtiles = nullptr;
+  size_t dtiles_cap = 0;
   double* partial = nullptr;
+  size_t partial_cap = 0;
   double* dt = nullptr;
   double* dtlog = nullptr;
   int32_t dtlog_steps = 0;
@@ -233,7 +264,6 @@ struct rhy_batch {
   double* inflow = nullptr;
   hipGraphExec_t graph = nullptr;
   uint64_t graph_key = 0;
-  int64_t cells_elems = 0, flux_elems = 0;
 };
 
 namespace {
@@ -259,46 +289,117 @@ Consts consts_of(const rhy_params_t* p) {
   return k;
 }
 
-// Rebuild the tile table for the current tile shape; box b's update tiles
-// are contiguous so its CFL partials are the range [part_begin, part_end).
+// Geometry and arena footprint (elements) of a box.
+int box_geom(int ndim, const int32_t* dims, Geom& G) {
+  int64_t st = 1;
+  for (int d = 0; d < 3; ++d) {
+    const int n = dims[d];
+    if (d < ndim && n < 2) return bfail(RHY_ERR_SHAPE, "active extents must be >= 2");
+    if (d >= ndim && n != 1) return bfail(RHY_ERR_SHAPE, "inactive extents must be 1");
+    G.n[d] = n;
+    G.g[d] = d < ndim ? 2 : 0;
+    G.s[d] = st;
+    st *= (int64_t)(n + 2 * G.g[d]);
+  }
+  G.plane = st;
+  return RHY_OK;
+}
+
+int64_t round_up(int64_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+int64_t box_elems(const Geom& G, int ndim, int ncmp) {
+  return round_up(G.plane * ncmp) + round_up(G.plane * (ndim + 2) * ndim);
+}
+
+// first-fit in the free list; returns -1 when nothing fits
+int64_t arena_alloc(rhy_batch* b, int64_t len) {
+  for (auto it = b->free_blocks.begin(); it != b->free_blocks.end(); ++it) {
+    if (it->second < len) continue;
+    const int64_t off = it->first;
+    const int64_t rest = it->second - len;
+    b->free_blocks.erase(it);
+    if (rest > 0) b->free_blocks[off + len] = rest;
+    return off;
+  }
+  return -1;
+}
+
+void arena_free(rhy_batch* b, int64_t off, int64_t len) {
+  auto it = b->free_blocks.emplace(off, len).first;
+  // coalesce with the next and the previous block
+  auto nx = std::next(it);
+  if (nx != b->free_blocks.end() && it->first + it->second == nx->first) {
+    it->second += nx->second;
+    b->free_blocks.erase(nx);
+  }
+  if (it != b->free_blocks.begin()) {
+    auto pv = std::prev(it);
+    if (pv->first + pv->second == it->first) {
+      pv->second += it->second;
+      b->free_blocks.erase(it);
+    }
+  }
+}
+
+int grow(void** ptr, size_t* cap, size_t need, size_t elem) {
+  if (need <= *cap) return RHY_OK;
+  if (*ptr) hipFree(*ptr);
+  *ptr = nullptr;
+  *cap = 0;
+  size_t n = std::max(need, (size_t)64);
+  if (hipMalloc(ptr, n * elem) != hipSuccess) return bfail(RHY_ERR_OOM, "batch table allocation");
+  *cap = n;
+  return RHY_OK;
+}
+
+// Rebuild the tile table over the active boxes for the current tile shape;
+// box b's update tiles are contiguous so its CFL partials are
+// [part_begin, part_end).
 int build_tiles(rhy_batch* b) {
   b->htiles.clear();
+  b->active.clear();
+  for (int i = 0; i < (int)b->slots.size(); ++i)
+    if (b->slots[i].active) b->active.push_back(i);
   const int nd = b->ndim;
   for (int a = 0; a < nd; ++a)
-    for (int i = 0; i < b->nboxes; ++i) {
-      const Geom& G = b->hboxes[i].G;
+    for (int i : b->active) {
+      const Geom& G = b->slots[i].B.G;
       const int64_t n = flux_threads(G, a, a == 0 ? b->nf_x : b->nf_yz);
       for (int64_t base = 0; base < n; base += b->block)
         b->htiles.push_back({i, a, base, (int32_t)std::min<int64_t>(b->block, n - base), 0});
     }
   b->nflux_tiles = (int32_t)b->htiles.size();
   int32_t part = 0;
-  for (int i = 0; i < b->nboxes; ++i) {
-    const Geom& G = b->hboxes[i].G;
-    const int64_t n = (int64_t)G.n[0] * G.n[1] * G.n[2];
-    b->hboxes[i].part_begin = part;
+  for (int i : b->active) {
+    BBox& B = b->slots[i].B;
+    const int64_t n = (int64_t)B.G.n[0] * B.G.n[1] * B.G.n[2];
+    B.part_begin = part;
     for (int64_t base = 0; base < n; base += b->block) {
       b->htiles.push_back({i, 3, base, (int32_t)std::min<int64_t>(b->block, n - base), 0});
       ++part;
     }
-    b->hboxes[i].part_end = part;
+    B.part_end = part;
   }
   b->nupd_tiles = part;
-  if (b->dtiles) hipFree(b->dtiles);
-  if (b->partial) hipFree(b->partial);
-  b->dtiles = nullptr;
-  b->partial = nullptr;
-  if (hipMalloc(&b->dtiles, sizeof(BTile) * b->htiles.size()) != hipSuccess ||
-      hipMalloc(&b->partial, sizeof(double) * std::max(1, b->nupd_tiles)) != hipSuccess)
-    return bfail(RHY_ERR_OOM, "tile table allocation failed");
-  BHIP_TRY(hipMemcpy(b->dtiles, b->htiles.data(), sizeof(BTile) * b->htiles.size(),
-                     hipMemcpyHostToDevice));
-  BHIP_TRY(hipMemcpy(b->dboxes, b->hboxes.data(), sizeof(BBox) * b->nboxes,
-                     hipMemcpyHostToDevice));
+  int rc;
+  if ((rc = grow((void**)&b->dtiles, &b->dtiles_cap, b->htiles.size(), sizeof(BTile))) ||
+      (rc = grow((void**)&b->partial, &b->partial_cap, (size_t)std::max(1, part),
+                 sizeof(double))))
+    return rc;
+  if (!b->htiles.empty())
+    BHIP_TRY(hipMemcpy(b->dtiles, b->htiles.data(), sizeof(BTile) * b->htiles.size(),
+                       hipMemcpyHostToDevice));
+  std::vector<BBox> hb(b->slots.size());
+  for (size_t i = 0; i < b->slots.size(); ++i) hb[i] = b->slots[i].B;
+  BHIP_TRY(hipMemcpy(b->dboxes, hb.data(), sizeof(BBox) * hb.size(), hipMemcpyHostToDevice));
+  if (!b->active.empty())
+    BHIP_TRY(hipMemcpy(b->dactive, b->active.data(), sizeof(int32_t) * b->active.size(),
+                       hipMemcpyHostToDevice));
   if (b->graph) {
     hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
+  b->dirty = false;
   return RHY_OK;
 }
 
@@ -308,78 +409,64 @@ unsigned persistent_grid(rhy_batch* b, int32_t ntiles) {
 }
 
 template <int ND, typename R, class PW>
-void launch_flux_phase(rhy_batch* b, const Consts& k) {
+void launch_flux_phase(rhy_batch* b, const Consts& k, int lim) {
   const unsigned g = persistent_grid(b, b->nflux_tiles);
-  const R* c = static_cast<const R*>(b->cells);
-  R* f = static_cast<R*>(b->flux);
-  switch (k.limiter) {
-    case 1:
-      hipLaunchKernelGGL((batch_flux_kernel<ND, 1, R, PW>), dim3(g), dim3(b->block), 0, b->stream,
-                         k, b->dboxes, b->dtiles, b->nflux_tiles, c, f, b->dt, b->nf_x, b->nf_yz);
-      break;
-    case 2:
-      hipLaunchKernelGGL((batch_flux_kernel<ND, 2, R, PW>), dim3(g), dim3(b->block), 0, b->stream,
-                         k, b->dboxes, b->dtiles, b->nflux_tiles, c, f, b->dt, b->nf_x, b->nf_yz);
-      break;
-    case 3:
-      hipLaunchKernelGGL((batch_flux_kernel<ND, 3, R, PW>), dim3(g), dim3(b->block), 0, b->stream,
-                         k, b->dboxes, b->dtiles, b->nflux_tiles, c, f, b->dt, b->nf_x, b->nf_yz);
-      break;
-    default:
-      hipLaunchKernelGGL((batch_flux_kernel<ND, 4, R, PW>), dim3(g), dim3(b->block), 0, b->stream,
-                         k, b->dboxes, b->dtiles, b->nflux_tiles, c, f, b->dt, b->nf_x, b->nf_yz);
-      break;
+  R* a = static_cast<R*>(b->arena);
+#define RHY_BATCH_FLUX(L)                                                                   \
+  hipLaunchKernelGGL((batch_flux_kernel<ND, L, R, PW>), dim3(g), dim3(b->block), 0, b->stream, \
+                     k, b->dboxes, b->dtiles, b->nflux_tiles, a, b->dt, b->nf_x, b->nf_yz)
+  switch (lim) {
+    case 0: RHY_BATCH_FLUX(0); break;
+    case 1: RHY_BATCH_FLUX(1); break;
+    case 2: RHY_BATCH_FLUX(2); break;
+    case 3: RHY_BATCH_FLUX(3); break;
+    default: RHY_BATCH_FLUX(4); break;
   }
+#undef RHY_BATCH_FLUX
 }
 
 template <int ND, typename R>
-void launch_update_phase(rhy_batch* b, const Consts& k, int order, BcTypes bc) {
+void launch_update_phase(rhy_batch* b, const Consts& k, int order) {
   const unsigned g = persistent_grid(b, b->nupd_tiles);
-  R* c = static_cast<R*>(b->cells);
-  const R* f = static_cast<const R*>(b->flux);
+  R* a = static_cast<R*>(b->arena);
   if (order == RHY_MEMORY_INTENSIVE)
     hipLaunchKernelGGL((batch_update_kernel<ND, 2, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, c, f, b->dt, bc,
-                       b->partial);
+                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, a, b->dt, b->partial);
   else
     hipLaunchKernelGGL((batch_update_kernel<ND, 1, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, c, f, b->dt, bc,
-                       b->partial);
+                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, a, b->dt, b->partial);
 }
 
 template <int ND, typename R, class PW>
-void enqueue_step(rhy_batch* b, const Consts& k, int order, BcTypes bc) {
-  hipLaunchKernelGGL(batch_dt_kernel, dim3(b->nboxes), dim3(256), 0, b->stream, b->dboxes,
-                     b->partial, b->dt, b->dtlog, b->step, b->dtlog_steps, b->nboxes);
-  launch_flux_phase<ND, R, PW>(b, k);
-  launch_update_phase<ND, R>(b, k, order, bc);
+void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim) {
+  hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0, b->stream,
+                     b->dboxes, b->dactive, b->partial, b->dt, b->dtlog, b->step, b->dtlog_steps,
+                     (int32_t)b->slots.size());
+  launch_flux_phase<ND, R, PW>(b, k, lim);
+  launch_update_phase<ND, R>(b, k, order);
   hipLaunchKernelGGL(batch_step_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step);
 }
 
 template <int ND, typename R>
-int prologue(rhy_batch* b, const rhy_params_t* p, const Consts& k) {
+int prologue(rhy_batch* b, const Consts& k) {
   // ghost fill of every box, then CFL partials of the current state
-  BHIP_TRY(hipMemcpyAsync(b->inflow, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
-                          hipMemcpyHostToDevice, b->stream));
-  BcTypes bc{};
-  for (int f = 0; f < 6; ++f) bc.t[f] = p->bc_types[f];
-  R* c = static_cast<R*>(b->cells);
-  for (int i = 0; i < b->nboxes; ++i) {
-    const Geom& G = b->hboxes[i].G;
+  R* a = static_cast<R*>(b->arena);
+  for (int i : b->active) {
+    const BBox& B = b->slots[i].B;
     int64_t maxface = 1;
-    for (int a = 0; a < ND; ++a) {
+    for (int ax = 0; ax < ND; ++ax) {
       int64_t m = 1;
       for (int d = 0; d < 3; ++d)
-        if (d != a) m *= G.n[d];
+        if (d != ax) m *= B.G.n[d];
       maxface = std::max(maxface, m);
     }
     hipLaunchKernelGGL((bc_kernel<ND, R>), dim3(cdiv(maxface, 256), 2 * ND), dim3(256), 0,
-                       b->stream, G, bc, b->inflow, b->ncmp, c + b->hboxes[i].cell_off, -1);
+                       b->stream, B.G, B.bc, b->inflow, b->ncmp, a + B.cell_off, -1);
   }
   const unsigned g = persistent_grid(b, b->nupd_tiles);
   hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
                      b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles,
-                     static_cast<const R*>(b->cells), b->partial);
+                     static_cast<const R*>(b->arena), b->partial);
   hipLaunchKernelGGL(batch_set_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step, 0);
   BHIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -388,53 +475,64 @@ int prologue(rhy_batch* b, const rhy_params_t* p, const Consts& k) {
 template <int ND, typename R, class PW>
 int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph) {
   const Consts k = consts_of(p);
-  BcTypes bc{};
-  for (int f = 0; f < 6; ++f) bc.t[f] = p->bc_types[f];
-  // courant * minval(dx) per box
-  bool changed = false;
-  for (auto& B : b->hboxes) {
+  // a fixed batch takes limiter and faces from the parameters, a pool keeps
+  // each box's own
+  bool changed = b->dirty;
+  for (Slot& S : b->slots) {
+    if (!S.active) continue;
+    BBox& B = S.B;
     double m = B.dx[0];
     for (int d = 1; d < ND; ++d) m = std::min(m, B.dx[d]);
     const double cm = p->courant * m;
     changed = changed || cm != B.cmdx;
     B.cmdx = cm;
+    if (!b->pool) {
+      for (int f = 0; f < 6; ++f) {
+        changed = changed || B.bc.t[f] != p->bc_types[f];
+        B.bc.t[f] = p->bc_types[f];
+      }
+      changed = changed || B.limiter != p->limiter;
+      B.limiter = p->limiter;
+    }
   }
-  if (changed) {
-    BHIP_TRY(hipMemcpy(b->dboxes, b->hboxes.data(), sizeof(BBox) * b->nboxes,
-                       hipMemcpyHostToDevice));
+  int rc;
+  if (changed && (rc = build_tiles(b))) return rc;
+  if (b->active.empty()) {
+    b->last_nsteps = 0;
+    return RHY_OK;
   }
-  if (nsteps > b->dtlog_steps) {
+  const int lim = b->pool ? 0 : p->limiter;
+  const int64_t need = (int64_t)nsteps * (int64_t)b->slots.size();
+  if (need > (int64_t)b->dtlog_steps * (int64_t)b->slots.size()) {
     if (b->dtlog) hipFree(b->dtlog);
     b->dtlog = nullptr;
-    if (hipMalloc(&b->dtlog, sizeof(double) * (size_t)nsteps * b->nboxes) != hipSuccess)
+    if (hipMalloc(&b->dtlog, sizeof(double) * (size_t)need) != hipSuccess)
       return bfail(RHY_ERR_OOM, "dt log allocation failed");
     b->dtlog_steps = nsteps;
   }
-  int rc = prologue<ND, R>(b, p, k);
-  if (rc) return rc;
+  BHIP_TRY(hipMemcpyAsync(b->inflow, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
+                          hipMemcpyHostToDevice, b->stream));
+  if ((rc = prologue<ND, R>(b, k))) return rc;
   if (!use_graph) {
-    for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW>(b, k, p->solver_type, bc);
+    for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
     BHIP_TRY(hipGetLastError());
     b->last_nsteps = nsteps;
     return RHY_OK;
   }
-  // the captured step depends on the parameters and the tile shape
+  // the captured step depends on the parameters and the tile table
   uint64_t key = 1469598103934665603ull;
   auto mix = [&key](const void* data, size_t n) {
     const unsigned char* c = static_cast<const unsigned char*>(data);
     for (size_t i = 0; i < n; ++i) key = (key ^ c[i]) * 1099511628211ull;
   };
   mix(p, sizeof(*p));
-  mix(&b->block, sizeof(b->block));
-  mix(&b->nf_x, sizeof(b->nf_x));
-  mix(&b->nf_yz, sizeof(b->nf_yz));
-  mix(&b->grid_cap, sizeof(b->grid_cap));
+  mix(&b->dtlog_steps, sizeof(b->dtlog_steps));
   if (!b->graph || b->graph_key != key) {
     if (b->graph) hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     hipGraph_t g = nullptr;
     BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step<ND, R, PW>(b, k, p->solver_type, bc);
+    enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
     BHIP_TRY(hipStreamEndCapture(b->stream, &g));
     hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -443,6 +541,97 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   }
   for (int32_t s = 0; s < nsteps; ++s) BHIP_TRY(hipGraphLaunch(b->graph, b->stream));
   b->last_nsteps = nsteps;
+  return RHY_OK;
+}
+
+rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t arena_elems,
+                     int32_t max_slots, int* rc) {
+  *rc = rhy_init(0);
+  if (*rc) {
+    bfail(*rc, rhy_last_error());
+    return nullptr;
+  }
+  if (ndim < 1 || ndim > 3 || max_slots < 1) {
+    *rc = bfail(RHY_ERR_ARG, "bad batch arguments");
+    return nullptr;
+  }
+  if (ncmp < ndim + 2 || ncmp > RHY_MAX_CMP) {
+    *rc = bfail(RHY_ERR_ARG, "ncmp out of range");
+    return nullptr;
+  }
+  if (precision != RHY_FP64 && precision != RHY_MIXED) {
+    *rc = bfail(RHY_ERR_ARG, "unsupported precision");
+    return nullptr;
+  }
+  rhy_batch* b = new rhy_batch();
+  b->ndim = ndim;
+  b->ncmp = ncmp;
+  b->precision = precision;
+  b->esz = precision == RHY_MIXED ? sizeof(float) : sizeof(double);
+  b->slots.resize(max_slots);
+  b->arena_elems = arena_elems;
+  if (arena_elems > 0) b->free_blocks[0] = arena_elems;
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      (arena_elems > 0 && hipMalloc(&b->arena, b->esz * (size_t)arena_elems) != hipSuccess) ||
+      hipMalloc(&b->dboxes, sizeof(BBox) * max_slots) != hipSuccess ||
+      hipMalloc(&b->dactive, sizeof(int32_t) * max_slots) != hipSuccess ||
+      hipMalloc(&b->dt, sizeof(double) * max_slots) != hipSuccess ||
+      hipMalloc(&b->step, sizeof(int32_t)) != hipSuccess ||
+      hipMalloc(&b->inflow, sizeof(double) * 6 * RHY_MAX_CMP) != hipSuccess) {
+    rhy_batch_destroy(b);
+    *rc = bfail(RHY_ERR_OOM, "batch allocation failed");
+    return nullptr;
+  }
+  *rc = RHY_OK;
+  return b;
+}
+
+int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* cells,
+            int32_t limiter, const int32_t* bc_types, int32_t* slot_out) {
+  Geom G{};
+  int rc;
+  if ((rc = box_geom(b->ndim, dims, G))) return rc;
+  int slot = -1;
+  for (int i = 0; i < (int)b->slots.size(); ++i)
+    if (!b->slots[i].active) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) return bfail(RHY_ERR_OOM, "no free slot");
+  const int64_t len = box_elems(G, b->ndim, b->ncmp);
+  const int64_t off = arena_alloc(b, len);
+  if (off < 0) return bfail(RHY_ERR_OOM, "box does not fit in the arena");
+  Slot& S = b->slots[slot];
+  S.active = true;
+  S.off = off;
+  S.len = len;
+  S.B = BBox{};
+  S.B.G = G;
+  S.B.cell_off = off;
+  S.B.flux_off = off + round_up(G.plane * b->ncmp);
+  for (int d = 0; d < 3; ++d) S.B.dx[d] = dx[d];
+  S.B.cmdx = -1.0;
+  S.B.limiter = limiter;
+  for (int f = 0; f < 6; ++f) S.B.bc.t[f] = bc_types ? bc_types[f] : 1;
+  b->dirty = true;
+  if (cells) {
+    const size_t n = (size_t)G.plane * b->ncmp;
+    if (b->precision == RHY_MIXED) {
+      std::vector<float> tmp(n);
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
+      BHIP_TRY(hipMemcpyAsync(static_cast<float*>(b->arena) + off, tmp.data(), sizeof(float) * n,
+                              hipMemcpyHostToDevice, b->stream));
+      BHIP_TRY(hipStreamSynchronize(b->stream));
+    } else {
+      BHIP_TRY(hipMemcpyAsync(static_cast<double*>(b->arena) + off, cells, sizeof(double) * n,
+                              hipMemcpyHostToDevice, b->stream));
+      BHIP_TRY(hipStreamSynchronize(b->stream));
+    }
+  } else {
+    BHIP_TRY(hipMemsetAsync(static_cast<char*>(b->arena) + off * b->esz, 0,
+                            b->esz * (size_t)G.plane * b->ncmp, b->stream));
+  }
+  *slot_out = slot;
   return RHY_OK;
 }
 
@@ -457,61 +646,21 @@ int rhy_batch_create(int32_t ndim, int32_t nboxes, const int32_t* dims, const do
   int rc;
   if (!out) return bfail(RHY_ERR_ARG, "out is NULL");
   *out = nullptr;
-  if ((rc = rhy_init(0))) return bfail(rc, rhy_last_error());
-  if (ndim < 1 || ndim > 3 || nboxes < 1 || !dims || !dx)
-    return bfail(RHY_ERR_ARG, "bad batch arguments");
-  if (ncmp < ndim + 2 || ncmp > RHY_MAX_CMP) return bfail(RHY_ERR_ARG, "ncmp out of range");
-  if (precision != RHY_FP64 && precision != RHY_MIXED)
-    return bfail(RHY_ERR_ARG, "unsupported precision");
-  rhy_batch* b = new rhy_batch();
-  b->ndim = ndim;
-  b->ncmp = ncmp;
-  b->precision = precision;
-  b->nboxes = nboxes;
-  b->esz = precision == RHY_MIXED ? sizeof(float) : sizeof(double);
-  int64_t coff = 0, foff = 0;
+  if (nboxes < 1 || !dims || !dx) return bfail(RHY_ERR_ARG, "bad batch arguments");
+  int64_t total = 0;
   for (int i = 0; i < nboxes; ++i) {
-    BBox B{};
-    int64_t st = 1;
-    for (int d = 0; d < 3; ++d) {
-      const int n = dims[3 * i + d];
-      if (d < ndim && n < 2) {
-        delete b;
-        return bfail(RHY_ERR_SHAPE, "active extents must be >= 2");
-      }
-      if (d >= ndim && n != 1) {
-        delete b;
-        return bfail(RHY_ERR_SHAPE, "inactive extents must be 1");
-      }
-      B.G.n[d] = n;
-      B.G.g[d] = d < ndim ? 2 : 0;
-      B.G.s[d] = st;
-      st *= (int64_t)(n + 2 * B.G.g[d]);
-      B.dx[d] = dx[3 * i + d];
+    Geom G{};
+    if ((rc = box_geom(ndim, dims + 3 * i, G))) return rc;
+    total += box_elems(G, ndim, ncmp);
+  }
+  rhy_batch* b = new_batch(ndim, ncmp, precision, total, nboxes, &rc);
+  if (!b) return rc;
+  for (int i = 0; i < nboxes; ++i) {
+    int32_t slot;
+    if ((rc = add_box(b, dims + 3 * i, dx + 3 * i, nullptr, 2, nullptr, &slot))) {
+      rhy_batch_destroy(b);
+      return rc;
     }
-    B.G.plane = st;
-    B.cell_off = coff;
-    B.flux_off = foff;
-    B.cmdx = -1.0;
-    coff += st * ncmp;
-    foff += st * (ndim + 2) * ndim;
-    b->hboxes.push_back(B);
-  }
-  b->cells_elems = coff;
-  b->flux_elems = foff;
-  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&b->cells, b->esz * (size_t)coff) != hipSuccess ||
-      hipMalloc(&b->flux, b->esz * (size_t)foff) != hipSuccess ||
-      hipMalloc(&b->dboxes, sizeof(BBox) * nboxes) != hipSuccess ||
-      hipMalloc(&b->dt, sizeof(double) * nboxes) != hipSuccess ||
-      hipMalloc(&b->step, sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&b->inflow, sizeof(double) * 6 * RHY_MAX_CMP) != hipSuccess) {
-    rhy_batch_destroy(b);
-    return bfail(RHY_ERR_OOM, "batch allocation failed");
-  }
-  if (hipMemsetAsync(b->cells, 0, b->esz * (size_t)coff, b->stream) != hipSuccess) {
-    rhy_batch_destroy(b);
-    return bfail(RHY_ERR_HIP, "initial clear failed");
   }
   if ((rc = build_tiles(b))) {
     rhy_batch_destroy(b);
@@ -521,11 +670,66 @@ int rhy_batch_create(int32_t ndim, int32_t nboxes, const int32_t* dims, const do
   return RHY_OK;
 }
 
+int rhy_pool_create(int32_t ndim, int32_t ncmp, int32_t precision, int64_t budget_bytes,
+                    int32_t max_boxes, rhy_batch_t** out) {
+  int rc;
+  if (!out) return bfail(RHY_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (budget_bytes <= 0) return bfail(RHY_ERR_ARG, "budget must be positive");
+  const size_t esz = precision == RHY_MIXED ? sizeof(float) : sizeof(double);
+  rhy_batch* b = new_batch(ndim, ncmp, precision, budget_bytes / (int64_t)esz / kAlign * kAlign,
+                           max_boxes, &rc);
+  if (!b) return rc;
+  b->pool = true;
+  *out = b;
+  return RHY_OK;
+}
+
+int rhy_pool_box_bytes(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t precision,
+                       int64_t* bytes) {
+  int rc;
+  if (!dims || !bytes) return bfail(RHY_ERR_ARG, "NULL argument");
+  Geom G{};
+  if ((rc = box_geom(ndim, dims, G))) return rc;
+  *bytes = box_elems(G, ndim, ncmp) * (precision == RHY_MIXED ? 4 : 8);
+  return RHY_OK;
+}
+
+int rhy_pool_add(rhy_batch_t* b, const int32_t dims[3], const double dx[3], const double* cells,
+                 int32_t limiter, const int32_t bc_types[6], int32_t* slot) {
+  if (!b || !dims || !dx || !cells || !slot) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (!b->pool) return bfail(RHY_ERR_ARG, "not a pool");
+  if (limiter < 1 || limiter > 4) return bfail(RHY_ERR_ARG, "unknown slope limiter");
+  return add_box(b, dims, dx, cells, limiter, bc_types, slot);
+}
+
+int rhy_pool_remove(rhy_batch_t* b, int32_t slot) {
+  if (!b || slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].active)
+    return bfail(RHY_ERR_ARG, "bad slot");
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  arena_free(b, b->slots[slot].off, b->slots[slot].len);
+  b->slots[slot].active = false;
+  b->dirty = true;
+  return RHY_OK;
+}
+
+int rhy_pool_free_bytes(rhy_batch_t* b, int64_t* free_bytes, int64_t* largest_bytes) {
+  if (!b || !free_bytes || !largest_bytes) return bfail(RHY_ERR_ARG, "NULL argument");
+  int64_t tot = 0, big = 0;
+  for (auto& kv : b->free_blocks) {
+    tot += kv.second;
+    big = std::max(big, kv.second);
+  }
+  *free_bytes = tot * (int64_t)b->esz;
+  *largest_bytes = big * (int64_t)b->esz;
+  return RHY_OK;
+}
+
 int rhy_batch_destroy(rhy_batch_t* b) {
   if (!b) return RHY_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->graph) hipGraphExecDestroy(b->graph);
-  for (void* p : {(void*)b->cells, (void*)b->flux, (void*)b->dboxes, (void*)b->dtiles,
+  for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dtiles,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow})
     if (p) hipFree(p);
@@ -536,41 +740,45 @@ int rhy_batch_destroy(rhy_batch_t* b) {
 
 int rhy_batch_bytes(rhy_batch_t* b, int64_t* bytes) {
   if (!b || !bytes) return bfail(RHY_ERR_ARG, "NULL argument");
-  *bytes = (int64_t)b->esz * (b->cells_elems + b->flux_elems);
+  *bytes = (int64_t)b->esz * b->arena_elems;
   return RHY_OK;
 }
 
+static int slot_ok(rhy_batch_t* b, int32_t slot) {
+  return b && slot >= 0 && slot < (int32_t)b->slots.size() && b->slots[slot].active;
+}
+
 int rhy_batch_upload(rhy_batch_t* b, int32_t box, const double* cells) {
-  if (!b || !cells || box < 0 || box >= b->nboxes) return bfail(RHY_ERR_ARG, "bad upload");
-  const BBox& B = b->hboxes[box];
+  if (!slot_ok(b, box) || !cells) return bfail(RHY_ERR_ARG, "bad upload");
+  const BBox& B = b->slots[box].B;
   const size_t n = (size_t)B.G.plane * b->ncmp;
   if (b->precision == RHY_MIXED) {
     std::vector<float> tmp(n);
     for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
-    BHIP_TRY(hipMemcpyAsync(static_cast<float*>(b->cells) + B.cell_off, tmp.data(),
+    BHIP_TRY(hipMemcpyAsync(static_cast<float*>(b->arena) + B.cell_off, tmp.data(),
                             sizeof(float) * n, hipMemcpyHostToDevice, b->stream));
     BHIP_TRY(hipStreamSynchronize(b->stream));
     return RHY_OK;
   }
-  BHIP_TRY(hipMemcpyAsync(static_cast<double*>(b->cells) + B.cell_off, cells,
+  BHIP_TRY(hipMemcpyAsync(static_cast<double*>(b->arena) + B.cell_off, cells,
                           sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
   BHIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
 }
 
 int rhy_batch_download(rhy_batch_t* b, int32_t box, double* cells) {
-  if (!b || !cells || box < 0 || box >= b->nboxes) return bfail(RHY_ERR_ARG, "bad download");
-  const BBox& B = b->hboxes[box];
+  if (!slot_ok(b, box) || !cells) return bfail(RHY_ERR_ARG, "bad download");
+  const BBox& B = b->slots[box].B;
   const size_t n = (size_t)B.G.plane * b->ncmp;
   if (b->precision == RHY_MIXED) {
     std::vector<float> tmp(n);
-    BHIP_TRY(hipMemcpyAsync(tmp.data(), static_cast<float*>(b->cells) + B.cell_off,
+    BHIP_TRY(hipMemcpyAsync(tmp.data(), static_cast<float*>(b->arena) + B.cell_off,
                             sizeof(float) * n, hipMemcpyDeviceToHost, b->stream));
     BHIP_TRY(hipStreamSynchronize(b->stream));
     for (size_t i = 0; i < n; ++i) cells[i] = (double)tmp[i];
     return RHY_OK;
   }
-  BHIP_TRY(hipMemcpyAsync(cells, static_cast<double*>(b->cells) + B.cell_off,
+  BHIP_TRY(hipMemcpyAsync(cells, static_cast<double*>(b->arena) + B.cell_off,
                           sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
   BHIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
@@ -593,7 +801,8 @@ int rhy_batch_set_tiles(rhy_batch_t* b, int32_t block, int32_t nf_x, int32_t nf_
 int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph) {
   if (!b || !p) return bfail(RHY_ERR_ARG, "NULL argument");
   if (p->ndim != b->ndim) return bfail(RHY_ERR_ARG, "params.ndim does not match the batch");
-  if (p->limiter < 1 || p->limiter > 4) return bfail(RHY_ERR_ARG, "unknown slope limiter");
+  if (!b->pool && (p->limiter < 1 || p->limiter > 4))
+    return bfail(RHY_ERR_ARG, "unknown slope limiter");
   if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
     return bfail(RHY_ERR_SOLVER, "Unknown solver type");
   if (nsteps < 0) return bfail(RHY_ERR_ARG, "nsteps < 0");
@@ -620,7 +829,7 @@ int rhy_batch_sync(rhy_batch_t* b) {
 int rhy_batch_last_dts(rhy_batch_t* b, int32_t n, double* dts) {
   if (!b || !dts || n < 0 || n > b->last_nsteps) return bfail(RHY_ERR_ARG, "bad dts request");
   if (n == 0) return RHY_OK;
-  BHIP_TRY(hipMemcpyAsync(dts, b->dtlog, sizeof(double) * (size_t)n * b->nboxes,
+  BHIP_TRY(hipMemcpyAsync(dts, b->dtlog, sizeof(double) * (size_t)n * b->slots.size(),
                           hipMemcpyDeviceToHost, b->stream));
   BHIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;

@@ -102,6 +102,33 @@ def blast_3d(n=256, seed=0, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, d
     return _fill(3, dims, ncmp, gamma, rho, v, p)
 
 
+def blast_2d(dims, seed=0, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, ncmp=None):
+    """The 2D analogue of blast_3d: unit energy as pressure in the cells whose
+    centre lies within `radius` cells of the box centre, p = (gamma-1) /
+    (N_in * dx^2), over rho = 1 + jitter*U[-1, 1], p = p0, at rest."""
+    g = rng(seed)
+    ncmp = ncmp or ncmp_for(2)
+    ny, nx = dims[1], dims[0]
+    rho = 1.0 + jitter * g.uniform(-1.0, 1.0, size=(ny, nx))
+    j = np.arange(ny)[:, None] + 0.5 - ny / 2.0
+    i = np.arange(nx)[None, :] + 0.5 - nx / 2.0
+    inside = (i * i + j * j) <= radius * radius
+    n_in = int(inside.sum())
+    dx = 1.0 / max(dims)
+    p = np.full((ny, nx), p0)
+    p[inside] = (gamma - 1.0) * 1.0 / (n_in * dx ** 2)
+    v = np.zeros((2, ny, nx))
+    return _fill(2, dims, ncmp, gamma, rho, v, p)
+
+
+def blast(ndim, dims, seed=0, gamma=5.0 / 3.0, ncmp=None):
+    if ndim == 3:
+        return blast_3d(dims=dims, seed=seed, gamma=gamma, ncmp=ncmp)
+    if ndim == 2:
+        return blast_2d(dims, seed=seed, gamma=gamma, ncmp=ncmp)
+    raise ValueError("blast needs ndim 2 or 3")
+
+
 def random_two_state(ndim, dims, seed=0, gamma=5.0 / 3.0, ncmp=None, vmax=0.0):
     """Seeded two-state problem along a random axis: rho, p ~ U[0.1, 1] per side,
     optional velocities U[-vmax, vmax] per side."""

@@ -1,0 +1,261 @@
+"""Serving loop: a stream of independent hydro jobs, continuously batched.
+
+Each job is one box (2D or 3D) with its own initial state, slope limiter,
+face types and step count.  Jobs arrive over time; `HydroServer.serve`
+admits them in arrival order while the resident set fits an HBM budget,
+keeps every admitted job's state resident on the device, advances all
+resident jobs together (one pool per dimensionality -- rhy_pool_* in
+include/rhyme_hip.h -- each on its own stream, so 2D and 3D quanta overlap)
+for a quantum of steps, then retires the jobs that reached their step count
+and refills the freed room from the queue before the next quantum.
+
+A box in a pool evolves exactly as it would alone (its own ghost fill and
+CFL dt), so every job's result is bit-identical to running it by itself
+with hydro.run; the reference has no counterpart (it advances one
+configuration per process, src/main.f90:106-177).
+
+Admission: a job is admitted when the bytes of all resident jobs plus its
+own (state planes + flux scratch, batch.box_bytes) stay within the budget and
+its pool's arena has a free block large enough; otherwise it waits, and so
+does every later job (first come, first served).  Each pool's arena is sized
+to the whole budget, so the budget is the only limit that binds except for
+fragmentation.
+"""
+import collections
+import concurrent.futures
+import itertools
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import hydro, ics
+from .batch import PoolFull, SamrPool, box_bytes
+from .hydro import FP64, ncmp_for
+
+
+@dataclass
+class Job:
+    """One request: box shape and spacing, initial state, limiter, faces."""
+    jid: int
+    ndim: int
+    dims: list
+    dx: list
+    limiter: int
+    bc_types: list
+    steps: int
+    arrival: float = 0.0        # seconds after the start of serve()
+    kind: str = ""
+    seed: int = 0
+    cells: np.ndarray = None    # initial state, hydro.box_shape layout
+    # filled in by the server
+    admitted: float = None
+    finished: float = None
+    done: int = 0
+    slot: int = None
+    result: np.ndarray = None
+    dts: list = field(default_factory=list)
+
+    def zones(self):
+        return int(np.prod(self.dims))
+
+    def make_cells(self, gamma=5.0 / 3.0):
+        """Initial state of a generated job (`kind` two_state or blast)."""
+        if self.kind == "blast":
+            return ics.blast(self.ndim, self.dims, seed=self.seed, gamma=gamma)
+        return ics.random_two_state(self.ndim, self.dims, seed=self.seed, gamma=gamma)
+
+
+def serving_jobs(njobs=1000, rate=40.0, seed=0, steps=100, n2d=(64, 512), n3d=(32, 128)):
+    """The serving workload: Poisson arrivals at `rate` jobs/s, half 2D boxes
+    n x n with n log-uniform in n2d, half 3D cubes n^3 with n log-uniform in
+    n3d, dx = 1/n; seeded two-state (outflow faces) or blast (reflective
+    faces) initial state; limiter drawn uniformly from the four.  Cells are
+    not generated here (Job.make_cells)."""
+    g = ics.rng(seed)
+    gaps = g.exponential(1.0 / rate, size=njobs)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    ndims = np.array([2] * (njobs // 2) + [3] * (njobs - njobs // 2))
+    g.shuffle(ndims)
+    jobs = []
+    for i in range(njobs):
+        nd = int(ndims[i])
+        lo, hi = n2d if nd == 2 else n3d
+        n = int(round(lo * (hi / lo) ** g.uniform()))
+        kind = "blast" if g.uniform() < 0.5 else "two_state"
+        bc = hydro.REFLECTIVE if kind == "blast" else hydro.OUTFLOW
+        jobs.append(Job(jid=i, ndim=nd, dims=[n] * nd, dx=[1.0 / n] * nd,
+                        limiter=int(g.integers(1, 5)), bc_types=[bc] * 6, steps=steps,
+                        arrival=float(arrivals[i]), kind=kind, seed=int(g.integers(1 << 30))))
+    return jobs
+
+
+def job_params(ndim, courant=0.2, solver=hydro.MEMORY_INTENSIVE, tolerance=1e-6,
+               n_iteration=100, limiter=hydro.MINMOD, bc_types=None):
+    """Parameters shared by every job of one dimensionality (the pool's);
+    limiter and faces are per job."""
+    return hydro.make_params(
+        ndim, rp=hydro.RiemannProblem(irs=hydro.Irs(n_iteration=n_iteration,
+                                                    tolerance=tolerance)),
+        sl=hydro.SlopeLimiter(type=limiter, w=0.0), mh=hydro.MusclHancock(solver_type=solver),
+        cfl=hydro.Cfl(courant),
+        bc=hydro.SamrBc(types=bc_types or [hydro.REFLECTIVE] * 6))
+
+
+def run_alone(job, params, precision=FP64):
+    """The job by itself on one box through the drop-in path (hydro.run)."""
+    p = type(params).from_buffer_copy(params)
+    p.limiter = job.limiter
+    for f in range(6):
+        p.bc_types[f] = int(job.bc_types[f])
+    box = hydro.SamrBox(job.ndim, job.dims, precision=precision,
+                        cells=np.array(job.cells, dtype=np.float64))
+    box.upload()
+    dts = hydro.run(p, box, job.dx, job.steps)
+    out = box.download().copy()
+    box.close()
+    return out, dts
+
+
+class HydroServer:
+    """Continuous batching of 2D and 3D jobs under one HBM budget."""
+
+    def __init__(self, budget_bytes, precision=FP64, quantum=10, params=None, max_boxes=4096,
+                 tiles=None, keep_results=True, record_dts=False):
+        self.budget = int(budget_bytes)
+        self.precision = precision
+        self.quantum = int(quantum)
+        self.params = params or {2: job_params(2), 3: job_params(3)}
+        self.pools = {}
+        for nd in (2, 3):
+            self.pools[nd] = SamrPool(nd, self.budget, max_boxes=max_boxes,
+                                      ncmp=self.params[nd].ncmp, precision=precision)
+            if tiles and nd in tiles:
+                self.pools[nd].set_tiles(*tiles[nd])
+        self.keep_results = keep_results
+        self.record_dts = record_dts
+        self.resident = {2: {}, 3: {}}  # slot -> job
+        self.resident_bytes = 0
+        self.peak_bytes = 0
+        self.deferrals = 0
+        self.quanta = 0
+
+    def close(self):
+        for p in self.pools.values():
+            p.close()
+
+    def job_bytes(self, job):
+        return box_bytes(job.ndim, job.dims, self.params[job.ndim].ncmp, self.precision)
+
+    def _try_admit(self, job, now, pending):
+        nb = self.job_bytes(job)
+        if nb > self.budget:
+            raise ValueError("job %d needs %d bytes, more than the whole budget" % (job.jid, nb))
+        if self.resident_bytes + nb > self.budget:
+            return False
+        if job.cells is not None:
+            cells = job.cells
+        elif job.jid in pending:
+            cells = pending[job.jid].result()
+        else:
+            cells = job.make_cells(self.params[job.ndim].gamma)
+        try:
+            slot = self.pools[job.ndim].add(job.dims, job.dx, cells, job.limiter, job.bc_types)
+        except PoolFull:
+            return False
+        pending.pop(job.jid, None)
+        job.slot = slot
+        job.admitted = now
+        self.resident[job.ndim][slot] = job
+        self.resident_bytes += nb
+        self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
+        return True
+
+    def _retire(self, job, now):
+        pool = self.pools[job.ndim]
+        if self.keep_results:
+            job.result = pool.download(job.slot)
+        pool.remove(job.slot)
+        del self.resident[job.ndim][job.slot]
+        self.resident_bytes -= self.job_bytes(job)
+        job.finished = now
+        job.slot = None
+
+    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep, workers=8, lookahead=32):
+        """Serve `jobs` to completion in arrival order; returns them with
+        admitted/finished times (seconds from the start) filled in.  Jobs
+        without `cells` get their initial state generated on `workers` host
+        threads, up to `lookahead` jobs ahead of admission, while the device
+        runs (the generated arrays are dropped once uploaded)."""
+        queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
+        pending = {}
+        ex = None
+        if any(j.cells is None for j in jobs):
+            ex = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
+
+        def prefetch():
+            if ex is None:
+                return
+            for j in itertools.islice(queue, lookahead):
+                if j.cells is None and j.jid not in pending:
+                    pending[j.jid] = ex.submit(j.make_cells, self.params[j.ndim].gamma)
+
+        try:
+            return self._serve(queue, pending, prefetch, clock, sleep, jobs)
+        finally:
+            if ex is not None:
+                ex.shutdown(wait=True, cancel_futures=True)
+
+    def _serve(self, queue, pending, prefetch, clock, sleep, jobs):
+        t0 = clock()
+        left = len(queue)
+        while left:
+            prefetch()
+            now = clock() - t0
+            # admission, first come first served
+            while queue and queue[0].arrival <= now:
+                if not self._try_admit(queue[0], now, pending):
+                    self.deferrals += 1
+                    break
+                queue.popleft()
+            busy = [nd for nd in (2, 3) if self.resident[nd]]
+            if not busy:
+                if queue:
+                    sleep(max(0.0, queue[0].arrival - (clock() - t0)))
+                continue
+            # one quantum on every busy pool: enqueue all, then wait for all
+            steps = {}
+            for nd in busy:
+                res = self.resident[nd]
+                q = min(self.quantum, min(j.steps - j.done for j in res.values()))
+                steps[nd] = q
+                self.pools[nd].run_async(self.params[nd], q, graph=True)
+            for nd in busy:
+                self.pools[nd].sync()
+            self.quanta += 1
+            now = clock() - t0
+            for nd in busy:
+                q = steps[nd]
+                dts = self.pools[nd].last_dts(q) if self.record_dts else None
+                for slot, job in list(self.resident[nd].items()):
+                    if dts is not None:
+                        job.dts.extend(dts[:, slot].tolist())
+                    job.done += q
+                    if job.done >= job.steps:
+                        self._retire(job, now)
+                        left -= 1
+        return jobs
+
+
+def latency_report(jobs):
+    """Completion latency (finish - arrival) percentiles and throughput."""
+    lat = np.array([j.finished - j.arrival for j in jobs])
+    wait = np.array([j.admitted - j.arrival for j in jobs])
+    span = max(j.finished for j in jobs) - min(j.arrival for j in jobs)
+    zu = float(sum(j.zones() * j.steps for j in jobs))
+    return dict(
+        jobs=len(jobs), p50_ms=1e3 * float(np.percentile(lat, 50)),
+        p99_ms=1e3 * float(np.percentile(lat, 99)), max_ms=1e3 * float(lat.max()),
+        wait_p50_ms=1e3 * float(np.percentile(wait, 50)),
+        wait_p99_ms=1e3 * float(np.percentile(wait, 99)),
+        makespan_s=float(span), zone_updates=zu, zone_updates_per_s=zu / span)

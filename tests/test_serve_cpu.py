@@ -1,0 +1,128 @@
+"""Host logic of the serving loop, without a device: the workload generator
+(sizes, halves, arrivals, determinism), first-come-first-served admission
+under the byte budget, quanta that never overshoot a job's step count, and
+the latency report.  The device pools are replaced by a stand-in that keeps
+the same byte accounting (rhy_pool_box_bytes, a host-only call)."""
+import numpy as np
+import pytest
+
+from rhyme_amd import batch, hydro, serve
+
+
+class FakePool:
+    instances = []
+
+    def __init__(self, ndim, budget_bytes, max_boxes=4096, ncmp=None, precision=0):
+        self.ndim, self.budget, self.precision = ndim, budget_bytes, precision
+        self.ncmp = ncmp or hydro.ncmp_for(ndim)
+        self.boxes, self.used, self.runs = {}, 0, []
+        FakePool.instances.append(self)
+
+    def add(self, dims, dx, cells, limiter, bc_types=None):
+        nb = batch.box_bytes(self.ndim, dims, self.ncmp, self.precision)
+        if self.used + nb > self.budget:
+            raise batch.PoolFull("full")
+        slot = next(i for i in range(10 ** 6) if i not in self.boxes)
+        self.boxes[slot] = (list(dims), nb)
+        self.used += nb
+        return slot
+
+    def remove(self, slot):
+        self.used -= self.boxes.pop(slot)[1]
+
+    def download(self, slot):
+        return np.zeros(hydro.box_shape(self.ndim, self.boxes[slot][0], self.ncmp))
+
+    def run_async(self, params, nsteps, graph=True):
+        self.runs.append((nsteps, sorted(self.boxes)))
+
+    def sync(self):
+        pass
+
+    def set_tiles(self, *a):
+        pass
+
+    def last_dts(self, n):
+        return np.zeros((n, 4096))
+
+    def close(self):
+        pass
+
+
+@pytest.fixture
+def fake_pools(monkeypatch):
+    FakePool.instances = []
+    monkeypatch.setattr(serve, "SamrPool", FakePool)
+    return FakePool
+
+
+def test_workload_shape_and_determinism():
+    jobs = serve.serving_jobs(njobs=1000, rate=50.0, seed=7)
+    again = serve.serving_jobs(njobs=1000, rate=50.0, seed=7)
+    assert [(j.dims, j.limiter, j.seed, j.arrival) for j in jobs] == \
+           [(j.dims, j.limiter, j.seed, j.arrival) for j in again]
+    nd = np.array([j.ndim for j in jobs])
+    assert (nd == 2).sum() == 500 and (nd == 3).sum() == 500
+    n2 = np.array([j.dims[0] for j in jobs if j.ndim == 2])
+    n3 = np.array([j.dims[0] for j in jobs if j.ndim == 3])
+    assert n2.min() >= 64 and n2.max() <= 512 and n3.min() >= 32 and n3.max() <= 128
+    # log-uniform: the median size is near the geometric mean of the range
+    assert 150 < np.median(n2) < 210 and 55 < np.median(n3) < 75
+    arr = np.array([j.arrival for j in jobs])
+    assert arr[0] == 0.0 and np.all(np.diff(arr) >= 0)
+    assert abs(arr[-1] / 999 - 1 / 50.0) < 0.2 / 50.0   # mean gap 1/rate
+    assert set(j.limiter for j in jobs) == {1, 2, 3, 4}
+    assert {j.kind for j in jobs} == {"blast", "two_state"}
+    for j in jobs:
+        assert j.bc_types == [hydro.REFLECTIVE if j.kind == "blast" else hydro.OUTFLOW] * 6
+        assert j.dx == [1.0 / j.dims[0]] * j.ndim
+
+
+def test_make_cells_shapes():
+    for j in serve.serving_jobs(njobs=8, seed=1, n2d=(8, 16), n3d=(6, 10)):
+        c = j.make_cells()
+        assert c.shape == hydro.box_shape(j.ndim, j.dims, hydro.ncmp_for(j.ndim))
+        assert np.all(c[0][(slice(2, -2),) * j.ndim] > 0)
+
+
+def test_fifo_admission_under_budget(fake_pools):
+    jobs = serve.serving_jobs(njobs=30, rate=1e9, seed=2, steps=7, n2d=(8, 32), n3d=(6, 16))
+    for j in jobs:
+        j.cells = np.zeros(1)
+    sizes = [batch.box_bytes(j.ndim, j.dims) for j in jobs]
+    budget = max(sizes) * 2
+    srv = serve.HydroServer(budget, quantum=3, keep_results=False)
+    srv.serve(jobs)
+    assert srv.peak_bytes <= budget and srv.deferrals > 0
+    # admission order is arrival order
+    order = sorted(jobs, key=lambda j: (j.admitted, j.jid))
+    assert [j.jid for j in order] == sorted(j.jid for j in jobs)
+    for j in jobs:
+        assert j.done == j.steps == 7
+        assert j.arrival <= j.admitted <= j.finished
+    # no quantum ran past the smallest remaining step count of its pool
+    for pool in FakePool.instances:
+        for q, _ in pool.runs:
+            assert 1 <= q <= 3
+
+
+def test_job_larger_than_budget_is_an_error(fake_pools):
+    jobs = serve.serving_jobs(njobs=2, rate=1e9, seed=2, steps=3, n2d=(64, 64), n3d=(32, 32))
+    for j in jobs:
+        j.cells = np.zeros(1)
+    srv = serve.HydroServer(1 << 16, quantum=3, keep_results=False)
+    with pytest.raises(ValueError):
+        srv.serve(jobs)
+
+
+def test_latency_report_fields():
+    jobs = serve.serving_jobs(njobs=4, rate=10.0, seed=0, steps=2, n2d=(8, 8), n3d=(8, 8))
+    for i, j in enumerate(jobs):
+        j.admitted = j.arrival + 0.01
+        j.finished = j.arrival + 0.1 * (i + 1)
+    rep = serve.latency_report(jobs)
+    assert rep["jobs"] == 4
+    assert np.isclose(rep["p50_ms"], 250.0) and rep["max_ms"] == pytest.approx(400.0)
+    assert rep["zone_updates"] == sum(2 * 8 ** j.ndim for j in jobs)
+    span = max(j.finished for j in jobs) - jobs[0].arrival
+    assert rep["zone_updates_per_s"] == pytest.approx(rep["zone_updates"] / span)

@@ -1,0 +1,114 @@
+"""The pool (rhy_pool_*) and the serving loop on the device: first-fit arena
+accounting, refusal when a box does not fit, and every served job's result
+and dt sequence bit-identical to the same job run alone through the drop-in
+path (hydro.run), while other jobs come and go around it; one small job per
+dimensionality also checked against the CPU oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from rhyme_amd import hydro, serve
+from rhyme_amd.batch import PoolFull, SamrPool, box_bytes
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def small_jobs(n=14, seed=5, steps=11):
+    jobs = serve.serving_jobs(njobs=n, rate=1e6, seed=seed, steps=steps, n2d=(12, 40),
+                              n3d=(8, 18))
+    for j in jobs:
+        j.cells = j.make_cells()
+    return jobs
+
+
+def test_pool_arena_first_fit_and_refusal():
+    say("pool arena accounting")
+    p = serve.job_params(3)
+    dims = [10, 12, 9]
+    nb = box_bytes(3, dims)
+    pool = SamrPool(3, 3 * nb + 100, max_boxes=8)
+    cells = np.zeros(hydro.box_shape(3, dims, p.ncmp))
+    s = [pool.add(dims, [0.1] * 3, cells, hydro.MINMOD) for _ in range(3)]
+    assert sorted(s) == [0, 1, 2]
+    with pytest.raises(PoolFull):
+        pool.add(dims, [0.1] * 3, cells, hydro.MINMOD)
+    free, big = pool.free_bytes()
+    assert free < nb
+    pool.remove(s[1])
+    free, big = pool.free_bytes()
+    assert big >= nb
+    s3 = pool.add(dims, [0.1] * 3, cells, hydro.VAN_LEER)
+    assert s3 == s[1]
+    for x in (s[0], s[2], s3):
+        pool.remove(x)
+    free, big = pool.free_bytes()
+    assert free == big  # coalesced back into one block
+    pool.close()
+
+
+def test_pool_slot_limit():
+    say("pool slot limit")
+    dims = [8, 8]
+    pool = SamrPool(2, 1 << 24, max_boxes=2)
+    cells = np.zeros(hydro.box_shape(2, dims, hydro.ncmp_for(2)))
+    pool.add(dims, [0.1] * 2, cells, hydro.MINMOD)
+    pool.add(dims, [0.1] * 2, cells, hydro.MINMOD)
+    with pytest.raises(PoolFull):
+        pool.add(dims, [0.1] * 2, cells, hydro.MINMOD)
+    pool.close()
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_served_jobs_identical_to_alone(precision):
+    jobs = small_jobs()
+    # budget for about a third of the jobs: admission must defer some
+    sizes = sorted(box_bytes(j.ndim, j.dims, precision=precision) for j in jobs)
+    budget = sum(sizes[-5:])
+    say("serve identity precision", precision, "jobs", len(jobs), "budget", budget)
+    srv = serve.HydroServer(budget, precision=precision, quantum=3, record_dts=True)
+    srv.serve(jobs)
+    assert srv.deferrals > 0 and srv.peak_bytes <= budget
+    for j in jobs:
+        ref, rdts = serve.run_alone(j, srv.params[j.ndim], precision=precision)
+        say(" job", j.jid, j.ndim, j.dims, "lim", j.limiter, j.kind)
+        assert len(j.dts) == j.steps
+        np.testing.assert_array_equal(np.array(j.dts), rdts)
+        np.testing.assert_array_equal(j.result, ref)
+    srv.close()
+
+
+def test_served_jobs_match_oracle():
+    jobs = [j for j in small_jobs(n=6, seed=9, steps=6)]
+    srv = serve.HydroServer(1 << 30, quantum=4)
+    srv.serve(jobs)
+    for j in jobs[:4]:
+        say("served vs oracle job", j.jid, j.ndim, j.dims)
+        p = type(srv.params[j.ndim]).from_buffer_copy(srv.params[j.ndim])
+        p.limiter = j.limiter
+        for f in range(6):
+            p.bc_types[f] = j.bc_types[f]
+        ref = j.cells.copy()
+        oracle.run(oracle.params_from(p), j.dims, ref, j.dx, j.steps)
+        inner = (slice(0, j.ndim + 2),) + tuple(slice(2, -2) for _ in range(j.ndim))
+        for c in range(j.ndim + 2):
+            a, b = j.result[inner][c], ref[inner][c]
+            assert np.abs(a - b).sum() <= 1e-12 * max(np.abs(b).sum(), 1e-300)
+    srv.close()
+
+
+def test_serving_report_small_stream():
+    jobs = serve.serving_jobs(njobs=40, rate=400.0, seed=3, steps=10, n2d=(32, 96),
+                              n3d=(16, 40))
+    for j in jobs:
+        j.cells = j.make_cells()
+    srv = serve.HydroServer(1 << 30, quantum=5, keep_results=False)
+    srv.serve(jobs)
+    rep = serve.latency_report(jobs)
+    say("small stream", rep)
+    assert rep["jobs"] == 40 and rep["p99_ms"] >= rep["p50_ms"] > 0
+    assert all(j.done == j.steps and j.finished >= j.admitted >= j.arrival for j in jobs)
+    srv.close()
