@@ -155,8 +155,9 @@ def serving_leg(njobs, rate, budget_gib, quantum, seed, precision=0):
     state, limiter uniform over the four; admitted first come first served
     under an HBM budget, state resident, continuously batched in quanta.
     Initial states are generated on host threads ahead of admission and
-    uploaded at admission; finished states are downloaded at retirement
-    (both inside the measured latency)."""
+    uploaded at admission (inside the measured latency); a job completes at
+    the end of the quantum in which it reaches its step count, when its slot
+    is freed (final states are not copied back here)."""
     from rhyme_amd import serve
     jobs = serve.serving_jobs(njobs=njobs, rate=rate, seed=seed)
     srv = serve.HydroServer(int(budget_gib * (1 << 30)), precision=precision, quantum=quantum,
@@ -168,7 +169,7 @@ def serving_leg(njobs, rate, budget_gib, quantum, seed, precision=0):
     srv.close()
     rep.update(rate_jobs_per_s=rate, budget_gib=budget_gib, quantum_steps=quantum,
                peak_resident_gib=srv.peak_bytes / float(1 << 30), admission_deferrals=srv.deferrals,
-               quanta=srv.quanta, wall_s=wall, unit="zone-updates/s (aggregate), ms latency",
+               quanta=srv.quanta, wall_s=wall, host_seconds=srv.seconds, unit="zone-updates/s (aggregate), ms latency",
                precision="mixed" if precision else "fp64")
     return rep
 
@@ -194,6 +195,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals, jobs/s")
     ap.add_argument("--serve-budget-gib", type=float, default=16.0)
     ap.add_argument("--serve-quantum", type=int, default=10)
+    ap.add_argument("--serve-saturated-rate", type=float, default=1000.0,
+                    help="second pass with arrivals faster than service (capacity); 0: skip")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run one plain device copy of 1 GiB (torch), the "
                          "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
@@ -259,6 +262,10 @@ def main():
     if not a.skip_serve:
         line["serving"] = serving_leg(a.serve_jobs, a.serve_rate, a.serve_budget_gib,
                                       a.serve_quantum, a.seed)
+        if a.serve_saturated_rate > 0:
+            line["serving"]["saturated"] = serving_leg(
+                a.serve_jobs, a.serve_saturated_rate, a.serve_budget_gib, a.serve_quantum,
+                a.seed)
     if not a.skip_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)

@@ -70,7 +70,7 @@ struct BTile {
 
 // LIM > 0: every box uses that limiter; LIM == 0: each box's own.
 template <int ND, int LIM, typename R, class PW>
-__global__ __launch_bounds__(256) void batch_flux_kernel(Consts k, const BBox* __restrict__ boxes,
+__global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Consts k, const BBox* __restrict__ boxes,
                                                          const BTile* __restrict__ tiles,
                                                          int32_t ntiles, R* __restrict__ arena,
                                                          const double* __restrict__ dt,
@@ -230,6 +230,7 @@ constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) b
 
 struct Slot {
   bool active = false;
+  bool needs_bc = false;  // ghosts not yet filled since the state was set
   BBox B{};
   int64_t off = 0, len = 0;  // arena block (elements)
 };
@@ -248,6 +249,8 @@ struct rhy_batch {
   int32_t nflux_tiles = 0, nupd_tiles = 0;
   int32_t block = 256, nf_x = 4, nf_yz = 8, grid_cap = 0;
   bool dirty = true;
+  bool partials_valid = false;  // CFL partials match the state and tile table
+  rhy_params_t last_params{};
   hipStream_t stream = nullptr;
   void* arena = nullptr;
   BBox* dboxes = nullptr;
@@ -356,6 +359,9 @@ int grow(void** ptr, size_t* cap, size_t need, size_t elem) {
 // box b's update tiles are contiguous so its CFL partials are
 // [part_begin, part_end).
 int build_tiles(rhy_batch* b) {
+  // the tables below are read by work that may still be in flight
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  b->partials_valid = false;
   b->htiles.clear();
   b->active.clear();
   for (int i = 0; i < (int)b->slots.size(); ++i)
@@ -447,12 +453,19 @@ void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim) {
   hipLaunchKernelGGL(batch_step_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step);
 }
 
+// Ghost fill, then CFL partials of the current state.  A fixed batch fills
+// every box at every run, as rhy_run does.  In a pool the fused update keeps
+// the ghosts of a box current from step to step, so only boxes whose state
+// was just set get a fill, and the partials are recomputed only when the
+// tile table, the parameters or a state changed since the last step.
 template <int ND, typename R>
 int prologue(rhy_batch* b, const Consts& k) {
-  // ghost fill of every box, then CFL partials of the current state
   R* a = static_cast<R*>(b->arena);
   for (int i : b->active) {
-    const BBox& B = b->slots[i].B;
+    Slot& S = b->slots[i];
+    if (b->pool && !S.needs_bc) continue;
+    S.needs_bc = false;
+    const BBox& B = S.B;
     int64_t maxface = 1;
     for (int ax = 0; ax < ND; ++ax) {
       int64_t m = 1;
@@ -463,10 +476,13 @@ int prologue(rhy_batch* b, const Consts& k) {
     hipLaunchKernelGGL((bc_kernel<ND, R>), dim3(cdiv(maxface, 256), 2 * ND), dim3(256), 0,
                        b->stream, B.G, B.bc, b->inflow, b->ncmp, a + B.cell_off, -1);
   }
-  const unsigned g = persistent_grid(b, b->nupd_tiles);
-  hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                     b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles,
-                     static_cast<const R*>(b->arena), b->partial);
+  if (!b->pool || !b->partials_valid) {
+    const unsigned g = persistent_grid(b, b->nupd_tiles);
+    hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
+                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles,
+                       static_cast<const R*>(b->arena), b->partial);
+  }
+  b->partials_valid = true;  // from here on every update rewrites them
   hipLaunchKernelGGL(batch_set_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step, 0);
   BHIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -475,6 +491,15 @@ int prologue(rhy_batch* b, const Consts& k) {
 template <int ND, typename R, class PW>
 int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph) {
   const Consts k = consts_of(p);
+  if (std::memcmp(p, &b->last_params, sizeof(*p)) != 0) {
+    // new parameters: inflow states, ghosts and wave speeds all follow them
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    BHIP_TRY(hipMemcpy(b->inflow, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
+                       hipMemcpyHostToDevice));
+    b->last_params = *p;
+    b->partials_valid = false;
+    for (Slot& S : b->slots) S.needs_bc = S.active;
+  }
   // a fixed batch takes limiter and faces from the parameters, a pool keeps
   // each box's own
   bool changed = b->dirty;
@@ -504,14 +529,13 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   const int lim = b->pool ? 0 : p->limiter;
   const int64_t need = (int64_t)nsteps * (int64_t)b->slots.size();
   if (need > (int64_t)b->dtlog_steps * (int64_t)b->slots.size()) {
+    BHIP_TRY(hipStreamSynchronize(b->stream));
     if (b->dtlog) hipFree(b->dtlog);
     b->dtlog = nullptr;
     if (hipMalloc(&b->dtlog, sizeof(double) * (size_t)need) != hipSuccess)
       return bfail(RHY_ERR_OOM, "dt log allocation failed");
     b->dtlog_steps = nsteps;
   }
-  BHIP_TRY(hipMemcpyAsync(b->inflow, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
-                          hipMemcpyHostToDevice, b->stream));
   if ((rc = prologue<ND, R>(b, k))) return rc;
   if (!use_graph) {
     for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
@@ -613,6 +637,7 @@ int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* c
   S.B.cmdx = -1.0;
   S.B.limiter = limiter;
   for (int f = 0; f < 6; ++f) S.B.bc.t[f] = bc_types ? bc_types[f] : 1;
+  S.needs_bc = true;
   b->dirty = true;
   if (cells) {
     const size_t n = (size_t)G.plane * b->ncmp;
@@ -751,6 +776,8 @@ static int slot_ok(rhy_batch_t* b, int32_t slot) {
 int rhy_batch_upload(rhy_batch_t* b, int32_t box, const double* cells) {
   if (!slot_ok(b, box) || !cells) return bfail(RHY_ERR_ARG, "bad upload");
   const BBox& B = b->slots[box].B;
+  b->slots[box].needs_bc = true;
+  b->partials_valid = false;
   const size_t n = (size_t)B.G.plane * b->ncmp;
   if (b->precision == RHY_MIXED) {
     std::vector<float> tmp(n);
@@ -810,13 +837,13 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
   switch (b->ndim) {
     case 1:
       return mixed ? run_t<1, float, PowFast>(b, p, nsteps, use_graph)
-                   : run_t<1, double, PowLib>(b, p, nsteps, use_graph);
+                   : run_t<1, double, PowF64>(b, p, nsteps, use_graph);
     case 2:
       return mixed ? run_t<2, float, PowFast>(b, p, nsteps, use_graph)
-                   : run_t<2, double, PowLib>(b, p, nsteps, use_graph);
+                   : run_t<2, double, PowF64>(b, p, nsteps, use_graph);
     default:
       return mixed ? run_t<3, float, PowFast>(b, p, nsteps, use_graph)
-                   : run_t<3, double, PowLib>(b, p, nsteps, use_graph);
+                   : run_t<3, double, PowF64>(b, p, nsteps, use_graph);
   }
 }
 

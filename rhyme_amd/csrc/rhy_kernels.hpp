@@ -10,9 +10,10 @@
 #include "rhy_physics.hpp"
 
 // Minimum waves per SIMD requested for the flux kernels (caps their VGPRs:
-// 4 -> 128, 3 -> 168, 2 -> 256); 0 leaves the choice to the compiler.
+// 4 -> 128, 3 -> 168, 2 -> 256); 0 leaves the choice to the compiler, which
+// can exceed 256 registers with AGPRs and drop to one wave per SIMD.
 #ifndef RHY_FLUX_WPE
-#define RHY_FLUX_WPE 0
+#define RHY_FLUX_WPE 2
 #endif
 #if RHY_FLUX_WPE > 0
 #define RHY_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RHY_FLUX_WPE)))
@@ -414,9 +415,9 @@ __global__ __launch_bounds__(256) void riemann_batch_kernel(Consts k, int64_t n,
     R[c] = r[q * NV + c];
   }
   const int a = axis[q];
-  if (a == 1) riemann_state<ND, 0>(k, L, R, U);
-  else if (ND > 1 && a == 2) riemann_state<ND, (ND > 1 ? 1 : 0)>(k, L, R, U);
-  else if (ND > 2 && a == 3) riemann_state<ND, (ND > 2 ? 2 : 0)>(k, L, R, U);
+  if (a == 1) riemann_state<ND, 0, PowF64>(k, L, R, U);
+  else if (ND > 1 && a == 2) riemann_state<ND, (ND > 1 ? 1 : 0), PowF64>(k, L, R, U);
+  else if (ND > 2 && a == 3) riemann_state<ND, (ND > 2 ? 2 : 0), PowF64>(k, L, R, U);
   else {
 #pragma unroll
     for (int c = 0; c < NV; ++c) U[c] = __builtin_nan("");

@@ -166,24 +166,40 @@ __device__ __forceinline__ void half_step(const Consts& k, const R* um, const R*
 // -------------------------------------------------- Riemann (IRS) ----
 
 // x**y as the reference evaluates it (libm pow; ocml on the device).
+// pw2 gives x**a and x**b of one base.
 struct PowLib {
   __device__ __forceinline__ static double pw(double x, double y) { return pow(x, y); }
+  __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
+                                             double& xb) {
+    xa = pow(x, a);
+    xb = pow(x, b);
+  }
 };
 
 // x**y = exp(y log x) in plain FP64 for x > 0 (relative error a few 1e-16,
-// not correctly rounded), for the mixed-precision form: log via
-// 2 atanh((m-1)/(m+1)) on m in [1/sqrt2, sqrt2), exp via a Cody-Waite
-// reduction and a degree-13 Taylor polynomial; other x go to pow().
+// not correctly rounded): log via 2 atanh((m-1)/(m+1)) on m in
+// [1/sqrt2, sqrt2), exp via a Cody-Waite reduction and a degree-13 Taylor
+// polynomial; other x, and |y log x| >= 700, go to pow().  pw2 evaluates
+// the log once for both exponents, with the same results as two pw calls.
 struct PowFast {
-  __device__ __forceinline__ static double pw(double x, double y) {
-    if (!(x > 0.0) || !(x < INFINITY)) return pow(x, y);
+  static constexpr double LN2_HI = 6.93147180369123816490e-01;
+  static constexpr double LN2_LO = 1.90821492927058770002e-10;
+
+  // ln x, as e*LN2_HI + (atanh series + e*LN2_LO), for 0 < x < inf
+  __device__ __forceinline__ static double ln(double x) {
     int e;
     double m = frexp(x, &e);
     if (m < 0.70710678118654752440) {
       m = m * 2.0;
       e -= 1;
     }
-    const double s = (m - 1.0) / (m + 1.0);
+    // s = (m - 1) / (m + 1) with a refined hardware reciprocal (m + 1 lies
+    // in [1.7, 2.5]; two Newton steps reach full precision)
+    const double d = m + 1.0;
+    double rd = __builtin_amdgcn_rcp(d);
+    rd = fma(fma(-d, rd, 1.0), rd, rd);
+    rd = fma(fma(-d, rd, 1.0), rd, rd);
+    const double s = (m - 1.0) * rd;
     const double z = s * s;
     double q = 2.0 / 23.0;
     q = fma(q, z, 2.0 / 21.0);
@@ -196,12 +212,13 @@ struct PowFast {
     q = fma(q, z, 2.0 / 7.0);
     q = fma(q, z, 2.0 / 5.0);
     q = fma(q, z, 2.0 / 3.0);
-    const double LN2_HI = 6.93147180369123816490e-01;
-    const double LN2_LO = 1.90821492927058770002e-10;
     const double lm = fma(s * z, q, 2.0 * s);
     const double de = (double)e;
-    const double t = y * (de * LN2_HI + (lm + de * LN2_LO));
-    if (!(fabs(t) < 700.0)) return pow(x, y);
+    return de * LN2_HI + (lm + de * LN2_LO);
+  }
+
+  // e**t for |t| < 700
+  __device__ __forceinline__ static double ex(double t) {
     const double kd = rint(t * 1.44269504088896338700e+00);
     const double r = (t - kd * LN2_HI) - kd * LN2_LO;
     double p = 1.0 / 6227020800.0;  // 1/13!
@@ -219,21 +236,71 @@ struct PowFast {
     p = fma(p, r * r, r);
     return ldexp(1.0 + p, (int)kd);
   }
+
+  __device__ __forceinline__ static double pw(double x, double y) {
+    if (!(x > 0.0) || !(x < INFINITY)) return pow(x, y);
+    const double t = y * ln(x);
+    if (!(fabs(t) < 700.0)) return pow(x, y);
+    return ex(t);
+  }
+
+  __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
+                                             double& xb) {
+    if (!(x > 0.0) || !(x < INFINITY)) {
+      xa = pow(x, a);
+      xb = pow(x, b);
+      return;
+    }
+    const double l = ln(x);
+    const double ta = a * l, tb = b * l;
+    xa = fabs(ta) < 700.0 ? ex(ta) : pow(x, a);
+    xb = fabs(tb) < 700.0 ? ex(tb) : pow(x, b);
+  }
 };
 
-// rhyme_irs_nonlinear_wave_function.f90:13-23
+// Exponentials of the FP64 path: PowFast unless RHY_FP64_POW_FAST=0 selects
+// libm pow (last-bit agreement with a libm host, ~20% slower flux kernels).
+#ifndef RHY_FP64_POW_FAST
+#define RHY_FP64_POW_FAST 1
+#endif
+#if RHY_FP64_POW_FAST
+using PowF64 = PowFast;
+#else
+using PowF64 = PowLib;
+#endif
+
+// rhyme_irs_nonlinear_wave_function.f90:13-23.  The terms that do not
+// depend on p (A_K, B_K of the shock branch, 2 c_K / (gamma-1) and
+// 1 / (rho_K c_K) of the rarefaction branch) are evaluated once per
+// interface side, with the reference's expressions, instead of in every
+// Newton iteration.
+struct WaveSide {
+  double pk, Ak, Bk, c2, irc;
+};
+
+__device__ __forceinline__ WaveSide wave_side(const Consts& k, double rho, double pk,
+                                              double cs) {
+  WaveSide w;
+  w.pk = pk;
+  w.Ak = 2.0 / (k.gp1 * rho);
+  w.Bk = k.gm1 * pk / k.gp1;
+  w.c2 = 2.0 * cs / k.gm1;
+  w.irc = 1.0 / (rho * cs);
+  return w;
+}
+
 template <class PW = PowLib>
-__device__ __forceinline__ void wave_fn(const Consts& k, double rho, double pk, double cs,
-                                        double p, double& f, double& fp) {
-  if (p > pk) {
-    double Ak = 2.0 / (k.gp1 * rho);
-    double Bk = k.gm1 * pk / k.gp1;
-    double factor = sqrt(Ak / (Bk + p));
-    f = (p - pk) * factor;
-    fp = factor * (1.0 - (p - pk) / (2.0 * (Bk + p)));
+__device__ __forceinline__ void wave_fn(const Consts& k, const WaveSide& w, double p, double& f,
+                                        double& fp) {
+  if (p > w.pk) {
+    double factor = sqrt(w.Ak / (w.Bk + p));
+    f = (p - w.pk) * factor;
+    fp = factor * (1.0 - (p - w.pk) / (2.0 * (w.Bk + p)));
   } else {
-    f = 2.0 * cs / k.gm1 * (PW::pw(p / pk, k.gm1_2g) - 1.0);
-    fp = 1.0 / (rho * cs) * PW::pw(p / pk, -k.gp1_2g);
+    double xa, xb;
+    PW::pw2(p / w.pk, k.gm1_2g, -k.gp1_2g, xa, xb);
+    f = w.c2 * (xa - 1.0);
+    fp = w.irc * xb;
   }
 }
 
@@ -300,14 +367,13 @@ __device__ __noinline__ double guess_n(const Consts& k, int i, double lr, double
 // Newton iteration from one starting guess (rhyme_irs_exec.f90:25-39).
 // Leaves lf/rf at the wave-function values of the last evaluated iterate.
 template <class PW = PowLib>
-__device__ __forceinline__ double newton(const Consts& k, double p_star, double lr, double lp,
-                                         double lc, double rr, double rp, double rc, double du,
-                                         double& lf, double& rf) {
+__device__ __forceinline__ double newton(const Consts& k, double p_star, const WaveSide& L,
+                                         const WaveSide& R, double du, double& lf, double& rf) {
   double p_prev = p_star;
   for (int i = 1; i <= k.n_iter; ++i) {
     double lfp, rfp;
-    wave_fn<PW>(k, lr, lp, lc, p_star, lf, lfp);
-    wave_fn<PW>(k, rr, rp, rc, p_star, rf, rfp);
+    wave_fn<PW>(k, L, p_star, lf, lfp);
+    wave_fn<PW>(k, R, p_star, rf, rfp);
     p_star = p_prev - (lf + rf + du) / (lfp + rfp);
     if (p_star < 0.0) break;
     if (2.0 * fabs(p_star - p_prev) / (p_star + p_prev) < k.tol) break;
@@ -322,12 +388,12 @@ __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv
                                            double lc, double rr, double rv, double rp, double rc,
                                            double& lf, double& rf) {
   const double du = rv - lv;
-  double p = newton<PW>(k, guess1<PW>(k, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp,
-                        rc, du, lf, rf);
+  const WaveSide L = wave_side(k, lr, lp, lc);
+  const WaveSide R = wave_side(k, rr, rp, rc);
+  double p = newton<PW>(k, guess1<PW>(k, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
   if (!(p > 0.0)) {
     for (int gi = 1; gi < 6; ++gi) {
-      p = newton<PW>(k, guess_n<PW>(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), lr, lp, lc, rr, rp,
-                     rc, du, lf, rf);
+      p = newton<PW>(k, guess_n<PW>(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
       if (p > 0.0) break;
     }
   }
@@ -335,17 +401,27 @@ __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv
   return p;
 }
 
-// rhyme_riemann_problem_w_kfan.f90:14-31 (is_right selects the sign of cs)
+// rhyme_riemann_problem_w_kfan.f90:14-31 (is_right selects the sign of
+// cs): the primitive state inside a rarefaction fan, velocity in v.
+template <int ND, int A, class PW = PowLib>
+__device__ __forceinline__ void w_kfan_prim(const Consts& k, double srho, const double* sv,
+                                            double sp, double scs, double dxdt, bool is_right,
+                                            double& rho, double* v, double& p) {
+  double cs = is_right ? -scs : scs;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = sv[d];
+  double xa, xb;
+  PW::pw2(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1, 1.0 / k.gm1_2g, xa, xb);
+  rho = srho * xa;
+  v[A] = 2.0 / k.gp1 * (cs + k.gm1 / 2.0 * sv[A] + dxdt);
+  p = sp * xb;
+}
+
 template <int ND, int A, class PW = PowLib>
 __device__ __forceinline__ void w_kfan(const Consts& k, double srho, const double* sv, double sp,
                                        double scs, double dxdt, bool is_right, double* u) {
-  double cs = is_right ? -scs : scs;
-  double v[ND];
-#pragma unroll
-  for (int d = 0; d < ND; ++d) v[d] = sv[d];
-  double rho = srho * PW::pw(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1);
-  v[A] = 2.0 / k.gp1 * (cs + k.gm1 / 2.0 * sv[A] + dxdt);
-  double p = sp * PW::pw(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 1.0 / k.gm1_2g);
+  double rho, v[ND], p;
+  w_kfan_prim<ND, A, PW>(k, srho, sv, sp, scs, dxdt, is_right, rho, v, p);
   prim_to_cons<ND>(k, rho, v, p, u);
 }
 
@@ -433,70 +509,45 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
       double ps = irs_exec<PW>(k, l_rho, lv[A], lp, lc, r_rho, rv[A], rp, rc, lf, rf);
       if (ps < k.w_p) ps = k.w_p;
       double us = 0.5 * ((rv[A] + lv[A]) + (rf - lf));
-      // sampling at x/t = 0: rhyme_riemann_problem_sampling.f90:13-47
-      if (dxdt > us) {  // right of the contact
-        double ps_pr = ps / rp;
-        if (ps > rp) {
-          double speed = rv[A] + rc * sqrt(k.gp1_2g * ps_pr + k.gm1_2g);
-          if (dxdt < speed) {
-            double rho = r_rho * (k.gm1_gp1 + ps_pr) / (k.gm1_gp1 * ps_pr + 1.0);
-            double v[ND];
+      // sampling at x/t = 0 (rhyme_riemann_problem_sampling.f90:13-47).  The
+      // reference's right-of-contact and left-of-contact branches mirror each
+      // other; here one path serves both, on the side K of the sampled point
+      // with the signed sound speed sc = +c_R or -c_L (a - c == a + (-c) and
+      // (-c)*x == -(c*x) exactly, so every value matches its branch).
+      const bool right = dxdt > us;
+      const double kr = right ? r_rho : l_rho;
+      const double kp = right ? rp : lp;
+      const double kc = right ? rc : lc;
+      const double sc = right ? rc : -lc;
+      double kv[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) v[d] = rv[d];
-            v[A] = us;
-            prim_to_cons<ND>(k, rho, v, ps, u);
-          } else {
-            prim_to_cons<ND>(k, r_rho, rv, rp, u);
-          }
-        } else {
-          double fan_cs = rc * PW::pw(ps_pr, k.gm1_2g);
-          double speedH = rv[A] + rc;
-          double speedT = us + fan_cs;
-          if (dxdt < speedT) {
-            double rho = r_rho * PW::pw(ps_pr, k.g_inv);
-            double v[ND];
+      for (int d = 0; d < ND; ++d) kv[d] = right ? rv[d] : lv[d];
+      const double ps_pk = ps / kp;
+      double srho = kr, sp = kp, sv[ND];
 #pragma unroll
-            for (int d = 0; d < ND; ++d) v[d] = rv[d];
-            v[A] = us;
-            prim_to_cons<ND>(k, rho, v, ps, u);
-          } else if (dxdt < speedH) {
-            w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
-          } else {
-            prim_to_cons<ND>(k, r_rho, rv, rp, u);
-          }
+      for (int d = 0; d < ND; ++d) sv[d] = kv[d];
+      if (ps > kp) {  // shock on side K
+        double speed = kv[A] + sc * sqrt(k.gp1_2g * ps_pk + k.gm1_2g);
+        if (right ? dxdt < speed : dxdt > speed) {
+          srho = kr * (k.gm1_gp1 + ps_pk) / (k.gm1_gp1 * ps_pk + 1.0);
+          sv[A] = us;
+          sp = ps;
         }
-      } else {  // left of the contact
-        double ps_pl = ps / lp;
-        if (ps > lp) {
-          double speed = lv[A] - lc * sqrt(k.gp1_2g * ps_pl + k.gm1_2g);
-          if (dxdt > speed) {
-            double rho = l_rho * (k.gm1_gp1 + ps_pl) / (k.gm1_gp1 * ps_pl + 1.0);
-            double v[ND];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) v[d] = lv[d];
-            v[A] = us;
-            prim_to_cons<ND>(k, rho, v, ps, u);
-          } else {
-            prim_to_cons<ND>(k, l_rho, lv, lp, u);
-          }
-        } else {
-          double fan_cs = lc * PW::pw(ps_pl, k.gm1_2g);
-          double speedH = lv[A] - lc;
-          double speedT = us - fan_cs;
-          if (dxdt > speedT) {
-            double rho = l_rho * PW::pw(ps_pl, k.g_inv);
-            double v[ND];
-#pragma unroll
-            for (int d = 0; d < ND; ++d) v[d] = lv[d];
-            v[A] = us;
-            prim_to_cons<ND>(k, rho, v, ps, u);
-          } else if (dxdt > speedH) {
-            w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
-          } else {
-            prim_to_cons<ND>(k, l_rho, lv, lp, u);
-          }
+      } else {  // rarefaction on side K
+        double xa, xb;
+        PW::pw2(ps_pk, k.gm1_2g, k.g_inv, xa, xb);
+        double fan_cs = kc * xa;
+        double speedH = kv[A] + sc;
+        double speedT = us + (right ? fan_cs : -fan_cs);
+        if (right ? dxdt < speedT : dxdt > speedT) {
+          srho = kr * xb;
+          sv[A] = us;
+          sp = ps;
+        } else if (right ? dxdt < speedH : dxdt > speedH) {
+          w_kfan_prim<ND, A, PW>(k, kr, kv, kp, kc, dxdt, right, srho, sv, sp);
         }
       }
+      prim_to_cons<ND>(k, srho, sv, sp, u);
     }
   }
 }

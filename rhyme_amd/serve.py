@@ -139,6 +139,9 @@ class HydroServer:
         self.peak_bytes = 0
         self.deferrals = 0
         self.quanta = 0
+        # host seconds spent admitting (incl. uploads), running quanta
+        # (enqueue to sync), retiring (incl. downloads) and idle
+        self.seconds = dict(admit=0.0, quanta=0.0, retire=0.0, idle=0.0)
 
     def close(self):
         for p in self.pools.values():
@@ -211,17 +214,21 @@ class HydroServer:
         left = len(queue)
         while left:
             prefetch()
-            now = clock() - t0
+            ta = clock()
+            now = ta - t0
             # admission, first come first served
             while queue and queue[0].arrival <= now:
                 if not self._try_admit(queue[0], now, pending):
                     self.deferrals += 1
                     break
                 queue.popleft()
+            tb = clock()
+            self.seconds["admit"] += tb - ta
             busy = [nd for nd in (2, 3) if self.resident[nd]]
             if not busy:
                 if queue:
                     sleep(max(0.0, queue[0].arrival - (clock() - t0)))
+                self.seconds["idle"] += clock() - tb
                 continue
             # one quantum on every busy pool: enqueue all, then wait for all
             steps = {}
@@ -233,7 +240,9 @@ class HydroServer:
             for nd in busy:
                 self.pools[nd].sync()
             self.quanta += 1
-            now = clock() - t0
+            tc = clock()
+            self.seconds["quanta"] += tc - tb
+            now = tc - t0
             for nd in busy:
                 q = steps[nd]
                 dts = self.pools[nd].last_dts(q) if self.record_dts else None
@@ -244,6 +253,7 @@ class HydroServer:
                     if job.done >= job.steps:
                         self._retire(job, now)
                         left -= 1
+            self.seconds["retire"] += clock() - tc
         return jobs
 
 
