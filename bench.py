@@ -78,6 +78,7 @@ def batched_leg(steps, warmup, seed):
     conditions, van Albada, outflow faces, each box with its own dt; every
     phase of a step is one launch over all boxes, the step is replayed from a
     hipGraph, tile shape from the autotuner.  FP64 and mixed."""
+    import numpy as np
     from rhyme_amd import autotune, hydro, ics
     from rhyme_amd.batch import SamrBatch
     boxes = ics.amr_boxes(seed=seed)
@@ -98,9 +99,10 @@ def batched_leg(steps, warmup, seed):
         b.sync()
         el = time.perf_counter() - t0
         zones = b.zones()
+        finite = all(bool(np.isfinite(b.download(i)).all()) for i in range(b.n))
         out[name] = dict(value=zones * steps / el, unit="zone-updates/s",
                          ms_per_step=1e3 * el / steps, zones_per_step=zones,
-                         tiles=list(best), device_bytes=b.nbytes())
+                         tiles=list(best), device_bytes=b.nbytes(), state_finite=finite)
         b.close()
     out["mixed_speedup_vs_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
     return out

@@ -192,7 +192,7 @@ void launch_flux_r(rhy_box* b, const Consts& k, const double* dx) {
 template <int ND>
 int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
   if (b->precision == RHY_MIXED)
-    launch_flux_r<ND, float, PowFast>(b, k, dx);
+    launch_flux_r<ND, float, MixedMath>(b, k, dx);
   else
     launch_flux_r<ND, double, PowF64>(b, k, dx);
   HIP_TRY(hipGetLastError());

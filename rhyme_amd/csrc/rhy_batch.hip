@@ -836,13 +836,13 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
   const bool mixed = b->precision == RHY_MIXED;
   switch (b->ndim) {
     case 1:
-      return mixed ? run_t<1, float, PowFast>(b, p, nsteps, use_graph)
+      return mixed ? run_t<1, float, MixedMath>(b, p, nsteps, use_graph)
                    : run_t<1, double, PowF64>(b, p, nsteps, use_graph);
     case 2:
-      return mixed ? run_t<2, float, PowFast>(b, p, nsteps, use_graph)
+      return mixed ? run_t<2, float, MixedMath>(b, p, nsteps, use_graph)
                    : run_t<2, double, PowF64>(b, p, nsteps, use_graph);
     default:
-      return mixed ? run_t<3, float, PowFast>(b, p, nsteps, use_graph)
+      return mixed ? run_t<3, float, MixedMath>(b, p, nsteps, use_graph)
                    : run_t<3, double, PowF64>(b, p, nsteps, use_graph);
   }
 }

@@ -149,8 +149,9 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 // (loop B, :100-140), stored at the low cell's offset in F (NV planes).
 //
 // R is the storage/bulk type (double: the FP64 drop-in; float: the mixed
-// form, whose half step runs in FP32 and whose Riemann solve is promoted to
-// FP64 with the PowFast exponentials).  LIM is the slope limiter (slid).
+// form, whose half step and the conversions around the Riemann solve run in
+// FP32 while the star-state solve and the sampling run in FP64 with the
+// MixedMath policy).  LIM is the slope limiter (slid).
 // One thread's walk: nf_max consecutive faces normal to A starting at face
 // chunk `t` of the box (thread t of the axis' index space); returns false
 // if t is past the end.
@@ -182,28 +183,52 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
   load_state<ND, R>(cells, G.plane, o, u0);
   load_state<ND, R>(cells, G.plane, o + s, up);
   half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rp);
-  // Only rp (the right state of the low cell) is carried across the
-  // Riemann solve; the three cells of the next half step are re-read
-  // (cache hits), which keeps the register count near the single-face one.
+  if constexpr (sizeof(R) == sizeof(float)) {
+    // FP32 cells: a sliding window carries two cells to the next half step
+    // and loads one new cell, issued before the Riemann solve so that its
+    // latency hides behind it (cell f0 + m + 3 at face m, i.e. at most cell
+    // f0 + nf + 1, the last one the half steps read)
+    R nx[NV];
+    load_state<ND, R>(cells, G.plane, o + 2 * s, nx);
 #pragma unroll 1
-  for (int m = 0; m < nf; ++m) {
-    const int64_t oc = o + (m + 1) * s;
-    load_state<ND, R>(cells, G.plane, oc - s, um);
-    load_state<ND, R>(cells, G.plane, oc, u0);
-    load_state<ND, R>(cells, G.plane, oc + s, up);
-    half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
-    double Ld[NV], Rd[NV], Fd[NV];
+    for (int m = 0; m < nf; ++m) {
 #pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      Ld[c] = (double)rp[c];
-      Rd[c] = (double)lw[c];
+      for (int c = 0; c < NV; ++c) {
+        um[c] = u0[c];
+        u0[c] = up[c];
+        up[c] = nx[c];
+      }
+      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+      if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
+      R Fr[NV];
+      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+      const int64_t of = o + m * s;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        F[of + c * G.plane] = Fr[c];
+        rp[c] = rn[c];
+      }
     }
-    riemann_flux<ND, A, PW>(k, Ld, Rd, Fd);
-    const int64_t of = o + m * s;
+  } else {
+    // FP64 cells: only rp (the right state of the low cell) is carried
+    // across the Riemann solve; the three cells of the next half step are
+    // re-read (cache hits), which keeps the register count near the
+    // single-face one.
+#pragma unroll 1
+    for (int m = 0; m < nf; ++m) {
+      const int64_t oc = o + (m + 1) * s;
+      load_state<ND, R>(cells, G.plane, oc - s, um);
+      load_state<ND, R>(cells, G.plane, oc, u0);
+      load_state<ND, R>(cells, G.plane, oc + s, up);
+      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+      R Fr[NV];
+      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+      const int64_t of = o + m * s;
 #pragma unroll
-    for (int c = 0; c < NV; ++c) {
-      F[of + c * G.plane] = (R)Fd[c];
-      rp[c] = rn[c];
+      for (int c = 0; c < NV; ++c) {
+        F[of + c * G.plane] = Fr[c];
+        rp[c] = rn[c];
+      }
     }
   }
 }

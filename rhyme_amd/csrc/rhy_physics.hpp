@@ -42,6 +42,24 @@ __device__ __forceinline__ double tiny_of<double>() { return kTiny; }
 template <>
 __device__ __forceinline__ float tiny_of<float>() { return 1.17549435e-38f; }
 
+// a / b and sqrt in the bulk type: IEEE for double (the FP64 path, whose
+// results match the oracle bit for bit), hardware reciprocal times a and
+// hardware square root for float (the mixed form; <= 1.5 ulp).
+template <typename R>
+__device__ __forceinline__ R rdiv(R a, R b);
+template <>
+__device__ __forceinline__ double rdiv<double>(double a, double b) { return a / b; }
+template <>
+__device__ __forceinline__ float rdiv<float>(float a, float b) {
+  return a * __builtin_amdgcn_rcpf(b);
+}
+template <typename R>
+__device__ __forceinline__ R root(R x);
+template <>
+__device__ __forceinline__ double root<double>(double x) { return sqrt(x); }
+template <>
+__device__ __forceinline__ float root<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // ---------------------------------------------------------------- EOS ----
 
 // rhyme_hydro_base_specific_kinetic_energy.f90:9
@@ -50,13 +68,13 @@ __device__ __forceinline__ R sp_kin_e(const R* u) {
   R s = R(0);
 #pragma unroll
   for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
-  return R(.5) * s / (u[0] * u[0]);
+  return rdiv<R>(R(.5) * s, u[0] * u[0]);
 }
 
 // rhyme_hydro_base_specific_internal_energy.f90:9
 template <int ND, typename R = double>
 __device__ __forceinline__ R sp_int_e(const R* u) {
-  return u[ND + 1] / u[0] - sp_kin_e<ND, R>(u);
+  return rdiv<R>(u[ND + 1], u[0]) - sp_kin_e<ND, R>(u);
 }
 
 // rhyme_ideal_gas_pressure.f90:9
@@ -66,10 +84,10 @@ __device__ __forceinline__ R pressure(const Consts& k, const R* u) {
 }
 
 // rhyme_ideal_gas_speed_of_sound.f90:9 with temperature_per_mu.f90:9-10
-template <int ND>
-__device__ __forceinline__ double speed_of_sound(const Consts& k, const double* u) {
-  double t_mu = sp_int_e<ND>(u) * (k.g - 1.0) / k.kb_amu;
-  return sqrt(k.g * k.kb_amu * t_mu);
+template <int ND, typename R = double>
+__device__ __forceinline__ R speed_of_sound(const Consts& k, const R* u) {
+  R t_mu = rdiv<R>(sp_int_e<ND, R>(u) * R(k.g - 1.0), R(k.kb_amu));
+  return root<R>(R(k.g) * R(k.kb_amu) * t_mu);
 }
 
 // rhyme_ideal_gas_flux.f90:12-23; A is the 0-based axis.
@@ -83,22 +101,21 @@ __device__ __forceinline__ void flux(const Consts& k, const R* u, R* f) {
   R p = pressure<ND, R>(k, u);
   f[0] = u[1 + A];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) f[1 + d] = u[1 + d] * f[0] / u[0];
+  for (int d = 0; d < ND; ++d) f[1 + d] = rdiv<R>(u[1 + d] * f[0], u[0]);
   f[1 + A] = f[1 + A] + p;
-  f[ND + 1] = f[0] / u[0] * (u[ND + 1] + p);
+  f[ND + 1] = rdiv<R>(f[0], u[0]) * (u[ND + 1] + p);
 }
 
 // rhyme_ideal_gas_primitive_vars_to_conserved.f90:9-11
-template <int ND>
-__device__ __forceinline__ void prim_to_cons(const Consts& k, double rho, const double* v,
-                                             double p, double* u) {
-  double s = 0.0;
+template <int ND, typename R = double>
+__device__ __forceinline__ void prim_to_cons(const Consts& k, R rho, const R* v, R p, R* u) {
+  R s = R(0);
   u[0] = rho;
 #pragma unroll
   for (int d = 0; d < ND; ++d) u[1 + d] = rho * v[d];
 #pragma unroll
   for (int d = 0; d < ND; ++d) s += v[d] * v[d];
-  u[ND + 1] = .5 * rho * s + p / (k.g - 1.0);
+  u[ND + 1] = R(.5) * rho * s + rdiv<R>(p, R(k.g - 1.0));
 }
 
 // ------------------------------------------------------- slope limiter ----
@@ -120,16 +137,16 @@ __device__ __forceinline__ R limit_one(int type, R w, R ul, R u, R ur) {
   R dR = ur - u;
   R d = R(.5) * (R(1) + w) * dL + R(.5) * (R(1) + w) * dR;
   R denom = ur - u;
-  R r = fabs(denom) < tiny_of<R>() ? R(0) : (u - ul) / denom;
+  R r = fabs(denom) < tiny_of<R>() ? R(0) : rdiv<R>(u - ul, denom);
   if (!(r > R(0))) return R(0);  // every limiter gives 0 for r <= 0
-  R xi = R(2) / (R(1) - w + (R(1) + w) * r);
+  R xi = rdiv<R>(R(2), R(1) - w + (R(1) + w) * r);
   switch (t) {
     case 1:  // van Leer
-      return rmin(R(2) * r / (R(1) + r), xi) * d;
+      return rmin(rdiv<R>(R(2) * r, R(1) + r), xi) * d;
     case 2:  // minmod
       return r > R(1) ? rmin(R(1), xi) * d : r * d;
     case 3:  // van Albada
-      return rmin(r * (R(1) + r) / (R(1) + r * r), xi) * d;
+      return rmin(rdiv<R>(r * (R(1) + r), R(1) + r * r), xi) * d;
     case 4:  // superbee
       if (r > R(1)) return rmin(rmin(r, xi), R(2)) * d;
       if (r > R(.5)) return d;
@@ -165,9 +182,17 @@ __device__ __forceinline__ void half_step(const Consts& k, const R* um, const R*
 
 // -------------------------------------------------- Riemann (IRS) ----
 
-// x**y as the reference evaluates it (libm pow; ocml on the device).
-// pw2 gives x**a and x**b of one base.
+// Arithmetic policies of the FP64 Riemann solve: pw(x, y) = x**y, pw2 gives
+// x**a and x**b of one base, dv(a, b) = a / b, sq(x) = sqrt(x); kFast lets
+// the wave function use algebraically equal forms with fewer divisions and
+// powers (the reference's forms otherwise).
+//
+// PowLib: x**y as the reference evaluates it (libm pow; ocml on the device),
+// IEEE division and sqrt.
 struct PowLib {
+  static constexpr bool kFast = false;
+  __device__ __forceinline__ static double dv(double a, double b) { return a / b; }
+  __device__ __forceinline__ static double sq(double x) { return sqrt(x); }
   __device__ __forceinline__ static double pw(double x, double y) { return pow(x, y); }
   __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
                                              double& xb) {
@@ -176,12 +201,16 @@ struct PowLib {
   }
 };
 
-// x**y = exp(y log x) in plain FP64 for x > 0 (relative error a few 1e-16,
-// not correctly rounded): log via 2 atanh((m-1)/(m+1)) on m in
-// [1/sqrt2, sqrt2), exp via a Cody-Waite reduction and a degree-13 Taylor
-// polynomial; other x, and |y log x| >= 700, go to pow().  pw2 evaluates
-// the log once for both exponents, with the same results as two pw calls.
+// PowFast: IEEE division and sqrt; x**y = exp(y log x) in plain FP64 for
+// x > 0 (relative error a few 1e-16, not correctly rounded): log via
+// 2 atanh((m-1)/(m+1)) on m in [1/sqrt2, sqrt2), exp via a Cody-Waite
+// reduction and a degree-13 Taylor polynomial; other x, and |y log x| >= 700,
+// go to pow().  pw2 evaluates the log once for both exponents, with the same
+// results as two pw calls.
 struct PowFast {
+  static constexpr bool kFast = false;
+  __device__ __forceinline__ static double dv(double a, double b) { return a / b; }
+  __device__ __forceinline__ static double sq(double x) { return sqrt(x); }
   static constexpr double LN2_HI = 6.93147180369123816490e-01;
   static constexpr double LN2_LO = 1.90821492927058770002e-10;
 
@@ -258,6 +287,32 @@ struct PowFast {
   }
 };
 
+// MixedMath (the mixed form): FP64 throughout, with PowFast exponentials;
+// a / b as a times the hardware reciprocal of b refined by one Newton step
+// (within ~1e-15 of the IEEE quotient), sqrt from the hardware reciprocal
+// square root refined by a Newton and a Goldschmidt step (within a few ulp);
+// arguments outside [2^-1000, 2^1000], zero and non-finite ones included,
+// take the IEEE operation.
+struct MixedMath : PowFast {
+  static constexpr bool kFast = true;
+  __device__ __forceinline__ static double dv(double a, double b) {
+    const double ab = fabs(b);
+    if (!(ab > 0x1p-1000 && ab < 0x1p+1000)) return a / b;
+    double r = __builtin_amdgcn_rcp(b);
+    r = fma(fma(-b, r, 1.0), r, r);
+    return a * r;
+  }
+  __device__ __forceinline__ static double sq(double x) {
+    if (!(x > 0x1p-1000 && x < 0x1p+1000)) return sqrt(x);
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    const double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    return fma(fma(-g, g, x), h, g);
+  }
+};
+
 // Exponentials of the FP64 path: PowFast unless RHY_FP64_POW_FAST=0 selects
 // libm pow (last-bit agreement with a libm host, ~20% slower flux kernels).
 #ifndef RHY_FP64_POW_FAST
@@ -278,29 +333,46 @@ struct WaveSide {
   double pk, Ak, Bk, c2, irc;
 };
 
+template <class PW = PowLib>
 __device__ __forceinline__ WaveSide wave_side(const Consts& k, double rho, double pk,
                                               double cs) {
   WaveSide w;
   w.pk = pk;
-  w.Ak = 2.0 / (k.gp1 * rho);
-  w.Bk = k.gm1 * pk / k.gp1;
-  w.c2 = 2.0 * cs / k.gm1;
-  w.irc = 1.0 / (rho * cs);
+  w.Ak = PW::dv(2.0, k.gp1 * rho);
+  w.Bk = PW::dv(k.gm1 * pk, k.gp1);
+  w.c2 = PW::dv(2.0 * cs, k.gm1);
+  w.irc = PW::dv(1.0, rho * cs);
   return w;
 }
 
 template <class PW = PowLib>
 __device__ __forceinline__ void wave_fn(const Consts& k, const WaveSide& w, double p, double& f,
                                         double& fp) {
-  if (p > w.pk) {
-    double factor = sqrt(w.Ak / (w.Bk + p));
-    f = (p - w.pk) * factor;
-    fp = factor * (1.0 - (p - w.pk) / (2.0 * (w.Bk + p)));
+  if constexpr (PW::kFast) {
+    // one reciprocal of B_K + p for both shock terms, and
+    // (p/p_K)**(-(gamma+1)/2gamma) = (p/p_K)**((gamma-1)/2gamma) / (p/p_K)
+    if (p > w.pk) {
+      const double ib = PW::dv(1.0, w.Bk + p);
+      const double factor = PW::sq(w.Ak * ib);
+      f = (p - w.pk) * factor;
+      fp = factor * (1.0 - 0.5 * (p - w.pk) * ib);
+    } else {
+      const double x = PW::dv(p, w.pk);
+      const double xa = PW::pw(x, k.gm1_2g);
+      f = w.c2 * (xa - 1.0);
+      fp = w.irc * PW::dv(xa, x);
+    }
   } else {
-    double xa, xb;
-    PW::pw2(p / w.pk, k.gm1_2g, -k.gp1_2g, xa, xb);
-    f = w.c2 * (xa - 1.0);
-    fp = w.irc * xb;
+    if (p > w.pk) {
+      double factor = sqrt(PW::dv(w.Ak, w.Bk + p));
+      f = (p - w.pk) * factor;
+      fp = factor * (1.0 - PW::dv(p - w.pk, 2.0 * (w.Bk + p)));
+    } else {
+      double xa, xb;
+      PW::pw2(PW::dv(p, w.pk), k.gm1_2g, -k.gp1_2g, xa, xb);
+      f = w.c2 * (xa - 1.0);
+      fp = w.irc * xb;
+    }
   }
 }
 
@@ -313,20 +385,21 @@ __device__ __forceinline__ double guess1(const Consts& k, double lr, double lv, 
   double ppv = dmax(0.5 * (lp + rp) + 0.5 * (lv - rv) * cup, 0.0);
   double p_min = dmin(lp, rp);
   double p_max = dmax(lp, rp);
-  double q_max = p_max / p_min;
+  double q_max = PW::dv(p_max, p_min);
   double g;
   if (q_max <= 2.0 && (p_min < ppv && ppv < p_max)) {
     g = ppv;
   } else if (ppv < p_min) {
-    double pq = PW::pw(lp / rp, k.gm1_2g);
-    double um = (pq * lv / lc + rv / rc + 2.0 / k.gm1 * (pq - 1.0)) / (pq / lc + 1.0 / rc);
-    double ptl = 1.0 + k.gm1 / 2.0 * (lv - um) / lc;
-    double ptr = 1.0 + k.gm1 / 2.0 * (um - rv) / rc;
-    g = 0.5 * (lp * PW::pw(ptl, 1.0 / k.gm1_2g) + rp * PW::pw(ptr, 1.0 / k.gm1_2g));
+    double pq = PW::pw(PW::dv(lp, rp), k.gm1_2g);
+    double um = PW::dv(PW::dv(pq * lv, lc) + PW::dv(rv, rc) + PW::dv(2.0, k.gm1) * (pq - 1.0),
+                       PW::dv(pq, lc) + PW::dv(1.0, rc));
+    double ptl = 1.0 + PW::dv(PW::dv(k.gm1, 2.0) * (lv - um), lc);
+    double ptr = 1.0 + PW::dv(PW::dv(k.gm1, 2.0) * (um - rv), rc);
+    g = 0.5 * (lp * PW::pw(ptl, PW::dv(1.0, k.gm1_2g)) + rp * PW::pw(ptr, PW::dv(1.0, k.gm1_2g)));
   } else {
-    double gel = sqrt(((2.0 / (k.gp1)) / lr) / (k.gm1_gp1 * lp + ppv));
-    double ger = sqrt(((2.0 / (k.gp1)) / rr) / (k.gm1_gp1 * rp + ppv));
-    g = (gel * lp + ger * rp - (rv - lv)) / (gel + ger);
+    double gel = PW::sq(PW::dv(PW::dv(PW::dv(2.0, k.gp1), lr), k.gm1_gp1 * lp + ppv));
+    double ger = PW::sq(PW::dv(PW::dv(PW::dv(2.0, k.gp1), rr), k.gm1_gp1 * rp + ppv));
+    g = PW::dv(gel * lp + ger * rp - (rv - lv), gel + ger);
   }
   return dmax(g, kTiny);
 }
@@ -374,9 +447,9 @@ __device__ __forceinline__ double newton(const Consts& k, double p_star, const W
     double lfp, rfp;
     wave_fn<PW>(k, L, p_star, lf, lfp);
     wave_fn<PW>(k, R, p_star, rf, rfp);
-    p_star = p_prev - (lf + rf + du) / (lfp + rfp);
+    p_star = p_prev - PW::dv(lf + rf + du, lfp + rfp);
     if (p_star < 0.0) break;
-    if (2.0 * fabs(p_star - p_prev) / (p_star + p_prev) < k.tol) break;
+    if (PW::dv(2.0 * fabs(p_star - p_prev), p_star + p_prev) < k.tol) break;
     p_prev = p_star;
   }
   return p_star;
@@ -388,8 +461,8 @@ __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv
                                            double lc, double rr, double rv, double rp, double rc,
                                            double& lf, double& rf) {
   const double du = rv - lv;
-  const WaveSide L = wave_side(k, lr, lp, lc);
-  const WaveSide R = wave_side(k, rr, rp, rc);
+  const WaveSide L = wave_side<PW>(k, lr, lp, lc);
+  const WaveSide R = wave_side<PW>(k, rr, rp, rc);
   double p = newton<PW>(k, guess1<PW>(k, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
   if (!(p > 0.0)) {
     for (int gi = 1; gi < 6; ++gi) {
@@ -411,97 +484,92 @@ __device__ __forceinline__ void w_kfan_prim(const Consts& k, double srho, const 
 #pragma unroll
   for (int d = 0; d < ND; ++d) v[d] = sv[d];
   double xa, xb;
-  PW::pw2(2.0 / k.gp1 + k.gm1_gp1 / cs * (sv[A] - dxdt), 2.0 / k.gm1, 1.0 / k.gm1_2g, xa, xb);
+  PW::pw2(PW::dv(2.0, k.gp1) + PW::dv(k.gm1_gp1, cs) * (sv[A] - dxdt), PW::dv(2.0, k.gm1),
+          PW::dv(1.0, k.gm1_2g), xa, xb);
   rho = srho * xa;
-  v[A] = 2.0 / k.gp1 * (cs + k.gm1 / 2.0 * sv[A] + dxdt);
+  v[A] = PW::dv(2.0, k.gp1) * (cs + PW::dv(k.gm1, 2.0) * sv[A] + dxdt);
   p = sp * xb;
 }
 
-template <int ND, int A, class PW = PowLib>
-__device__ __forceinline__ void w_kfan(const Consts& k, double srho, const double* sv, double sp,
-                                       double scs, double dxdt, bool is_right, double* u) {
-  double rho, v[ND], p;
-  w_kfan_prim<ND, A, PW>(k, srho, sv, sp, scs, dxdt, is_right, rho, v, p);
-  prim_to_cons<ND>(k, rho, v, p, u);
-}
-
-// conv_prim_to_cons(rp%w_vacuum, u): velocities of w_vacuum are 0.
-template <int ND>
-__device__ __forceinline__ void vacuum_state(const Consts& k, double* u) {
-  double v[ND];
+// Primitive variables of one side of an interface (…_solve.f90:70-75):
+// v = m / rho, p = max(calc_p, w_vacuum(p)), c = max(calc_cs, tiny), formed
+// in the type of the half states (FP32 in the mixed form) and widened.
+template <int ND, typename RI>
+__device__ __forceinline__ void side_prims(const Consts& k, const RI* s, double* v, double& p,
+                                           double& c) {
 #pragma unroll
-  for (int d = 0; d < ND; ++d) v[d] = 0.0;
-  prim_to_cons<ND>(k, k.w_rho, v, k.w_p, u);
+  for (int d = 0; d < ND; ++d) v[d] = (double)rdiv<RI>(s[1 + d], s[0]);
+  p = dmax((double)pressure<ND, RI>(k, s), k.w_p);
+  c = dmax((double)speed_of_sound<ND, RI>(k, s), kTiny);
 }
 
 // rhyme_riemann_problem_solve.f90:3-104 at dx = 0 (Godunov state, x/t = 0):
-// the conserved state u sampled on the interface.  l is the right half-step
-// state of the cell on the low side of the interface, r the left half-step
-// state of the cell on the high side.
-template <int ND, int A, class PW = PowLib>
-__device__ __forceinline__ void riemann_state(const Consts& k, const double* l, const double* r,
-                                              double* u) {
+// the primitive state (rho, v, p) sampled on the interface.  l is the right
+// half-step state of the cell on the low side of the interface, r the left
+// half-step state of the cell on the high side.  Every branch of the
+// reference ends in conv_prim_vars_to_cons of the state chosen here.
+template <int ND, int A, class PW, typename RI>
+__device__ __forceinline__ void riemann_sample(const Consts& k, const RI* l, const RI* r,
+                                               double& srho, double* sv, double& sp) {
   const double dxdt = 0.0;  // dx/dt with dx = 0 and dt > 0
-  double l_rho = dmax(l[0], k.w_rho);
-  double r_rho = dmax(r[0], k.w_rho);
+  const double l_rho = dmax((double)l[0], k.w_rho);
+  const double r_rho = dmax((double)r[0], k.w_rho);
   const double e_rho = k.e_rho;
-  bool vac_r = r_rho < e_rho && l_rho > e_rho;
-  bool vac_l = l_rho < e_rho && r_rho > e_rho;
-  bool vac_b = l_rho < e_rho && r_rho < e_rho;
-  double lv[ND], rv[ND];
+  const bool vac_r = r_rho < e_rho && l_rho > e_rho;
+  const bool vac_l = l_rho < e_rho && r_rho > e_rho;
+  const bool vac_b = l_rho < e_rho && r_rho < e_rho;
+  double lv[ND], rv[ND], lp, lc, rp, rc;
+  // (the reference converts only the non-vacuum side in the one-sided
+  // vacuum branches; the other side's values are unused there)
+  side_prims<ND, RI>(k, l, lv, lp, lc);
+  side_prims<ND, RI>(k, r, rv, rp, rc);
+  // default: the vacuum state w_vacuum, at rest
+  srho = k.w_rho;
+  sp = k.w_p;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) sv[d] = 0.0;
+  auto take = [&](double rho, const double* v, double p) {
+    srho = rho;
+    sp = p;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) sv[d] = v[d];
+  };
 
   if (vac_r) {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) lv[d] = l[1 + d] / l[0];
-    double lp = dmax(pressure<ND>(k, l), k.w_p);
-    double lc = dmax(speed_of_sound<ND>(k, l), kTiny);
-    double s_starl = lv[A] - 2 * lc / k.gm1;
-    if (dxdt > s_starl)
-      vacuum_state<ND>(k, u);
-    else if (dxdt > lv[A] - lc)
-      w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
-    else
-      prim_to_cons<ND>(k, l_rho, lv, lp, u);
+    double s_starl = lv[A] - PW::dv(2 * lc, k.gm1);
+    if (dxdt > s_starl) {
+      // vacuum
+    } else if (dxdt > lv[A] - lc) {
+      w_kfan_prim<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, srho, sv, sp);
+    } else {
+      take(l_rho, lv, lp);
+    }
   } else if (vac_l) {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) rv[d] = r[1 + d] / r[0];
-    double rp = dmax(pressure<ND>(k, r), k.w_p);
-    double rc = dmax(speed_of_sound<ND>(k, r), kTiny);
-    double s_starr = rv[A] + 2 * rc / k.gm1;
+    double s_starr = rv[A] + PW::dv(2 * rc, k.gm1);
     if (dxdt > rv[A] + rc)
-      prim_to_cons<ND>(k, r_rho, rv, rp, u);
+      take(r_rho, rv, rp);
     else if (dxdt > s_starr)
-      w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
-    else
-      vacuum_state<ND>(k, u);
+      w_kfan_prim<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, srho, sv, sp);
   } else if (vac_b) {
-    vacuum_state<ND>(k, u);
+    // vacuum
   } else {
-#pragma unroll
-    for (int d = 0; d < ND; ++d) lv[d] = l[1 + d] / l[0];
-    double lp = dmax(pressure<ND>(k, l), k.w_p);
-    double lc = dmax(speed_of_sound<ND>(k, l), kTiny);
-#pragma unroll
-    for (int d = 0; d < ND; ++d) rv[d] = r[1 + d] / r[0];
-    double rp = dmax(pressure<ND>(k, r), k.w_p);
-    double rc = dmax(speed_of_sound<ND>(k, r), kTiny);
-    double s_starl = lv[A] - 2 * lc / k.gm1;
-    double s_starr = rv[A] + 2 * rc / k.gm1;
+    double s_starl = lv[A] - PW::dv(2 * lc, k.gm1);
+    double s_starr = rv[A] + PW::dv(2 * rc, k.gm1);
     double du = rv[A] - lv[A];
-    double du_crit = 2 * (lc + rc) / k.gm1;
+    double du_crit = PW::dv(2 * (lc + rc), k.gm1);
     if (du_crit < du) {  // generation of vacuum (:83-98)
       if (dxdt > s_starr) {
         if (dxdt > rv[A] + rc)
-          prim_to_cons<ND>(k, r_rho, rv, rp, u);
+          take(r_rho, rv, rp);
         else
-          w_kfan<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, u);
+          w_kfan_prim<ND, A, PW>(k, r_rho, rv, rp, rc, dxdt, true, srho, sv, sp);
       } else if (dxdt > s_starl) {
-        vacuum_state<ND>(k, u);
+        // vacuum
       } else {
         if (dxdt > lv[A] - lc)
-          w_kfan<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, u);
+          w_kfan_prim<ND, A, PW>(k, l_rho, lv, lp, lc, dxdt, false, srho, sv, sp);
         else
-          prim_to_cons<ND>(k, l_rho, lv, lp, u);
+          take(l_rho, lv, lp);
       }
     } else {
       // star state: rhyme_riemann_problem_star.f90:3-66
@@ -522,14 +590,12 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
       double kv[ND];
 #pragma unroll
       for (int d = 0; d < ND; ++d) kv[d] = right ? rv[d] : lv[d];
-      const double ps_pk = ps / kp;
-      double srho = kr, sp = kp, sv[ND];
-#pragma unroll
-      for (int d = 0; d < ND; ++d) sv[d] = kv[d];
+      const double ps_pk = PW::dv(ps, kp);
+      take(kr, kv, kp);
       if (ps > kp) {  // shock on side K
-        double speed = kv[A] + sc * sqrt(k.gp1_2g * ps_pk + k.gm1_2g);
+        double speed = kv[A] + sc * PW::sq(k.gp1_2g * ps_pk + k.gm1_2g);
         if (right ? dxdt < speed : dxdt > speed) {
-          srho = kr * (k.gm1_gp1 + ps_pk) / (k.gm1_gp1 * ps_pk + 1.0);
+          srho = PW::dv(kr * (k.gm1_gp1 + ps_pk), k.gm1_gp1 * ps_pk + 1.0);
           sv[A] = us;
           sp = ps;
         }
@@ -547,19 +613,33 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
           w_kfan_prim<ND, A, PW>(k, kr, kv, kp, kc, dxdt, right, srho, sv, sp);
         }
       }
-      prim_to_cons<ND>(k, srho, sv, sp, u);
     }
   }
 }
 
-// Riemann solve + calc_flux of the sampled state
-// (…_solve_memory_intensive.f90:110-116): the intercell flux.
+// The conserved state sampled on the interface (FP64 unit-level solve).
 template <int ND, int A, class PW = PowLib>
-__device__ __forceinline__ void riemann_flux(const Consts& k, const double* l, const double* r,
-                                             double* F) {
-  double u[ND + 2];
-  riemann_state<ND, A, PW>(k, l, r, u);
-  flux<ND, A>(k, u, F);
+__device__ __forceinline__ void riemann_state(const Consts& k, const double* l, const double* r,
+                                              double* u) {
+  double srho, sv[ND], sp;
+  riemann_sample<ND, A, PW, double>(k, l, r, srho, sv, sp);
+  prim_to_cons<ND, double>(k, srho, sv, sp, u);
+}
+
+// Riemann solve + calc_flux of the sampled state
+// (…_solve_memory_intensive.f90:110-116): the intercell flux.  R is the
+// type of the half states and of the flux (double: FP64 throughout; float:
+// the mixed form, whose conversions to and from primitive variables and
+// flux run in FP32 around the FP64 star-state solve and sampling).
+template <int ND, int A, class PW, typename R>
+__device__ __forceinline__ void riemann_flux(const Consts& k, const R* l, const R* r, R* F) {
+  double srho, sv[ND], sp;
+  riemann_sample<ND, A, PW, R>(k, l, r, srho, sv, sp);
+  R v[ND], u[ND + 2];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = (R)sv[d];
+  prim_to_cons<ND, R>(k, (R)srho, v, (R)sp, u);
+  flux<ND, A, R>(k, u, F);
 }
 
 }  // namespace rhy
