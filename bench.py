@@ -156,10 +156,10 @@ def serving_leg(njobs, rate, budget_gib, quantum, seed, precision=0):
     log-uniform in [32, 128]), 100 steps each, seeded two-state or blast
     state, limiter uniform over the four; admitted first come first served
     under an HBM budget, state resident, continuously batched in quanta.
-    Initial states are generated on host threads ahead of admission and
-    uploaded at admission (inside the measured latency); a job completes at
-    the end of the quantum in which it reaches its step count, when its slot
-    is freed (final states are not copied back here)."""
+    A job's initial state is painted on the device at admission (inside the
+    measured latency); a job completes at the end of the quantum in which it
+    reaches its step count, when its slot is freed (final states are not
+    copied back here)."""
     from rhyme_amd import serve
     jobs = serve.serving_jobs(njobs=njobs, rate=rate, seed=seed)
     srv = serve.HydroServer(int(budget_gib * (1 << 30)), precision=precision, quantum=quantum,
@@ -268,6 +268,11 @@ def main():
             line["serving"]["saturated"] = serving_leg(
                 a.serve_jobs, a.serve_saturated_rate, a.serve_budget_gib, a.serve_quantum,
                 a.seed)
+            line["serving"]["mixed_saturated"] = serving_leg(
+                a.serve_jobs, a.serve_saturated_rate, a.serve_budget_gib, a.serve_quantum,
+                a.seed, precision=1)
+            line["serving"]["mixed_saturated_vs_plain_fp64"] = (
+                line["serving"]["mixed_saturated"]["zone_updates_per_s"] / zu)
     if not a.skip_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         c = cpu_leg(a.cpu_n, a.cpu_steps, a.seed, threads)

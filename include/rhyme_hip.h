@@ -157,6 +157,36 @@ int rhy_box_set_profiling(rhy_box_t *box, int32_t on);
 int rhy_box_kernel_stats(rhy_box_t *box, int32_t kidx, int64_t *launches,
                          double *total_ms);
 
+/* ---- initial conditions painted on the device ----
+ * No counterpart in the reference, which paints on the host
+ * (rhyme_drawing_init, src/rhyme_drawing/src/rhyme_drawing_init.f90:19-78)
+ * and uploads nothing.  Two seeded states at rest, written into every
+ * component plane (ghosts and the non-hydro components zero, as after
+ * rhyme_samr_bc_init), conserved variables formed as conv_prim_vars_to_cons
+ * does (E = .5*rho*0 + p/(gamma-1)):
+ *   RHY_IC_TWO_STATE  rho_l, p_l for cells whose 0-based index along `axis`
+ *                     is below dims[axis]/2, rho_r, p_r above;
+ *   RHY_IC_BLAST      rho = 1 + jitter*(-1 + 2u), u the SplitMix64 draw
+ *                     (Steele, Lea, Flood 2014) number q of stream `seed`,
+ *                     q the 0-based C-order interior index ((k*ny + j)*nx + i),
+ *                     u = (draw >> 11) * 2^-53; p = p0, except in the cells
+ *                     whose centre offset from the box centre (q_d + 0.5 -
+ *                     n_d/2, summed squares in axis order) is <= radius^2,
+ *                     where p = (gamma-1)*energy / (N_in * dx^ndim),
+ *                     dx = 1/max(dims), N_in the number of such cells. */
+#define RHY_IC_TWO_STATE 1
+#define RHY_IC_BLAST 2
+typedef struct rhy_ic {
+    int32_t kind;
+    int32_t axis;
+    double gamma;
+    double rho_l, p_l, rho_r, p_r;
+    double radius, jitter, p0, energy;
+    uint64_t seed;
+} rhy_ic_t;
+/* paint the box's device state (the host mirror is not touched) */
+int rhy_box_paint(rhy_box_t *box, const rhy_ic_t *ic);
+
 /* ---- unit-level batches (host arrays, NVAR = ndim+2 values per record,
  * record-major) so the reference's known-answer tests can run on the GPU ---- */
 
@@ -221,6 +251,10 @@ int rhy_pool_box_bytes(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_
 int rhy_pool_add(rhy_batch_t *pool, const int32_t dims[3], const double dx[3],
                  const double *cells, int32_t limiter, const int32_t bc_types[6],
                  int32_t *slot);
+/* as rhy_pool_add, with the state painted on the device from `ic` */
+int rhy_pool_add_ic(rhy_batch_t *pool, const int32_t dims[3], const double dx[3],
+                    const rhy_ic_t *ic, int32_t limiter, const int32_t bc_types[6],
+                    int32_t *slot);
 int rhy_pool_remove(rhy_batch_t *pool, int32_t slot);
 /* total free bytes of the arena and the largest free block */
 int rhy_pool_free_bytes(rhy_batch_t *pool, int64_t *free_bytes, int64_t *largest_bytes);

@@ -28,12 +28,12 @@ EXPORTS = [
     "rhy_mh_solve", "rhy_run", "rhy_run_async", "rhy_box_last_dts",
     "rhy_box_set_profiling", "rhy_box_kernel_stats",
     "rhy_riemann_batch", "rhy_limiter_batch", "rhy_half_step_batch",
-    "rhy_flux_batch",
+    "rhy_flux_batch", "rhy_box_paint",
     "rhy_batch_create", "rhy_batch_destroy", "rhy_batch_bytes", "rhy_batch_upload",
     "rhy_batch_download", "rhy_batch_set_tiles", "rhy_batch_run_async", "rhy_batch_sync",
     "rhy_batch_last_dts", "rhy_batch_last_error",
-    "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_remove",
-    "rhy_pool_free_bytes",
+    "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_add_ic",
+    "rhy_pool_remove", "rhy_pool_free_bytes",
 ]
 
 
@@ -55,6 +55,35 @@ class RhyParams(C.Structure):
         ("reserved", C.c_int32),
         ("cons_inflows", (C.c_double * MAX_CMP) * 6),
     ]
+
+
+RHY_IC_TWO_STATE, RHY_IC_BLAST = 1, 2
+
+
+class RhyIC(C.Structure):
+    """rhy_ic_t: a seeded initial state painted on the device."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("axis", C.c_int32),
+        ("gamma", C.c_double),
+        ("rho_l", C.c_double),
+        ("p_l", C.c_double),
+        ("rho_r", C.c_double),
+        ("p_r", C.c_double),
+        ("radius", C.c_double),
+        ("jitter", C.c_double),
+        ("p0", C.c_double),
+        ("energy", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
+def make_ic(ic):
+    """RhyIC from a dict of its fields (missing fields 0)."""
+    out = RhyIC()
+    for k, v in ic.items():
+        setattr(out, k, v)
+    return out
 
 
 class RhyError(RuntimeError):
@@ -101,6 +130,7 @@ def load():
         "rhy_half_step_batch": (C.c_int, [pp, C.c_int64, dp, dp, ip, C.c_double,
                                           C.c_double, dp, dp]),
         "rhy_flux_batch": (C.c_int, [pp, C.c_int64, dp, ip, dp]),
+        "rhy_box_paint": (C.c_int, [vp, C.POINTER(RhyIC)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -35,6 +35,9 @@ _BATCH_SIGS = {
     "rhy_pool_add": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
                                C.POINTER(C.c_double), C.c_int32, C.POINTER(C.c_int32),
                                C.POINTER(C.c_int32)]),
+    "rhy_pool_add_ic": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_double),
+                                  C.POINTER(_lib.RhyIC), C.c_int32, C.POINTER(C.c_int32),
+                                  C.POINTER(C.c_int32)]),
     "rhy_pool_remove": (C.c_int, [C.c_void_p, C.c_int32]),
     "rhy_pool_free_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64),
                                       C.POINTER(C.c_int64)]),
@@ -200,6 +203,28 @@ class SamrPool:
         if rc == _lib.RHY_ERR_OOM:
             raise PoolFull(_L().rhy_batch_last_error().decode())
         _check(rc, "rhy_pool_add")
+        self.boxes[slot.value] = list(dims)[:self.ndim]
+        return slot.value
+
+    def add_ic(self, dims, dx, ic, limiter, bc_types=None):
+        """Place a box whose state is painted on the device from `ic` (a dict
+        of rhy_ic_t fields, see ics.ic_two_state / ics.ic_blast); returns its
+        slot, raises PoolFull when it does not fit."""
+        d3 = np.ones(3, dtype=np.int32)
+        x3 = np.ones(3, dtype=np.float64)
+        d3[:self.ndim] = list(dims)[:self.ndim]
+        x3[:self.ndim] = list(dx)[:self.ndim]
+        bc = None
+        if bc_types is not None:
+            bc = (C.c_int32 * 6)(*[int(t) for t in bc_types])
+        c_ic = _lib.make_ic(ic)
+        slot = C.c_int32(-1)
+        rc = _L().rhy_pool_add_ic(self._h, iptr(d3), dptr(x3), C.byref(c_ic), int(limiter),
+                                  C.cast(bc, C.POINTER(C.c_int32)) if bc is not None else None,
+                                  C.byref(slot))
+        if rc == _lib.RHY_ERR_OOM:
+            raise PoolFull(_L().rhy_batch_last_error().decode())
+        _check(rc, "rhy_pool_add_ic")
         self.boxes[slot.value] = list(dims)[:self.ndim]
         return slot.value
 

@@ -526,6 +526,38 @@ int rhy_bc_set_boundaries(rhy_box_t* b, const rhy_params_t* p) {
   return RHY_OK;
 }
 
+int rhy_box_paint(rhy_box_t* b, const rhy_ic_t* ic) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!ic || (ic->kind != RHY_IC_TWO_STATE && ic->kind != RHY_IC_BLAST))
+    return fail(RHY_ERR_ARG, "unknown initial condition kind");
+  if (ic->kind == RHY_IC_TWO_STATE && (ic->axis < 0 || ic->axis >= b->ndim))
+    return fail(RHY_ERR_ARG, "two-state axis out of range");
+  if (b->ncmp < b->ndim + 2) return fail(RHY_ERR_ARG, "box has too few components");
+  const PaintIC pc =
+      make_paint(ic->kind, ic->axis, ic->gamma, ic->rho_l, ic->p_l, ic->rho_r, ic->p_r,
+                 ic->radius, ic->jitter, ic->p0, ic->energy, ic->seed, b->ndim, b->dims);
+  const Geom& G = b->geom;
+  const dim3 grid(grid_for(G.plane));
+  const bool mixed = b->precision == RHY_MIXED;
+#define RHY_PAINT(ND)                                                                         \
+  do {                                                                                        \
+    if (mixed)                                                                                \
+      hipLaunchKernelGGL((paint_kernel<ND, float>), grid, dim3(kThreads), 0, b->stream, G, pc, \
+                         b->ncmp, static_cast<float*>(b->cells));                             \
+    else                                                                                      \
+      hipLaunchKernelGGL((paint_kernel<ND, double>), grid, dim3(kThreads), 0, b->stream, G,   \
+                         pc, b->ncmp, static_cast<double*>(b->cells));                        \
+  } while (0)
+  if (b->ndim == 1) RHY_PAINT(1);
+  else if (b->ndim == 2) RHY_PAINT(2);
+  else RHY_PAINT(3);
+#undef RHY_PAINT
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
 int rhy_bc_set_face(rhy_box_t* b, const rhy_params_t* p, int32_t face) {
   int rc;
   if ((rc = check_box(b))) return rc;

@@ -65,18 +65,52 @@ struct BTile {
   int32_t kind;
   int64_t base;
   int32_t count;
-  int32_t pad;
 };
+
+// A segment is the run of tiles that covers one phase of one box: tiles
+// [start, next segment's start) take blockDim.x consecutive threads each of
+// the `threads` threads of box `box`'s phase `kind`.  Segments are stored in
+// tile order (all flux phases, then the updates), a few per box, so the
+// table changes size with the boxes and not with their zones.
+struct BSeg {
+  int32_t start;
+  int32_t box;
+  int32_t kind;
+  int32_t pad;
+  int64_t threads;
+};
+
+// Tile ti by binary search over the segments (uniform in the workgroup).
+__device__ __forceinline__ BTile tile_of(const BSeg* __restrict__ segs, int32_t nseg,
+                                         int32_t ti) {
+  int lo = 0, hi = nseg;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (segs[mid].start <= ti)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const BSeg S = segs[lo];
+  BTile T;
+  T.box = S.box;
+  T.kind = S.kind;
+  T.base = (int64_t)(ti - S.start) * blockDim.x;
+  const int64_t left = S.threads - T.base;
+  T.count = (int32_t)(left < (int64_t)blockDim.x ? left : (int64_t)blockDim.x);
+  return T;
+}
 
 // LIM > 0: every box uses that limiter; LIM == 0: each box's own.
 template <int ND, int LIM, typename R, class PW>
 __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Consts k, const BBox* __restrict__ boxes,
-                                                         const BTile* __restrict__ tiles,
-                                                         int32_t ntiles, R* __restrict__ arena,
+                                                         const BSeg* __restrict__ segs,
+                                                         int32_t nseg, int32_t ntiles,
+                                                         R* __restrict__ arena,
                                                          const double* __restrict__ dt,
                                                          int32_t nf_x, int32_t nf_yz) {
   for (int32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-    const BTile T = tiles[ti];
+    const BTile T = tile_of(segs, nseg, ti);
     if ((int32_t)threadIdx.x >= T.count) continue;
     const BBox& B = boxes[T.box];
     Consts kb = k;
@@ -97,14 +131,15 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Cons
 // images and CFL partial of every box; tile ti writes partial[ti - first].
 template <int ND, int ORDER, typename R>
 __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox* __restrict__ boxes,
-                                                           const BTile* __restrict__ tiles,
-                                                           int32_t first, int32_t ntiles,
+                                                           const BSeg* __restrict__ segs,
+                                                           int32_t nseg, int32_t first,
+                                                           int32_t ntiles,
                                                            R* __restrict__ arena,
                                                            const double* __restrict__ dt,
                                                            double* __restrict__ partial) {
   constexpr int NV = ND + 2;
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
-    const BTile T = tiles[ti];
+    const BTile T = tile_of(segs, nseg, ti);
     const BBox& B = boxes[T.box];
     const Geom& G = B.G;
     double speed = 0.0;
@@ -156,12 +191,13 @@ __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox*
 // CFL partials of the current state (prologue), same tiling as the update.
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void batch_cfl_kernel(Consts k, const BBox* __restrict__ boxes,
-                                                        const BTile* __restrict__ tiles,
-                                                        int32_t first, int32_t ntiles,
+                                                        const BSeg* __restrict__ segs,
+                                                        int32_t nseg, int32_t first,
+                                                        int32_t ntiles,
                                                         const R* __restrict__ arena,
                                                         double* __restrict__ partial) {
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
-    const BTile T = tiles[ti];
+    const BTile T = tile_of(segs, nseg, ti);
     const BBox& B = boxes[T.box];
     const Geom& G = B.G;
     double v = 0.0;
@@ -245,7 +281,7 @@ struct rhy_batch {
   std::vector<int32_t> active;
   std::map<int64_t, int64_t> free_blocks;  // offset -> length, elements
   int64_t arena_elems = 0;
-  std::vector<BTile> htiles;
+  std::vector<BSeg> hsegs;
   int32_t nflux_tiles = 0, nupd_tiles = 0;
   int32_t block = 256, nf_x = 4, nf_yz = 8, grid_cap = 0;
   bool dirty = true;
@@ -255,8 +291,8 @@ struct rhy_batch {
   void* arena = nullptr;
   BBox* dboxes = nullptr;
   int32_t* dactive = nullptr;
-  BTile* dtiles = nullptr;
-  size_t dtiles_cap = 0;
+  BSeg* dsegs = nullptr;
+  size_t dsegs_cap = 0;
   double* partial = nullptr;
   size_t partial_cap = 0;
   double* dt = nullptr;
@@ -362,42 +398,42 @@ int build_tiles(rhy_batch* b) {
   // the tables below are read by work that may still be in flight
   BHIP_TRY(hipStreamSynchronize(b->stream));
   b->partials_valid = false;
-  b->htiles.clear();
+  b->hsegs.clear();
   b->active.clear();
   for (int i = 0; i < (int)b->slots.size(); ++i)
     if (b->slots[i].active) b->active.push_back(i);
   const int nd = b->ndim;
+  int32_t start = 0;
   for (int a = 0; a < nd; ++a)
     for (int i : b->active) {
       const Geom& G = b->slots[i].B.G;
       const int64_t n = flux_threads(G, a, a == 0 ? b->nf_x : b->nf_yz);
-      for (int64_t base = 0; base < n; base += b->block)
-        b->htiles.push_back({i, a, base, (int32_t)std::min<int64_t>(b->block, n - base), 0});
+      b->hsegs.push_back({start, i, a, 0, n});
+      start += (int32_t)cdiv(n, b->block);
     }
-  b->nflux_tiles = (int32_t)b->htiles.size();
-  int32_t part = 0;
+  b->nflux_tiles = start;
   for (int i : b->active) {
     BBox& B = b->slots[i].B;
     const int64_t n = (int64_t)B.G.n[0] * B.G.n[1] * B.G.n[2];
-    B.part_begin = part;
-    for (int64_t base = 0; base < n; base += b->block) {
-      b->htiles.push_back({i, 3, base, (int32_t)std::min<int64_t>(b->block, n - base), 0});
-      ++part;
-    }
-    B.part_end = part;
+    B.part_begin = start - b->nflux_tiles;
+    b->hsegs.push_back({start, i, 3, 0, n});
+    start += (int32_t)cdiv(n, b->block);
+    B.part_end = start - b->nflux_tiles;
   }
-  b->nupd_tiles = part;
+  b->nupd_tiles = start - b->nflux_tiles;
   int rc;
-  if ((rc = grow((void**)&b->dtiles, &b->dtiles_cap, b->htiles.size(), sizeof(BTile))) ||
-      (rc = grow((void**)&b->partial, &b->partial_cap, (size_t)std::max(1, part),
+  if ((rc = grow((void**)&b->dsegs, &b->dsegs_cap, b->hsegs.size(), sizeof(BSeg))) ||
+      (rc = grow((void**)&b->partial, &b->partial_cap, (size_t)std::max(1, b->nupd_tiles),
                  sizeof(double))))
     return rc;
-  if (!b->htiles.empty())
-    BHIP_TRY(hipMemcpy(b->dtiles, b->htiles.data(), sizeof(BTile) * b->htiles.size(),
+  if (!b->hsegs.empty())
+    BHIP_TRY(hipMemcpy(b->dsegs, b->hsegs.data(), sizeof(BSeg) * b->hsegs.size(),
                        hipMemcpyHostToDevice));
-  std::vector<BBox> hb(b->slots.size());
-  for (size_t i = 0; i < b->slots.size(); ++i) hb[i] = b->slots[i].B;
-  BHIP_TRY(hipMemcpy(b->dboxes, hb.data(), sizeof(BBox) * hb.size(), hipMemcpyHostToDevice));
+  // descriptors up to the highest active slot
+  const size_t nb = b->active.empty() ? 0 : (size_t)b->active.back() + 1;
+  std::vector<BBox> hb(nb);
+  for (size_t i = 0; i < nb; ++i) hb[i] = b->slots[i].B;
+  if (nb) BHIP_TRY(hipMemcpy(b->dboxes, hb.data(), sizeof(BBox) * nb, hipMemcpyHostToDevice));
   if (!b->active.empty())
     BHIP_TRY(hipMemcpy(b->dactive, b->active.data(), sizeof(int32_t) * b->active.size(),
                        hipMemcpyHostToDevice));
@@ -420,7 +456,8 @@ void launch_flux_phase(rhy_batch* b, const Consts& k, int lim) {
   R* a = static_cast<R*>(b->arena);
 #define RHY_BATCH_FLUX(L)                                                                   \
   hipLaunchKernelGGL((batch_flux_kernel<ND, L, R, PW>), dim3(g), dim3(b->block), 0, b->stream, \
-                     k, b->dboxes, b->dtiles, b->nflux_tiles, a, b->dt, b->nf_x, b->nf_yz)
+                     k, b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles, a, b->dt, \
+                     b->nf_x, b->nf_yz)
   switch (lim) {
     case 0: RHY_BATCH_FLUX(0); break;
     case 1: RHY_BATCH_FLUX(1); break;
@@ -437,10 +474,12 @@ void launch_update_phase(rhy_batch* b, const Consts& k, int order) {
   R* a = static_cast<R*>(b->arena);
   if (order == RHY_MEMORY_INTENSIVE)
     hipLaunchKernelGGL((batch_update_kernel<ND, 2, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, a, b->dt, b->partial);
+                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                       b->nupd_tiles, a, b->dt, b->partial);
   else
     hipLaunchKernelGGL((batch_update_kernel<ND, 1, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles, a, b->dt, b->partial);
+                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                       b->nupd_tiles, a, b->dt, b->partial);
 }
 
 template <int ND, typename R, class PW>
@@ -479,7 +518,8 @@ int prologue(rhy_batch* b, const Consts& k) {
   if (!b->pool || !b->partials_valid) {
     const unsigned g = persistent_grid(b, b->nupd_tiles);
     hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dtiles, b->nflux_tiles, b->nupd_tiles,
+                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                     b->nupd_tiles,
                        static_cast<const R*>(b->arena), b->partial);
   }
   b->partials_valid = true;  // from here on every update rewrites them
@@ -610,8 +650,20 @@ rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t aren
   return b;
 }
 
+template <int ND>
+void launch_paint(rhy_batch* b, const Geom& G, const PaintIC& ic, int64_t off) {
+  const unsigned grid = cdiv(G.plane, 256);
+  if (b->precision == RHY_MIXED)
+    hipLaunchKernelGGL((paint_kernel<ND, float>), dim3(grid), dim3(256), 0, b->stream, G, ic,
+                       b->ncmp, static_cast<float*>(b->arena) + off);
+  else
+    hipLaunchKernelGGL((paint_kernel<ND, double>), dim3(grid), dim3(256), 0, b->stream, G, ic,
+                       b->ncmp, static_cast<double*>(b->arena) + off);
+}
+
+// Place a box: its state from host `cells`, painted from `ic`, or zero.
 int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* cells,
-            int32_t limiter, const int32_t* bc_types, int32_t* slot_out) {
+            const PaintIC* ic, int32_t limiter, const int32_t* bc_types, int32_t* slot_out) {
   Geom G{};
   int rc;
   if ((rc = box_geom(b->ndim, dims, G))) return rc;
@@ -652,6 +704,11 @@ int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* c
                               hipMemcpyHostToDevice, b->stream));
       BHIP_TRY(hipStreamSynchronize(b->stream));
     }
+  } else if (ic) {
+    if (b->ndim == 1) launch_paint<1>(b, G, *ic, off);
+    else if (b->ndim == 2) launch_paint<2>(b, G, *ic, off);
+    else launch_paint<3>(b, G, *ic, off);
+    BHIP_TRY(hipGetLastError());
   } else {
     BHIP_TRY(hipMemsetAsync(static_cast<char*>(b->arena) + off * b->esz, 0,
                             b->esz * (size_t)G.plane * b->ncmp, b->stream));
@@ -682,7 +739,7 @@ int rhy_batch_create(int32_t ndim, int32_t nboxes, const int32_t* dims, const do
   if (!b) return rc;
   for (int i = 0; i < nboxes; ++i) {
     int32_t slot;
-    if ((rc = add_box(b, dims + 3 * i, dx + 3 * i, nullptr, 2, nullptr, &slot))) {
+    if ((rc = add_box(b, dims + 3 * i, dx + 3 * i, nullptr, nullptr, 2, nullptr, &slot))) {
       rhy_batch_destroy(b);
       return rc;
     }
@@ -725,7 +782,23 @@ int rhy_pool_add(rhy_batch_t* b, const int32_t dims[3], const double dx[3], cons
   if (!b || !dims || !dx || !cells || !slot) return bfail(RHY_ERR_ARG, "NULL argument");
   if (!b->pool) return bfail(RHY_ERR_ARG, "not a pool");
   if (limiter < 1 || limiter > 4) return bfail(RHY_ERR_ARG, "unknown slope limiter");
-  return add_box(b, dims, dx, cells, limiter, bc_types, slot);
+  return add_box(b, dims, dx, cells, nullptr, limiter, bc_types, slot);
+}
+
+int rhy_pool_add_ic(rhy_batch_t* b, const int32_t dims[3], const double dx[3], const rhy_ic_t* ic,
+                    int32_t limiter, const int32_t bc_types[6], int32_t* slot) {
+  if (!b || !dims || !dx || !ic || !slot) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (!b->pool) return bfail(RHY_ERR_ARG, "not a pool");
+  if (limiter < 1 || limiter > 4) return bfail(RHY_ERR_ARG, "unknown slope limiter");
+  if (ic->kind != RHY_IC_TWO_STATE && ic->kind != RHY_IC_BLAST)
+    return bfail(RHY_ERR_ARG, "unknown initial condition kind");
+  if (ic->kind == RHY_IC_TWO_STATE && (ic->axis < 0 || ic->axis >= b->ndim))
+    return bfail(RHY_ERR_ARG, "two-state axis out of range");
+  if (b->ncmp < b->ndim + 2) return bfail(RHY_ERR_ARG, "too few components");
+  const PaintIC pc =
+      make_paint(ic->kind, ic->axis, ic->gamma, ic->rho_l, ic->p_l, ic->rho_r, ic->p_r,
+                 ic->radius, ic->jitter, ic->p0, ic->energy, ic->seed, b->ndim, dims);
+  return add_box(b, dims, dx, nullptr, &pc, limiter, bc_types, slot);
 }
 
 int rhy_pool_remove(rhy_batch_t* b, int32_t slot) {
@@ -754,7 +827,7 @@ int rhy_batch_destroy(rhy_batch_t* b) {
   if (!b) return RHY_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->graph) hipGraphExecDestroy(b->graph);
-  for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dtiles,
+  for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow})
     if (p) hipFree(p);

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "rhy_physics.hpp"
 
 // Minimum waves per SIMD requested for the flux kernels (caps their VGPRs:
@@ -420,6 +422,138 @@ static __global__ __launch_bounds__(1024) void dt_kernel(const double* __restric
 
 static __global__ void set_scalar_kernel(double* dst, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// ------------------------------------------------- initial conditions ----
+
+// SplitMix64 (G. Steele, D. Lea, C. Flood, "Fast splittable pseudorandom
+// number generators", OOPSLA 2014, with the output constants of S. Vigna's
+// splitmix64.c): draw number `counter` (0-based) of the stream `seed`.
+__host__ __device__ inline uint64_t splitmix64(uint64_t seed, uint64_t counter) {
+  uint64_t z = seed + (counter + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// A painted initial state (kind 1: two states split along `axis`; kind 2:
+// blast over a jittered medium), with the host-side derived values.
+struct PaintIC {
+  int32_t kind, axis;
+  double gm1;  // gamma - 1
+  double rho_l, p_l, rho_r, p_r;
+  double r2, jitter, p0, p_in;
+  uint64_t seed;
+};
+
+// Offset of the centre of 0-based cell q from the centre of an n-cell axis.
+__host__ __device__ inline double centre_offset(int q, int n) {
+  return ((double)q + 0.5) - (double)n / 2.0;
+}
+
+// Host: the derived blast pressure, (gamma-1) * energy / (N_in * dx^ndim)
+// with dx = 1/max(dims) and N_in the cells within `radius` of the centre
+// (counted over the window that can hold them).
+inline PaintIC make_paint(int32_t kind, int32_t axis, double gamma, double rho_l, double p_l,
+                          double rho_r, double p_r, double radius, double jitter, double p0,
+                          double energy, uint64_t seed, int ndim, const int32_t* dims) {
+  PaintIC ic{};
+  ic.kind = kind;
+  ic.axis = axis;
+  ic.gm1 = gamma - 1.0;
+  ic.rho_l = rho_l;
+  ic.p_l = p_l;
+  ic.rho_r = rho_r;
+  ic.p_r = p_r;
+  ic.r2 = radius * radius;
+  ic.jitter = jitter;
+  ic.p0 = p0;
+  ic.seed = seed;
+  if (kind == 2) {
+    int lo[3] = {0, 0, 0}, hi[3] = {1, 1, 1};
+    for (int d = 0; d < ndim; ++d) {
+      const double c = dims[d] / 2.0 - 0.5;
+      lo[d] = (int)std::floor(c - radius) - 1;
+      hi[d] = (int)std::ceil(c + radius) + 2;
+      lo[d] = lo[d] < 0 ? 0 : lo[d];
+      hi[d] = hi[d] > dims[d] ? dims[d] : hi[d];
+    }
+    int64_t n_in = 0;
+    for (int k = lo[2]; k < hi[2]; ++k)
+      for (int j = lo[1]; j < hi[1]; ++j)
+        for (int i = lo[0]; i < hi[0]; ++i) {
+          const double x = centre_offset(i, dims[0]);
+          double rr = x * x;
+          if (ndim > 1) {
+            const double y = centre_offset(j, dims[1]);
+            rr = rr + y * y;
+          }
+          if (ndim > 2) {
+            const double z = centre_offset(k, dims[2]);
+            rr = rr + z * z;
+          }
+          n_in += rr <= ic.r2;
+        }
+    int m = dims[0];
+    for (int d = 1; d < ndim; ++d) m = dims[d] > m ? dims[d] : m;
+    const double dx = 1.0 / m;
+    ic.p_in = (gamma - 1.0) * energy / ((double)n_in * std::pow(dx, ndim));
+  }
+  return ic;
+}
+
+// Every cell of every component plane: interior cells get the painted
+// conserved state (zero velocity; non-hydro components 0), ghosts 0.
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void paint_kernel(Geom G, PaintIC ic, int32_t ncmp,
+                                                    R* __restrict__ cells) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  int q[3];
+  bool inside = true;
+  int64_t rem = t;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int ext = G.n[d] + 2 * G.g[d];
+    q[d] = (int)(rem % ext) - G.g[d];
+    rem /= ext;
+    inside = inside && q[d] >= 0 && q[d] < G.n[d];
+  }
+  double rho = 0.0, p = 0.0;
+  if (inside) {
+    if (ic.kind == 1) {
+      const bool left = q[ic.axis] < G.n[ic.axis] / 2;
+      rho = left ? ic.rho_l : ic.rho_r;
+      p = left ? ic.p_l : ic.p_r;
+    } else {
+      const uint64_t lin = ((uint64_t)q[2] * G.n[1] + q[1]) * G.n[0] + q[0];
+      const double u = (double)(splitmix64(ic.seed, lin) >> 11) * 0x1p-53;
+      rho = 1.0 + ic.jitter * (-1.0 + 2.0 * u);
+      const double x = centre_offset(q[0], G.n[0]);
+      double rr = x * x;
+      if (ND > 1) {
+        const double y = centre_offset(q[1], G.n[1]);
+        rr = rr + y * y;
+      }
+      if (ND > 2) {
+        const double z = centre_offset(q[2], G.n[2]);
+        rr = rr + z * z;
+      }
+      p = rr <= ic.r2 ? ic.p_in : ic.p0;
+    }
+  }
+  for (int c = 0; c < ncmp; ++c) {
+    double v = 0.0;
+    if (inside) {
+      if (c == 0)
+        v = rho;
+      else if (c <= ND)
+        v = rho * 0.0;
+      else if (c == ND + 1)
+        v = .5 * rho * 0.0 + p / ic.gm1;
+    }
+    cells[t + c * G.plane] = (R)v;
+  }
 }
 
 // ------------------------------------------------------ unit batches ----

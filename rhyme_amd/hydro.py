@@ -203,6 +203,12 @@ class SamrBox:
     def sync(self):
         check(_lib.load().rhy_box_sync(self._h), "rhy_box_sync")
 
+    def paint(self, ic):
+        """Paint the device state from `ic` (a dict of rhy_ic_t fields); the
+        host mirror is not touched (download() to see it)."""
+        c_ic = _lib.make_ic(ic)
+        check(_lib.load().rhy_box_paint(self._h, C.byref(c_ic)), "rhy_box_paint")
+
     # -- timing helpers used by bench.py
     def set_profiling(self, on):
         check(_lib.load().rhy_box_set_profiling(self._h, 1 if on else 0),

@@ -184,3 +184,81 @@ def amr_boxes(seed=0, nboxes=200, base=128, lo=8, hi=64, gamma=5.0 / 3.0, ndim=3
             cells = random_two_state(ndim, dims, seed=s, gamma=gamma)
         out.append((dims, [dx0 / 2.0] * ndim, cells))
     return out
+
+
+# --- initial conditions painted on the device (rhy_ic_t) --------------------
+# Descriptors of the seeded states that rhy_box_paint / rhy_pool_add_ic
+# write on the device, and `paint`, the same arithmetic on the host, so that
+# a painted state can be checked value for value and a job can be run from
+# either.
+
+IC_TWO_STATE, IC_BLAST = 1, 2
+_MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed, counter):
+    """SplitMix64 (Steele, Lea, Flood 2014; constants of Vigna's
+    splitmix64.c): draw number `counter` (0-based, uint64 array) of stream
+    `seed`, in numpy uint64 arithmetic (wrapping mod 2^64)."""
+    c = np.asarray(counter, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed & _MASK64) + (c + np.uint64(1)) * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def ic_two_state(ndim, seed, gamma=5.0 / 3.0):
+    """The two-state descriptor of random_two_state (same draws: axis, then
+    rho_l, p_l, rho_r, p_r ~ U[0.1, 1]); painting it gives random_two_state's
+    array."""
+    g = rng(seed)
+    axis = int(g.integers(0, ndim))
+    rl, pl, rr, pr = (float(x) for x in g.uniform(0.1, 1.0, size=4))
+    return dict(kind=IC_TWO_STATE, axis=axis, gamma=gamma, rho_l=rl, p_l=pl, rho_r=rr, p_r=pr)
+
+
+def ic_blast(seed, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, energy=1.0):
+    """A blast over a jittered medium: rho = 1 + jitter*(-1 + 2u) with u the
+    SplitMix64 draw of the cell's C-order index, p = p0, and `energy`
+    deposited as pressure in the cells within `radius` cells of the centre."""
+    return dict(kind=IC_BLAST, axis=0, gamma=gamma, radius=radius, jitter=jitter, p0=p0,
+                energy=energy, seed=int(seed) & _MASK64)
+
+
+def _centre_offsets(n):
+    return (np.arange(n) + 0.5) - n / 2.0
+
+
+def paint(ic, ndim, dims, ncmp=None):
+    """Host array of the state rhy_box_paint writes for descriptor `ic`."""
+    ncmp = ncmp or ncmp_for(ndim)
+    dims = list(dims)[:ndim]
+    shape = dims[::-1]  # C order (z, y, x)
+    gamma = ic["gamma"]
+    if ic["kind"] == IC_TWO_STATE:
+        axis = ic["axis"]
+        idx = np.indices(shape)[::-1][axis]
+        left = idx < dims[axis] // 2
+        rho = np.where(left, ic["rho_l"], ic["rho_r"]).astype(np.float64)
+        p = np.where(left, ic["p_l"], ic["p_r"]).astype(np.float64)
+    elif ic["kind"] == IC_BLAST:
+        n = int(np.prod(dims))
+        u = (splitmix64(ic["seed"], np.arange(n, dtype=np.uint64)) >> np.uint64(11)).astype(
+            np.float64) * 2.0 ** -53
+        rho = (1.0 + ic["jitter"] * (-1.0 + 2.0 * u)).reshape(shape)
+        offs = [_centre_offsets(d) for d in dims]
+        grids = np.meshgrid(*offs[::-1], indexing="ij")[::-1]  # x, y, z arrays
+        rr = grids[0] * grids[0]
+        for d in range(1, ndim):
+            rr = rr + grids[d] * grids[d]
+        r2 = ic["radius"] * ic["radius"]
+        inside = rr <= r2
+        n_in = int(inside.sum())
+        dx = 1.0 / max(dims)
+        p = np.full(shape, ic["p0"])
+        p[inside] = (gamma - 1.0) * ic["energy"] / (n_in * dx ** ndim)
+    else:
+        raise ValueError("unknown initial condition kind %r" % ic["kind"])
+    v = np.zeros([ndim] + shape)
+    return _fill(ndim, dims, ncmp, gamma, rho, v, p)

@@ -22,8 +22,6 @@ to the whole budget, so the budget is the only limit that binds except for
 fragmentation.
 """
 import collections
-import concurrent.futures
-import itertools
 import time
 from dataclasses import dataclass, field
 
@@ -47,7 +45,8 @@ class Job:
     arrival: float = 0.0        # seconds after the start of serve()
     kind: str = ""
     seed: int = 0
-    cells: np.ndarray = None    # initial state, hydro.box_shape layout
+    ic: dict = None             # painted initial state (rhy_ic_t fields), or
+    cells: np.ndarray = None    # an explicit one, hydro.box_shape layout
     # filled in by the server
     admitted: float = None
     finished: float = None
@@ -59,19 +58,18 @@ class Job:
     def zones(self):
         return int(np.prod(self.dims))
 
-    def make_cells(self, gamma=5.0 / 3.0):
-        """Initial state of a generated job (`kind` two_state or blast)."""
-        if self.kind == "blast":
-            return ics.blast(self.ndim, self.dims, seed=self.seed, gamma=gamma)
-        return ics.random_two_state(self.ndim, self.dims, seed=self.seed, gamma=gamma)
+    def make_cells(self, ncmp=None):
+        """Host array of the job's initial state (its painted `ic`)."""
+        return ics.paint(self.ic, self.ndim, self.dims, ncmp)
 
 
 def serving_jobs(njobs=1000, rate=40.0, seed=0, steps=100, n2d=(64, 512), n3d=(32, 128)):
     """The serving workload: Poisson arrivals at `rate` jobs/s, half 2D boxes
     n x n with n log-uniform in n2d, half 3D cubes n^3 with n log-uniform in
     n3d, dx = 1/n; seeded two-state (outflow faces) or blast (reflective
-    faces) initial state; limiter drawn uniformly from the four.  Cells are
-    not generated here (Job.make_cells)."""
+    faces) initial state, described by its painting parameters (ics.ic_*):
+    the device paints it at admission; limiter drawn uniformly from the
+    four."""
     g = ics.rng(seed)
     gaps = g.exponential(1.0 / rate, size=njobs)
     arrivals = np.cumsum(gaps) - gaps[0]
@@ -84,9 +82,11 @@ def serving_jobs(njobs=1000, rate=40.0, seed=0, steps=100, n2d=(64, 512), n3d=(3
         n = int(round(lo * (hi / lo) ** g.uniform()))
         kind = "blast" if g.uniform() < 0.5 else "two_state"
         bc = hydro.REFLECTIVE if kind == "blast" else hydro.OUTFLOW
+        seed_i = int(g.integers(1 << 30))
+        ic = ics.ic_blast(seed_i) if kind == "blast" else ics.ic_two_state(nd, seed_i)
         jobs.append(Job(jid=i, ndim=nd, dims=[n] * nd, dx=[1.0 / n] * nd,
                         limiter=int(g.integers(1, 5)), bc_types=[bc] * 6, steps=steps,
-                        arrival=float(arrivals[i]), kind=kind, seed=int(g.integers(1 << 30))))
+                        arrival=float(arrivals[i]), kind=kind, seed=seed_i, ic=ic))
     return jobs
 
 
@@ -108,9 +108,13 @@ def run_alone(job, params, precision=FP64):
     p.limiter = job.limiter
     for f in range(6):
         p.bc_types[f] = int(job.bc_types[f])
-    box = hydro.SamrBox(job.ndim, job.dims, precision=precision,
-                        cells=np.array(job.cells, dtype=np.float64))
-    box.upload()
+    if job.cells is not None:
+        box = hydro.SamrBox(job.ndim, job.dims, precision=precision,
+                            cells=np.array(job.cells, dtype=np.float64))
+        box.upload()
+    else:
+        box = hydro.SamrBox(job.ndim, job.dims, precision=precision)
+        box.paint(job.ic)
     dts = hydro.run(p, box, job.dx, job.steps)
     out = box.download().copy()
     box.close()
@@ -150,23 +154,20 @@ class HydroServer:
     def job_bytes(self, job):
         return box_bytes(job.ndim, job.dims, self.params[job.ndim].ncmp, self.precision)
 
-    def _try_admit(self, job, now, pending):
+    def _try_admit(self, job, now):
         nb = self.job_bytes(job)
         if nb > self.budget:
             raise ValueError("job %d needs %d bytes, more than the whole budget" % (job.jid, nb))
         if self.resident_bytes + nb > self.budget:
             return False
-        if job.cells is not None:
-            cells = job.cells
-        elif job.jid in pending:
-            cells = pending[job.jid].result()
-        else:
-            cells = job.make_cells(self.params[job.ndim].gamma)
+        pool = self.pools[job.ndim]
         try:
-            slot = self.pools[job.ndim].add(job.dims, job.dx, cells, job.limiter, job.bc_types)
+            if job.cells is not None:
+                slot = pool.add(job.dims, job.dx, job.cells, job.limiter, job.bc_types)
+            else:
+                slot = pool.add_ic(job.dims, job.dx, job.ic, job.limiter, job.bc_types)
         except PoolFull:
             return False
-        pending.pop(job.jid, None)
         job.slot = slot
         job.admitted = now
         self.resident[job.ndim][slot] = job
@@ -184,41 +185,20 @@ class HydroServer:
         job.finished = now
         job.slot = None
 
-    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep, workers=8, lookahead=32):
+    def serve(self, jobs, clock=time.perf_counter, sleep=time.sleep):
         """Serve `jobs` to completion in arrival order; returns them with
-        admitted/finished times (seconds from the start) filled in.  Jobs
-        without `cells` get their initial state generated on `workers` host
-        threads, up to `lookahead` jobs ahead of admission, while the device
-        runs (the generated arrays are dropped once uploaded)."""
+        admitted/finished times (seconds from the start) filled in.  A job's
+        state comes from its `cells` (uploaded at admission) or is painted on
+        the device from its `ic`."""
         queue = collections.deque(sorted(jobs, key=lambda j: j.arrival))
-        pending = {}
-        ex = None
-        if any(j.cells is None for j in jobs):
-            ex = concurrent.futures.ThreadPoolExecutor(max_workers=workers)
-
-        def prefetch():
-            if ex is None:
-                return
-            for j in itertools.islice(queue, lookahead):
-                if j.cells is None and j.jid not in pending:
-                    pending[j.jid] = ex.submit(j.make_cells, self.params[j.ndim].gamma)
-
-        try:
-            return self._serve(queue, pending, prefetch, clock, sleep, jobs)
-        finally:
-            if ex is not None:
-                ex.shutdown(wait=True, cancel_futures=True)
-
-    def _serve(self, queue, pending, prefetch, clock, sleep, jobs):
         t0 = clock()
         left = len(queue)
         while left:
-            prefetch()
             ta = clock()
             now = ta - t0
             # admission, first come first served
             while queue and queue[0].arrival <= now:
-                if not self._try_admit(queue[0], now, pending):
+                if not self._try_admit(queue[0], now):
                     self.deferrals += 1
                     break
                 queue.popleft()

@@ -6,7 +6,7 @@ the same byte accounting (rhy_pool_box_bytes, a host-only call)."""
 import numpy as np
 import pytest
 
-from rhyme_amd import batch, hydro, serve
+from rhyme_amd import batch, hydro, ics, serve
 
 
 class FakePool:
@@ -26,6 +26,9 @@ class FakePool:
         self.boxes[slot] = (list(dims), nb)
         self.used += nb
         return slot
+
+    def add_ic(self, dims, dx, ic, limiter, bc_types=None):
+        return self.add(dims, dx, None, limiter, bc_types)
 
     def remove(self, slot):
         self.used -= self.boxes.pop(slot)[1]
@@ -87,8 +90,6 @@ def test_make_cells_shapes():
 
 def test_fifo_admission_under_budget(fake_pools):
     jobs = serve.serving_jobs(njobs=30, rate=1e9, seed=2, steps=7, n2d=(8, 32), n3d=(6, 16))
-    for j in jobs:
-        j.cells = np.zeros(1)
     sizes = [batch.box_bytes(j.ndim, j.dims) for j in jobs]
     budget = max(sizes) * 2
     srv = serve.HydroServer(budget, quantum=3, keep_results=False)
@@ -106,10 +107,19 @@ def test_fifo_admission_under_budget(fake_pools):
             assert 1 <= q <= 3
 
 
+def test_quanta_bounded_when_nothing_waits(fake_pools):
+    jobs = serve.serving_jobs(njobs=6, rate=1e9, seed=4, steps=10, n2d=(8, 16), n3d=(6, 8))
+    srv = serve.HydroServer(1 << 30, quantum=3, keep_results=False)
+    srv.serve(jobs)
+    assert srv.deferrals == 0
+    for pool in FakePool.instances:
+        for q, _ in pool.runs:
+            assert 1 <= q <= 3
+    assert all(j.done == 10 for j in jobs)
+
+
 def test_job_larger_than_budget_is_an_error(fake_pools):
     jobs = serve.serving_jobs(njobs=2, rate=1e9, seed=2, steps=3, n2d=(64, 64), n3d=(32, 32))
-    for j in jobs:
-        j.cells = np.zeros(1)
     srv = serve.HydroServer(1 << 16, quantum=3, keep_results=False)
     with pytest.raises(ValueError):
         srv.serve(jobs)
@@ -126,3 +136,48 @@ def test_latency_report_fields():
     assert rep["zone_updates"] == sum(2 * 8 ** j.ndim for j in jobs)
     span = max(j.finished for j in jobs) - jobs[0].arrival
     assert rep["zone_updates_per_s"] == pytest.approx(rep["zone_updates"] / span)
+
+
+def _splitmix_reference(seed, counter):
+    """SplitMix64 in Python integers, independent of numpy's uint64 wrap."""
+    m = (1 << 64) - 1
+    z = (seed + (counter + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    return z ^ (z >> 31)
+
+
+def test_splitmix64_mirror():
+    for seed in (0, 1, 12345, (1 << 64) - 1, 0x9E3779B97F4A7C15):
+        got = ics.splitmix64(seed, np.arange(50, dtype=np.uint64))
+        assert [int(v) for v in got] == [_splitmix_reference(seed, c) for c in range(50)]
+    # Vigna's splitmix64.c from state 0 gives 0xE220A8397B1DCDAF first
+    assert int(ics.splitmix64(0, np.array([0], dtype=np.uint64))[0]) == 0xE220A8397B1DCDAF
+
+
+@pytest.mark.parametrize("ndim,dims", [(1, [31]), (2, [12, 17]), (3, [9, 8, 10])])
+def test_paint_two_state_equals_seeded_two_state(ndim, dims):
+    for seed in range(4):
+        np.testing.assert_array_equal(ics.paint(ics.ic_two_state(ndim, seed), ndim, dims),
+                                      ics.random_two_state(ndim, dims, seed=seed))
+
+
+def test_paint_blast_properties():
+    dims = [20, 18, 16]
+    ic = ics.ic_blast(99, radius=3.0, jitter=0.01, p0=1e-5, energy=1.0)
+    c = ics.paint(ic, 3, dims)
+    inner = (slice(2, -2),) * 3
+    rho = c[0][inner]
+    assert np.all(rho >= 0.99) and np.all(rho < 1.01) and abs(rho.mean() - 1.0) < 1e-3
+    assert np.all(c[1][inner] == 0) and np.all(c[3][inner] == 0)
+    p = c[4][inner] * (2.0 / 3.0)        # E = p / (gamma - 1) at rest
+    hot = p > 1e-3
+    # cells within 3 cells of the centre (10, 9, 8) -> offsets q + .5 - n/2
+    k, j, i = np.indices(rho.shape)
+    rr = (i + .5 - 10) ** 2 + (j + .5 - 9) ** 2 + (k + .5 - 8) ** 2
+    assert np.array_equal(hot, rr <= 9.0)
+    # unit energy deposited: sum(p_in * dx^3) / (gamma-1) == 1 with dx = 1/20
+    assert abs((p[hot] * (1 / 20.0) ** 3).sum() / (2.0 / 3.0) - 1.0) < 1e-12
+    ghosts = c.copy()
+    ghosts[(slice(None),) + inner] = 0
+    assert not ghosts.any()

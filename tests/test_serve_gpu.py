@@ -1,13 +1,15 @@
-"""The pool (rhy_pool_*) and the serving loop on the device: first-fit arena
-accounting, refusal when a box does not fit, and every served job's result
-and dt sequence bit-identical to the same job run alone through the drop-in
-path (hydro.run), while other jobs come and go around it; one small job per
-dimensionality also checked against the CPU oracle."""
+"""The pool (rhy_pool_*), device painting (rhy_box_paint, rhy_pool_add_ic)
+and the serving loop on the device: first-fit arena accounting, refusal
+when a box does not fit, painted states equal to their host mirror
+(ics.paint), and every served job's result and dt sequence bit-identical to
+the same job run alone through the drop-in path (hydro.run), while other
+jobs come and go around it; small jobs also checked against the CPU
+oracle."""
 import numpy as np
 import pytest
 
 import oracle
-from rhyme_amd import hydro, serve
+from rhyme_amd import hydro, ics, serve
 from rhyme_amd.batch import PoolFull, SamrPool, box_bytes
 
 pytestmark = pytest.mark.gpu
@@ -17,12 +19,39 @@ def say(*a):
     print(*a, flush=True)
 
 
-def small_jobs(n=14, seed=5, steps=11):
+def small_jobs(n=14, seed=5, steps=11, host_cells=True):
     jobs = serve.serving_jobs(njobs=n, rate=1e6, seed=seed, steps=steps, n2d=(12, 40),
                               n3d=(8, 18))
-    for j in jobs:
-        j.cells = j.make_cells()
+    if host_cells:
+        for j in jobs:
+            j.cells = j.make_cells()
     return jobs
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_box_paint_matches_host_mirror(ndim, precision):
+    dims = {1: [37], 2: [21, 16], 3: [11, 9, 14]}[ndim]
+    for ic in (ics.ic_two_state(ndim, 3 + ndim), ics.ic_blast(11 + ndim, radius=2.5)):
+        say("paint", ndim, precision, ic["kind"])
+        box = hydro.SamrBox(ndim, dims, precision=precision)
+        box.paint(ic)
+        got = box.download()
+        want = ics.paint(ic, ndim, dims)
+        if precision == hydro.MIXED:
+            want = want.astype(np.float32).astype(np.float64)
+        np.testing.assert_array_equal(got, want)
+        box.close()
+
+
+def test_pool_add_ic_matches_host_mirror():
+    pool = SamrPool(3, 1 << 26, max_boxes=4)
+    jobs = [(ics.ic_blast(5), [12, 10, 9]), (ics.ic_two_state(3, 8), [7, 13, 6])]
+    slots = [pool.add_ic(d, [0.1] * 3, ic, hydro.VAN_LEER) for ic, d in jobs]
+    for s_, (ic, d) in zip(slots, jobs):
+        say("pool paint", ic["kind"], d)
+        np.testing.assert_array_equal(pool.download(s_), ics.paint(ic, 3, d))
+    pool.close()
 
 
 def test_pool_arena_first_fit_and_refusal():
@@ -81,6 +110,23 @@ def test_served_jobs_identical_to_alone(precision):
     srv.close()
 
 
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_painted_served_jobs_identical_to_alone(precision):
+    jobs = small_jobs(n=10, seed=21, steps=9, host_cells=False)
+    sizes = sorted(box_bytes(j.ndim, j.dims, precision=precision) for j in jobs)
+    budget = sum(sizes[-4:])
+    say("painted serve identity precision", precision, "budget", budget)
+    srv = serve.HydroServer(budget, precision=precision, quantum=4, record_dts=True)
+    srv.serve(jobs)
+    assert srv.deferrals > 0
+    for j in jobs:
+        ref, rdts = serve.run_alone(j, srv.params[j.ndim], precision=precision)
+        say(" job", j.jid, j.ndim, j.dims, "lim", j.limiter, j.kind)
+        np.testing.assert_array_equal(np.array(j.dts), rdts)
+        np.testing.assert_array_equal(j.result, ref)
+    srv.close()
+
+
 def test_served_jobs_match_oracle():
     jobs = [j for j in small_jobs(n=6, seed=9, steps=6)]
     srv = serve.HydroServer(1 << 30, quantum=4)
@@ -103,8 +149,6 @@ def test_served_jobs_match_oracle():
 def test_serving_report_small_stream():
     jobs = serve.serving_jobs(njobs=40, rate=400.0, seed=3, steps=10, n2d=(32, 96),
                               n3d=(16, 40))
-    for j in jobs:
-        j.cells = j.make_cells()
     srv = serve.HydroServer(1 << 30, quantum=5, keep_results=False)
     srv.serve(jobs)
     rep = serve.latency_report(jobs)
