@@ -222,8 +222,9 @@ int rhy_batch_destroy(rhy_batch_t *batch);
 int rhy_batch_bytes(rhy_batch_t *batch, int64_t *bytes);
 int rhy_batch_upload(rhy_batch_t *batch, int32_t box, const double *cells);
 int rhy_batch_download(rhy_batch_t *batch, int32_t box, double *cells);
-/* tile shape: threads per workgroup (64..256, multiple of 64), faces walked
- * per thread along x and along y/z, cap on persistent workgroups (0: auto) */
+/* tile shape: threads per workgroup (64..256, multiple of 64), steps of 64
+ * row cells per wave along x, faces walked per thread along y/z, cap on
+ * persistent workgroups (0: auto) */
 int rhy_batch_set_tiles(rhy_batch_t *batch, int32_t block, int32_t nf_x, int32_t nf_yz,
                         int32_t grid_cap);
 int rhy_batch_run_async(rhy_batch_t *batch, const rhy_params_t *p, int32_t nsteps,

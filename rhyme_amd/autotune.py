@@ -1,7 +1,7 @@
 """Tile-shape autotuner for the batched form.
 
-A tile shape is (threads per workgroup, faces walked per thread along x,
-faces per thread along y/z, cap on persistent workgroups).  `tune()` times a
+A tile shape is (threads per workgroup, steps of 64 x-row cells per wave
+along x, faces walked per thread along y/z, cap on persistent workgroups).  `tune()` times a
 few steps of the real box set under each candidate (the state is restored
 from the initial arrays afterwards), keeps the fastest, and caches it per
 box-set signature (dimensionality, precision, limiter, and the histogram of
@@ -15,8 +15,8 @@ import time
 import numpy as np
 
 CANDIDATES = [
-    (256, 4, 8, 0), (256, 2, 8, 0), (256, 1, 8, 0), (256, 4, 16, 0), (256, 2, 4, 0),
-    (128, 4, 8, 0), (128, 2, 4, 0), (64, 2, 4, 0), (256, 4, 8, 2048), (256, 2, 8, 1024),
+    (256, 8, 8, 0), (256, 4, 8, 0), (256, 16, 8, 0), (256, 8, 16, 0), (256, 4, 4, 0),
+    (128, 8, 8, 0), (128, 4, 4, 0), (64, 4, 4, 0), (256, 8, 8, 2048), (256, 4, 8, 1024),
 ]
 
 

@@ -82,7 +82,7 @@ class SamrBatch:
         _check(_L().rhy_batch_create(ndim, self.n, iptr(d3), dptr(x3), self.ncmp, precision,
                                      C.byref(h)), "rhy_batch_create")
         self._h = h
-        self.tiles = (256, 4, 8, 0)
+        self.tiles = (256, 8, 8, 0)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -113,7 +113,7 @@ class SamrBatch:
         _check(_L().rhy_batch_bytes(self._h, C.byref(b)), "rhy_batch_bytes")
         return b.value
 
-    def set_tiles(self, block=256, nf_x=4, nf_yz=8, grid_cap=0):
+    def set_tiles(self, block=256, nf_x=8, nf_yz=8, grid_cap=0):
         _check(_L().rhy_batch_set_tiles(self._h, block, nf_x, nf_yz, grid_cap),
                "rhy_batch_set_tiles")
         self.tiles = (block, nf_x, nf_yz, grid_cap)
@@ -169,7 +169,7 @@ class SamrPool:
                                     C.byref(h)), "rhy_pool_create")
         self._h = h
         self.boxes = {}  # slot -> dims
-        self.tiles = (256, 4, 8, 0)
+        self.tiles = (256, 8, 8, 0)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
@@ -248,7 +248,7 @@ class SamrPool:
         assert cells.shape == box_shape(self.ndim, self.boxes[slot], self.ncmp)
         _check(_L().rhy_batch_upload(self._h, slot, dptr(cells)), "rhy_batch_upload")
 
-    def set_tiles(self, block=256, nf_x=4, nf_yz=8, grid_cap=0):
+    def set_tiles(self, block=256, nf_x=8, nf_yz=8, grid_cap=0):
         _check(_L().rhy_batch_set_tiles(self._h, block, nf_x, nf_yz, grid_cap),
                "rhy_batch_set_tiles")
         self.tiles = (block, nf_x, nf_yz, grid_cap)

@@ -147,10 +147,10 @@ void collect_timings(rhy_box* b) {
   b->pending.clear();
 }
 
-// Faces walked per thread along each axis (x walks with a stride of NF
-// doubles between lanes, so it takes fewer; y and z lanes stay coalesced).
+// Steps of 64 row cells per wave along x (the wave-cooperative walk) and
+// faces walked per thread along y and z (lanes consecutive in x).
 #ifndef RHY_NF_X
-#define RHY_NF_X 4
+#define RHY_NF_X 8
 #endif
 #ifndef RHY_NF_YZ
 #define RHY_NF_YZ 8

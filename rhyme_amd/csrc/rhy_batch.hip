@@ -283,7 +283,7 @@ struct rhy_batch {
   int64_t arena_elems = 0;
   std::vector<BSeg> hsegs;
   int32_t nflux_tiles = 0, nupd_tiles = 0;
-  int32_t block = 256, nf_x = 4, nf_yz = 8, grid_cap = 0;
+  int32_t block = 256, nf_x = 8, nf_yz = 8, grid_cap = 0;
   bool dirty = true;
   bool partials_valid = false;  // CFL partials match the state and tile table
   rhy_params_t last_params{};

@@ -157,10 +157,73 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 // One thread's walk: nf_max consecutive faces normal to A starting at face
 // chunk `t` of the box (thread t of the axis' index space); returns false
 // if t is past the end.
+// Faces normal to x, walked by whole waves.  The cells of all x rows,
+// including the ghost cell on each side whose half steps the end faces use,
+// are numbered row after row, g = row * (n0 + 2) + c with c = 0..n0+1 the
+// cell's x index; at each of `steps` steps the 64 lanes of a wave take 64
+// consecutive cells, form their half steps, take the right state of the cell
+// below from lane l-1 (lane 0: lane 63's of the previous step, from a
+// warm-up step at the start) and solve the face between the two.  A wave's
+// loads and flux stores are contiguous along x, and each half step is formed
+// once (plus one warm-up step per wave).
+template <int ND, int LIM, typename R, class PW>
+__device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
+                                           const R* __restrict__ cells, R* __restrict__ F,
+                                           double dt, double dx, int steps, int64_t t) {
+  constexpr int NV = ND + 2;
+  const int64_t rowlen = G.n[0] + 2;
+  const int64_t ncells = rowlen * G.n[1] * G.n[2];
+  const int lane = (int)(t & 63);
+  const int64_t base = (t >> 6) * 64 * (int64_t)steps;  // first cell of the wave
+  if (base >= ncells) return;                            // the whole wave
+  const R hdtdx = (R)(.5 * dt / dx);
+  R carry[NV];
+#pragma unroll
+  for (int c = 0; c < NV; ++c) carry[c] = R(0);
+#pragma unroll 1
+  for (int m = -1; m < steps; ++m) {
+    const int64_t g = base + (int64_t)m * 64 + lane;
+    const bool valid = g >= 0 && g < ncells;
+    R lw[NV], rn[NV];
+    int ci = 0;
+    int64_t o = 0;
+    if (valid) {
+      const int64_t row = g / rowlen;
+      ci = (int)(g - row * rowlen);
+      o = goff(G, ci, (int)(row % G.n[1]) + 1, (int)(row / G.n[1]) + 1);
+      R um[NV], u0[NV], up[NV];
+      load_state<ND, R>(cells, G.plane, o - 1, um);
+      load_state<ND, R>(cells, G.plane, o, u0);
+      load_state<ND, R>(cells, G.plane, o + 1, up);
+      half_step<ND, 0, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) lw[c] = rn[c] = R(0);
+    }
+    R rp[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      const R below = __shfl_up(rn[c], 1);
+      rp[c] = lane == 0 ? carry[c] : below;
+      carry[c] = __shfl(rn[c], 63);
+    }
+    if (m >= 0 && valid && ci >= 1) {  // face ci - 1, between cells ci - 1 and ci
+      R Fr[NV];
+      riemann_flux<ND, 0, PW, R>(k, rp, lw, Fr);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) F[o - 1 + c * G.plane] = Fr[c];
+    }
+  }
+}
+
 template <int ND, int A, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
                                           double dt, double dx, int nf_max, int64_t t) {
+  if constexpr (A == 0) {
+    flux_xrows<ND, LIM, R, PW>(k, G, cells, F, dt, dx, nf_max, t);
+    return;
+  }
   constexpr int NV = ND + 2;
   const int nA = G.n[A];
   const int nchunk = (nA + 1 + nf_max - 1) / nf_max;
@@ -236,7 +299,12 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
 }
 
 // Threads of the flux index space of axis A for faces walked nf at a time.
+// (x: whole waves of 64 lanes, each over 64 * nf consecutive row cells)
 __host__ __device__ inline int64_t flux_threads(const Geom& G, int A, int nf) {
+  if (A == 0) {
+    const int64_t cells = (int64_t)(G.n[0] + 2) * G.n[1] * G.n[2];
+    return (cells + 64 * (int64_t)nf - 1) / (64 * (int64_t)nf) * 64;
+  }
   int64_t n = (G.n[A] + 1 + nf - 1) / nf;
   for (int d = 0; d < 3; ++d)
     if (d != A) n *= G.n[d];

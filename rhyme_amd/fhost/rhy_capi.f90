@@ -1,14 +1,17 @@
 ! rhy_capi.f90 -- ISO_C_BINDING view of include/rhyme_hip.h.
 !
-! The derived type rhy_params_t and every interface below mirror the C
-! declarations field for field; the Fortran host modules (rhyme_host.f90)
-! are the only users.
+! The derived types rhy_params_t and rhy_ic_t and every interface below
+! mirror the C declarations field for field.  The Fortran host modules
+! (rhyme_host.f90) use the box and unit-level entry points; the run loop,
+! batch, pool and painting interfaces are here for Fortran drivers that
+! advance many boxes or serve jobs.
 module rhy_capi
    use, intrinsic :: iso_c_binding
    implicit none
 
-   integer(c_int), parameter :: RHY_OK = 0
+   integer(c_int), parameter :: RHY_OK = 0, RHY_ERR_OOM = -3
    integer(c_int), parameter :: RHY_FP64 = 0, RHY_MIXED = 1
+   integer(c_int32_t), parameter :: RHY_IC_TWO_STATE = 1, RHY_IC_BLAST = 2
    integer, parameter :: RHY_MAX_CMP = 16
 
    type, bind(C) :: rhy_params_t
@@ -28,6 +31,15 @@ module rhy_capi
       integer(c_int32_t) :: reserved = 0
       real(c_double) :: cons_inflows(RHY_MAX_CMP, 6) = 0
    end type rhy_params_t
+
+   type, bind(C) :: rhy_ic_t
+      integer(c_int32_t) :: kind = 0
+      integer(c_int32_t) :: axis = 0
+      real(c_double) :: gamma = 0
+      real(c_double) :: rho_l = 0, p_l = 0, rho_r = 0, p_r = 0
+      real(c_double) :: radius = 0, jitter = 0, p0 = 0, energy = 0
+      integer(c_int64_t) :: seed = 0
+   end type rhy_ic_t
 
    interface
       integer(c_int) function rhy_init(device) bind(C, name="rhy_init")
@@ -130,6 +142,95 @@ module rhy_capi
          real(c_double), value :: dx, dt
          real(c_double), intent(out) :: l(*), r(*)
       end function rhy_half_step_batch
+
+      integer(c_int) function rhy_box_sync(box) bind(C, name="rhy_box_sync")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: box
+      end function rhy_box_sync
+
+      integer(c_int) function rhy_run(box, p, dx, nsteps, dts) bind(C, name="rhy_run")
+         import :: c_int, c_int32_t, c_ptr, c_double, rhy_params_t
+         type(c_ptr), value :: box
+         type(rhy_params_t), intent(in) :: p
+         real(c_double), intent(in) :: dx(3)
+         integer(c_int32_t), value :: nsteps
+         real(c_double), intent(out) :: dts(*)
+      end function rhy_run
+
+      integer(c_int) function rhy_box_paint(box, ic) bind(C, name="rhy_box_paint")
+         import :: c_int, c_ptr, rhy_ic_t
+         type(c_ptr), value :: box
+         type(rhy_ic_t), intent(in) :: ic
+      end function rhy_box_paint
+
+      integer(c_int) function rhy_batch_create(ndim, nboxes, dims, dx, ncmp, precision, out) &
+         bind(C, name="rhy_batch_create")
+         import :: c_int, c_int32_t, c_double, c_ptr
+         integer(c_int32_t), value :: ndim, nboxes, ncmp, precision
+         integer(c_int32_t), intent(in) :: dims(3, *)
+         real(c_double), intent(in) :: dx(3, *)
+         type(c_ptr), intent(out) :: out
+      end function rhy_batch_create
+
+      integer(c_int) function rhy_batch_destroy(batch) bind(C, name="rhy_batch_destroy")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function rhy_batch_destroy
+
+      integer(c_int) function rhy_batch_upload(batch, box, cells) &
+         bind(C, name="rhy_batch_upload")
+         import :: c_int, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: box
+         real(c_double), intent(in) :: cells(*)
+      end function rhy_batch_upload
+
+      integer(c_int) function rhy_batch_download(batch, box, cells) &
+         bind(C, name="rhy_batch_download")
+         import :: c_int, c_int32_t, c_double, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int32_t), value :: box
+         real(c_double), intent(out) :: cells(*)
+      end function rhy_batch_download
+
+      integer(c_int) function rhy_batch_run_async(batch, p, nsteps, use_graph) &
+         bind(C, name="rhy_batch_run_async")
+         import :: c_int, c_int32_t, c_ptr, rhy_params_t
+         type(c_ptr), value :: batch
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int32_t), value :: nsteps, use_graph
+      end function rhy_batch_run_async
+
+      integer(c_int) function rhy_batch_sync(batch) bind(C, name="rhy_batch_sync")
+         import :: c_int, c_ptr
+         type(c_ptr), value :: batch
+      end function rhy_batch_sync
+
+      integer(c_int) function rhy_pool_create(ndim, ncmp, precision, budget_bytes, &
+                                              max_boxes, out) bind(C, name="rhy_pool_create")
+         import :: c_int, c_int32_t, c_int64_t, c_ptr
+         integer(c_int32_t), value :: ndim, ncmp, precision, max_boxes
+         integer(c_int64_t), value :: budget_bytes
+         type(c_ptr), intent(out) :: out
+      end function rhy_pool_create
+
+      integer(c_int) function rhy_pool_add_ic(pool, dims, dx, ic, limiter, bc_types, slot) &
+         bind(C, name="rhy_pool_add_ic")
+         import :: c_int, c_int32_t, c_double, c_ptr, rhy_ic_t
+         type(c_ptr), value :: pool
+         integer(c_int32_t), intent(in) :: dims(3)
+         real(c_double), intent(in) :: dx(3)
+         type(rhy_ic_t), intent(in) :: ic
+         integer(c_int32_t), value :: limiter
+         integer(c_int32_t), intent(in) :: bc_types(6)
+         integer(c_int32_t), intent(out) :: slot
+      end function rhy_pool_add_ic
+
+      integer(c_int) function rhy_pool_remove(pool, slot) bind(C, name="rhy_pool_remove")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: pool
+         integer(c_int32_t), value :: slot
+      end function rhy_pool_remove
    end interface
 
 contains

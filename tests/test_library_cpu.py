@@ -39,6 +39,14 @@ def test_params_layout_matches_header():
     assert C.sizeof(oracle.OrcParams) == C.sizeof(_lib.RhyParams)
 
 
+def test_ic_layout_matches_header():
+    # int32 kind, int32 axis, double x9, uint64 seed
+    assert C.sizeof(_lib.RhyIC) == 4 + 4 + 8 * 9 + 8
+    assert _lib.RhyIC.gamma.offset == 8 and _lib.RhyIC.seed.offset == 80
+    txt = open(HEADER).read()
+    assert "double radius, jitter, p0, energy;" in txt and "uint64_t seed;" in txt
+
+
 def test_make_params_defaults_follow_reference():
     p = hydro.make_params(3)
     assert p.limiter == hydro.MINMOD and p.omega == 0.0        # slope_limiter_t
