@@ -157,6 +157,26 @@ int rhy_box_set_profiling(rhy_box_t *box, int32_t on);
 int rhy_box_kernel_stats(rhy_box_t *box, int32_t kidx, int64_t *launches,
                          double *total_ms);
 
+/* ---- diagnostics (rhyme_sanity_check_perform, rhyme_sanity_check/src/
+ * rhyme_sanity_check_perform.f90:34-94, and the totals of rhyme_samr_calc_
+ * total_mass/energy, rhyme_samr/src/rhyme_samr_calc_total_mass.f90:20-25) ----
+ * Over the interior: NaN cells per component, cells with rho < 0 and with
+ * e_tot < 0, the minima of rho and e_tot (NaNs skipped) with the 1-based
+ * (i,j,k) of their first occurrence in Fortran element order, and the sums of
+ * rho and e_tot; plus the reference's totals before its volume factor: the
+ * sums of the positive rho and e_tot values over the whole ghosted array.
+ * Sums are accumulated in double-double (compensated) and rounded once; the
+ * reduction order is fixed, so results repeat bit for bit. */
+typedef struct rhy_diag {
+    int64_t nan_count[RHY_MAX_CMP];
+    int64_t neg_rho, neg_e_tot;
+    double min_rho, min_e_tot;
+    int32_t min_rho_at[3], min_e_tot_at[3];
+    double mass, energy;
+    double mass_masked, energy_masked;
+} rhy_diag_t;
+int rhy_box_diagnostics(rhy_box_t *box, rhy_diag_t *out);
+
 /* ---- initial conditions painted on the device ----
  * No counterpart in the reference, which paints on the host
  * (rhyme_drawing_init, src/rhyme_drawing/src/rhyme_drawing_init.f90:19-78)
@@ -234,6 +254,8 @@ int rhy_batch_sync(rhy_batch_t *batch);
  * slot; entries of slots that were not active are left as they were) */
 int rhy_batch_last_dts(rhy_batch_t *batch, int32_t n, double *dts);
 const char *rhy_batch_last_error(void);
+/* rhy_box_diagnostics of one box (or pool slot) of a batch */
+int rhy_batch_diagnostics(rhy_batch_t *batch, int32_t box, rhy_diag_t *out);
 
 /* ---- pool: a batch whose boxes come and go (continuous batching) ----
  * No counterpart in the reference.  A pool holds a device arena of

@@ -28,10 +28,10 @@ EXPORTS = [
     "rhy_mh_solve", "rhy_run", "rhy_run_async", "rhy_box_last_dts",
     "rhy_box_set_profiling", "rhy_box_kernel_stats",
     "rhy_riemann_batch", "rhy_limiter_batch", "rhy_half_step_batch",
-    "rhy_flux_batch", "rhy_box_paint",
+    "rhy_flux_batch", "rhy_box_paint", "rhy_box_diagnostics",
     "rhy_batch_create", "rhy_batch_destroy", "rhy_batch_bytes", "rhy_batch_upload",
     "rhy_batch_download", "rhy_batch_set_tiles", "rhy_batch_run_async", "rhy_batch_sync",
-    "rhy_batch_last_dts", "rhy_batch_last_error",
+    "rhy_batch_last_dts", "rhy_batch_last_error", "rhy_batch_diagnostics",
     "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_add_ic",
     "rhy_pool_remove", "rhy_pool_free_bytes",
 ]
@@ -76,6 +76,31 @@ class RhyIC(C.Structure):
         ("energy", C.c_double),
         ("seed", C.c_uint64),
     ]
+
+
+class RhyDiag(C.Structure):
+    """rhy_diag_t: the diagnostics of one box."""
+    _fields_ = [
+        ("nan_count", C.c_int64 * MAX_CMP),
+        ("neg_rho", C.c_int64),
+        ("neg_e_tot", C.c_int64),
+        ("min_rho", C.c_double),
+        ("min_e_tot", C.c_double),
+        ("min_rho_at", C.c_int32 * 3),
+        ("min_e_tot_at", C.c_int32 * 3),
+        ("mass", C.c_double),
+        ("energy", C.c_double),
+        ("mass_masked", C.c_double),
+        ("energy_masked", C.c_double),
+    ]
+
+    def as_dict(self, ncmp, ndim):
+        return dict(nan_count=list(self.nan_count)[:ncmp], neg_rho=self.neg_rho,
+                    neg_e_tot=self.neg_e_tot, min_rho=self.min_rho, min_e_tot=self.min_e_tot,
+                    min_rho_at=list(self.min_rho_at)[:ndim],
+                    min_e_tot_at=list(self.min_e_tot_at)[:ndim], mass=self.mass,
+                    energy=self.energy, mass_masked=self.mass_masked,
+                    energy_masked=self.energy_masked)
 
 
 def make_ic(ic):
@@ -131,6 +156,7 @@ def load():
                                           C.c_double, dp, dp]),
         "rhy_flux_batch": (C.c_int, [pp, C.c_int64, dp, ip, dp]),
         "rhy_box_paint": (C.c_int, [vp, C.POINTER(RhyIC)]),
+        "rhy_box_diagnostics": (C.c_int, [vp, C.POINTER(RhyDiag)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

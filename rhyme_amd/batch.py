@@ -28,6 +28,7 @@ _BATCH_SIGS = {
     "rhy_batch_sync": (C.c_int, [C.c_void_p]),
     "rhy_batch_last_dts": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "rhy_batch_last_error": (C.c_char_p, []),
+    "rhy_batch_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_lib.RhyDiag)]),
     "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "rhy_pool_box_bytes": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
@@ -137,6 +138,11 @@ class SamrBatch:
 
     def zones(self):
         return int(sum(int(np.prod(d)) for d in self.dims))
+
+    def diagnostics(self, i):
+        d = _lib.RhyDiag()
+        _check(_L().rhy_batch_diagnostics(self._h, i, C.byref(d)), "rhy_batch_diagnostics")
+        return d.as_dict(self.ncmp, self.ndim)
 
 
 def box_bytes(ndim, dims, ncmp=None, precision=FP64):
@@ -268,3 +274,9 @@ class SamrPool:
 
     def zones(self):
         return int(sum(int(np.prod(d)) for d in self.boxes.values()))
+
+    def diagnostics(self, slot):
+        """rhy_diag_t of the box in `slot`, as a dict."""
+        d = _lib.RhyDiag()
+        _check(_L().rhy_batch_diagnostics(self._h, slot, C.byref(d)), "rhy_batch_diagnostics")
+        return d.as_dict(self.ncmp, self.ndim)

@@ -3,6 +3,7 @@
 // host mirror transfers, and the launch sequences of the module procedures.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -60,6 +61,7 @@ struct rhy_box {
   double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
   double* dt = nullptr;           // current dt
   double* dt_log = nullptr;       // per-step dt of the last run
+  void* diag = nullptr;           // workgroup partials of rhy_box_diagnostics
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
   bool profiling = false;
@@ -119,18 +121,27 @@ BcTypes make_bc(const rhy_params_t* p) {
 
 unsigned grid_for(int64_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
+// A launch is timed only if both events could be created and recorded;
+// otherwise it is left out of the statistics.
 void timed_begin(rhy_box* b, int kidx, TimedLaunch& tl) {
   tl.kidx = kidx;
   tl.a = tl.b = nullptr;
   if (!b->profiling) return;
-  hipEventCreate(&tl.a);
-  hipEventCreate(&tl.b);
-  hipEventRecord(tl.a, b->stream);
+  if (hipEventCreate(&tl.a) != hipSuccess || hipEventCreate(&tl.b) != hipSuccess ||
+      hipEventRecord(tl.a, b->stream) != hipSuccess) {
+    if (tl.a) (void)hipEventDestroy(tl.a);
+    if (tl.b) (void)hipEventDestroy(tl.b);
+    tl.a = tl.b = nullptr;
+  }
 }
 
 void timed_end(rhy_box* b, TimedLaunch& tl) {
-  if (!b->profiling) return;
-  hipEventRecord(tl.b, b->stream);
+  if (!b->profiling || !tl.a) return;
+  if (hipEventRecord(tl.b, b->stream) != hipSuccess) {
+    (void)hipEventDestroy(tl.a);
+    (void)hipEventDestroy(tl.b);
+    return;
+  }
   b->pending.push_back(tl);
 }
 
@@ -141,8 +152,8 @@ void collect_timings(rhy_box* b) {
       b->launches[tl.kidx] += 1;
       b->total_ms[tl.kidx] += ms;
     }
-    hipEventDestroy(tl.a);
-    hipEventDestroy(tl.b);
+    (void)hipEventDestroy(tl.a);
+    (void)hipEventDestroy(tl.b);
   }
   b->pending.clear();
 }
@@ -288,7 +299,7 @@ double min_dx(int nd, const double* dx) {
 
 int ensure_dt_log(rhy_box* b, int32_t n) {
   if (n <= b->dt_log_cap) return RHY_OK;
-  if (b->dt_log) hipFree(b->dt_log);
+  if (b->dt_log) (void)hipFree(b->dt_log);
   b->dt_log = nullptr;
   b->dt_log_cap = 0;
   if (hipMalloc(&b->dt_log, sizeof(double) * (size_t)n) != hipSuccess)
@@ -327,9 +338,9 @@ int check_box(rhy_box* b) {
 }
 
 void free_box_memory(rhy_box* b) {
-  if (b->cells) hipFree(b->cells);
-  if (b->flux) hipFree(b->flux);
-  if (b->partial) hipFree(b->partial);
+  if (b->cells) (void)hipFree(b->cells);
+  if (b->flux) (void)hipFree(b->flux);
+  if (b->partial) (void)hipFree(b->partial);
   b->cells = nullptr;
   b->flux = nullptr;
   b->partial = nullptr;
@@ -459,13 +470,14 @@ int rhy_box_resize(rhy_box_t* b, const int32_t dims[3]) {
 
 int rhy_box_destroy(rhy_box_t* b) {
   if (!b) return RHY_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
   collect_timings(b);
   free_box_memory(b);
-  if (b->scalars) hipFree(b->scalars);
-  if (b->dt) hipFree(b->dt);
-  if (b->dt_log) hipFree(b->dt_log);
-  if (b->stream) hipStreamDestroy(b->stream);
+  if (b->scalars) (void)hipFree(b->scalars);
+  if (b->dt) (void)hipFree(b->dt);
+  if (b->dt_log) (void)hipFree(b->dt_log);
+  if (b->diag) (void)hipFree(b->diag);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RHY_OK;
 }
@@ -523,6 +535,40 @@ int rhy_bc_set_boundaries(rhy_box_t* b, const rhy_params_t* p) {
   else rc = launch_bc<3>(b, p, -1);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+constexpr int kDiagBlocks = 256;
+
+int rhy_box_diagnostics(rhy_box_t* b, rhy_diag_t* out) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!out) return fail(RHY_ERR_ARG, "out is NULL");
+  if (!b->diag && hipMalloc(&b->diag, sizeof(DiagPartial) * kDiagBlocks) != hipSuccess)
+    return fail(RHY_ERR_OOM, "diagnostics buffer allocation failed");
+  const Geom& G = b->geom;
+  const unsigned nb = std::min<unsigned>(grid_for(G.plane), kDiagBlocks);
+  DiagPartial* dp = static_cast<DiagPartial*>(b->diag);
+  const bool mixed = b->precision == RHY_MIXED;
+#define RHY_DIAG(ND)                                                                          \
+  do {                                                                                        \
+    if (mixed)                                                                                \
+      hipLaunchKernelGGL((diag_kernel<ND, float>), dim3(nb), dim3(kThreads), 0, b->stream, G, \
+                         b->ncmp, static_cast<const float*>(b->cells), dp);                   \
+    else                                                                                      \
+      hipLaunchKernelGGL((diag_kernel<ND, double>), dim3(nb), dim3(kThreads), 0, b->stream,   \
+                         G, b->ncmp, static_cast<const double*>(b->cells), dp);               \
+  } while (0)
+  if (b->ndim == 1) RHY_DIAG(1);
+  else if (b->ndim == 2) RHY_DIAG(2);
+  else RHY_DIAG(3);
+#undef RHY_DIAG
+  HIP_TRY(hipGetLastError());
+  std::vector<DiagPartial> parts(nb);
+  HIP_TRY(hipMemcpyAsync(parts.data(), dp, sizeof(DiagPartial) * nb, hipMemcpyDeviceToHost,
+                         b->stream));
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  diag_finish(parts, G, out);
   return RHY_OK;
 }
 
@@ -682,7 +728,7 @@ namespace {
 struct DevBuf {
   void* p = nullptr;
   ~DevBuf() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
   }
 };
 

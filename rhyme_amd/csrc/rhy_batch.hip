@@ -303,6 +303,7 @@ struct rhy_batch {
   double* inflow = nullptr;
   hipGraphExec_t graph = nullptr;
   uint64_t graph_key = 0;
+  DiagPartial* diag = nullptr;  // workgroup partials of rhy_batch_diagnostics
 };
 
 namespace {
@@ -382,7 +383,7 @@ void arena_free(rhy_batch* b, int64_t off, int64_t len) {
 
 int grow(void** ptr, size_t* cap, size_t need, size_t elem) {
   if (need <= *cap) return RHY_OK;
-  if (*ptr) hipFree(*ptr);
+  if (*ptr) (void)hipFree(*ptr);
   *ptr = nullptr;
   *cap = 0;
   size_t n = std::max(need, (size_t)64);
@@ -438,7 +439,7 @@ int build_tiles(rhy_batch* b) {
     BHIP_TRY(hipMemcpy(b->dactive, b->active.data(), sizeof(int32_t) * b->active.size(),
                        hipMemcpyHostToDevice));
   if (b->graph) {
-    hipGraphExecDestroy(b->graph);
+    (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
   }
   b->dirty = false;
@@ -570,7 +571,7 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   const int64_t need = (int64_t)nsteps * (int64_t)b->slots.size();
   if (need > (int64_t)b->dtlog_steps * (int64_t)b->slots.size()) {
     BHIP_TRY(hipStreamSynchronize(b->stream));
-    if (b->dtlog) hipFree(b->dtlog);
+    if (b->dtlog) (void)hipFree(b->dtlog);
     b->dtlog = nullptr;
     if (hipMalloc(&b->dtlog, sizeof(double) * (size_t)need) != hipSuccess)
       return bfail(RHY_ERR_OOM, "dt log allocation failed");
@@ -592,14 +593,14 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   mix(p, sizeof(*p));
   mix(&b->dtlog_steps, sizeof(b->dtlog_steps));
   if (!b->graph || b->graph_key != key) {
-    if (b->graph) hipGraphExecDestroy(b->graph);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     hipGraph_t g = nullptr;
     BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
     BHIP_TRY(hipStreamEndCapture(b->stream, &g));
     hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
-    hipGraphDestroy(g);
+    (void)hipGraphDestroy(g);
     if (e != hipSuccess) return bfail(RHY_ERR_HIP, "hipGraphInstantiate failed");
     b->graph_key = key;
   }
@@ -825,13 +826,13 @@ int rhy_pool_free_bytes(rhy_batch_t* b, int64_t* free_bytes, int64_t* largest_by
 
 int rhy_batch_destroy(rhy_batch_t* b) {
   if (!b) return RHY_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
-  if (b->graph) hipGraphExecDestroy(b->graph);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
-                  (void*)b->inflow})
-    if (p) hipFree(p);
-  if (b->stream) hipStreamDestroy(b->stream);
+                  (void*)b->inflow, (void*)b->diag})
+    if (p) (void)hipFree(p);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return RHY_OK;
 }
@@ -923,6 +924,38 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
 int rhy_batch_sync(rhy_batch_t* b) {
   if (!b) return bfail(RHY_ERR_ARG, "batch is NULL");
   BHIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_batch_diagnostics(rhy_batch_t* b, int32_t box, rhy_diag_t* out) {
+  if (!slot_ok(b, box) || !out) return bfail(RHY_ERR_ARG, "bad diagnostics request");
+  constexpr int kBlocks = 256;
+  if (!b->diag && hipMalloc(&b->diag, sizeof(DiagPartial) * kBlocks) != hipSuccess)
+    return bfail(RHY_ERR_OOM, "diagnostics buffer allocation failed");
+  const BBox& B = b->slots[box].B;
+  const Geom& G = B.G;
+  const unsigned nb = std::min<unsigned>(cdiv(G.plane, 256), kBlocks);
+  const bool mixed = b->precision == RHY_MIXED;
+#define RHY_BDIAG(ND)                                                                        \
+  do {                                                                                       \
+    if (mixed)                                                                               \
+      hipLaunchKernelGGL((diag_kernel<ND, float>), dim3(nb), dim3(256), 0, b->stream, G,      \
+                         b->ncmp, static_cast<const float*>(b->arena) + B.cell_off, b->diag); \
+    else                                                                                     \
+      hipLaunchKernelGGL((diag_kernel<ND, double>), dim3(nb), dim3(256), 0, b->stream, G,     \
+                         b->ncmp, static_cast<const double*>(b->arena) + B.cell_off,          \
+                         b->diag);                                                           \
+  } while (0)
+  if (b->ndim == 1) RHY_BDIAG(1);
+  else if (b->ndim == 2) RHY_BDIAG(2);
+  else RHY_BDIAG(3);
+#undef RHY_BDIAG
+  BHIP_TRY(hipGetLastError());
+  std::vector<DiagPartial> parts(nb);
+  BHIP_TRY(hipMemcpyAsync(parts.data(), b->diag, sizeof(DiagPartial) * nb,
+                          hipMemcpyDeviceToHost, b->stream));
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  diag_finish(parts, G, out);
   return RHY_OK;
 }
 

@@ -8,7 +8,9 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <vector>
 
+#include "../../include/rhyme_hip.h"
 #include "rhy_physics.hpp"
 
 // Minimum waves per SIMD requested for the flux kernels (caps their VGPRs:
@@ -463,9 +465,9 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const R* __r
 // dt = courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the product
 // courant*minval(dx) is formed on the host.  Writes dt (and logs it) and the
 // raw max_u bits to maxu_out when given.
-static __global__ __launch_bounds__(1024) void dt_kernel(const double* __restrict__ partial, int32_t n,
-                                                  double cmdx, double* dt_out, double* dt_log,
-                                                  double* maxu_out) {
+[[maybe_unused]] static __global__ __launch_bounds__(1024) void dt_kernel(
+    const double* __restrict__ partial, int32_t n, double cmdx, double* dt_out, double* dt_log,
+    double* maxu_out) {
   __shared__ double wave_max[16];
   double v = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) v = partial[i] > v ? partial[i] : v;
@@ -488,8 +490,159 @@ static __global__ __launch_bounds__(1024) void dt_kernel(const double* __restric
   }
 }
 
-static __global__ void set_scalar_kernel(double* dst, double v) {
+[[maybe_unused]] static __global__ void set_scalar_kernel(double* dst, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// -------------------------------------------------------- diagnostics ----
+
+// (hi, lo) += (bh, bl) in double-double (Knuth's two-sum on the high parts;
+// needs -ffp-contract=off, which the whole library is built with).
+__host__ __device__ inline void dd_add(double& hi, double& lo, double bh, double bl) {
+  const double s = hi + bh;
+  const double bb = s - hi;
+  double e = (hi - (s - bb)) + (bh - bb);
+  e = e + (lo + bl);
+  hi = s + e;
+  lo = e - (hi - s);
+}
+
+// A workgroup's share of the diagnostics (see rhy_diag_t).
+struct DiagPartial {
+  double sum_hi[4], sum_lo[4];  // rho, e_tot (interior); rho > 0, e_tot > 0 (all)
+  double min_v[2];              // rho, e_tot (interior, NaN skipped)
+  int64_t min_at[2];            // linear interior index of the minimum, -1: none
+  int64_t neg[2];               // rho < 0, e_tot < 0 (interior)
+  int64_t nan[RHY_MAX_CMP];     // per component (interior)
+};
+
+__device__ __forceinline__ void min_take(double& v, int64_t& at, double ov, int64_t oat) {
+  // smaller value wins; on a tie the earlier cell (Fortran element order)
+  if (oat >= 0 && (at < 0 || ov < v || (ov == v && oat < at))) {
+    v = ov;
+    at = oat;
+  }
+}
+
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void diag_kernel(Geom G, int32_t ncmp,
+                                                   const R* __restrict__ cells,
+                                                   DiagPartial* __restrict__ out) {
+  constexpr int NV = ND + 2;
+  DiagPartial p;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) p.sum_hi[s] = p.sum_lo[s] = 0.0;
+  p.min_v[0] = p.min_v[1] = 0.0;
+  p.min_at[0] = p.min_at[1] = -1;
+  p.neg[0] = p.neg[1] = 0;
+#pragma unroll
+  for (int c = 0; c < RHY_MAX_CMP; ++c) p.nan[c] = 0;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < G.plane; t += stride) {
+    int q[3];
+    bool inside = true;
+    int64_t rem = t;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int ext = G.n[d] + 2 * G.g[d];
+      q[d] = (int)(rem % ext) - G.g[d];
+      rem /= ext;
+      inside = inside && q[d] >= 0 && q[d] < G.n[d];
+    }
+    const double rho = (double)cells[t];
+    const double e = (double)cells[t + (int64_t)(NV - 1) * G.plane];
+    if (rho > 0.0) dd_add(p.sum_hi[2], p.sum_lo[2], rho, 0.0);
+    if (e > 0.0) dd_add(p.sum_hi[3], p.sum_lo[3], e, 0.0);
+    if (!inside) continue;
+    const int64_t lin = ((int64_t)q[2] * G.n[1] + q[1]) * G.n[0] + q[0];
+    dd_add(p.sum_hi[0], p.sum_lo[0], rho, 0.0);
+    dd_add(p.sum_hi[1], p.sum_lo[1], e, 0.0);
+    if (rho < 0.0) p.neg[0] += 1;
+    if (e < 0.0) p.neg[1] += 1;
+    if (rho == rho) min_take(p.min_v[0], p.min_at[0], rho, lin);
+    if (e == e) min_take(p.min_v[1], p.min_at[1], e, lin);
+#pragma unroll
+    for (int c = 0; c < RHY_MAX_CMP; ++c) {
+      if (c < ncmp) {
+        const double v = (double)cells[t + (int64_t)c * G.plane];
+        if (v != v) p.nan[c] += 1;
+      }
+    }
+  }
+  // wave, then workgroup reduction in a fixed order
+  __shared__ DiagPartial wave_part[4];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const double oh = __shfl_xor(p.sum_hi[s], off, 64);
+      const double ol = __shfl_xor(p.sum_lo[s], off, 64);
+      dd_add(p.sum_hi[s], p.sum_lo[s], oh, ol);
+    }
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const double ov = __shfl_xor(p.min_v[m], off, 64);
+      const long long oat = __shfl_xor((long long)p.min_at[m], off, 64);
+      min_take(p.min_v[m], p.min_at[m], ov, oat);
+      p.neg[m] += __shfl_xor((long long)p.neg[m], off, 64);
+    }
+#pragma unroll
+    for (int c = 0; c < RHY_MAX_CMP; ++c) p.nan[c] += __shfl_xor((long long)p.nan[c], off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) wave_part[w] = p;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    DiagPartial r = wave_part[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) {
+      const DiagPartial& o = wave_part[i];
+      for (int s = 0; s < 4; ++s) dd_add(r.sum_hi[s], r.sum_lo[s], o.sum_hi[s], o.sum_lo[s]);
+      for (int m = 0; m < 2; ++m) {
+        min_take(r.min_v[m], r.min_at[m], o.min_v[m], o.min_at[m]);
+        r.neg[m] += o.neg[m];
+      }
+      for (int c = 0; c < RHY_MAX_CMP; ++c) r.nan[c] += o.nan[c];
+    }
+    out[blockIdx.x] = r;
+  }
+}
+
+// Host: the workgroup partials combined in order, into the C struct.
+inline void diag_finish(const std::vector<DiagPartial>& parts, const Geom& G,
+                        rhy_diag_t* out) {
+  double hi[4] = {0, 0, 0, 0}, lo[4] = {0, 0, 0, 0};
+  double mv[2] = {0, 0};
+  int64_t mat[2] = {-1, -1}, neg[2] = {0, 0}, nan[RHY_MAX_CMP] = {0};
+  for (const DiagPartial& p : parts) {
+    for (int s = 0; s < 4; ++s) dd_add(hi[s], lo[s], p.sum_hi[s], p.sum_lo[s]);
+    for (int m = 0; m < 2; ++m) {
+      if (p.min_at[m] >= 0 &&
+          (mat[m] < 0 || p.min_v[m] < mv[m] || (p.min_v[m] == mv[m] && p.min_at[m] < mat[m]))) {
+        mv[m] = p.min_v[m];
+        mat[m] = p.min_at[m];
+      }
+      neg[m] += p.neg[m];
+    }
+    for (int c = 0; c < RHY_MAX_CMP; ++c) nan[c] += p.nan[c];
+  }
+  *out = rhy_diag_t{};
+  for (int c = 0; c < RHY_MAX_CMP; ++c) out->nan_count[c] = nan[c];
+  out->neg_rho = neg[0];
+  out->neg_e_tot = neg[1];
+  out->mass = hi[0] + lo[0];
+  out->energy = hi[1] + lo[1];
+  out->mass_masked = hi[2] + lo[2];
+  out->energy_masked = hi[3] + lo[3];
+  double* mins[2] = {&out->min_rho, &out->min_e_tot};
+  int32_t* ats[2] = {out->min_rho_at, out->min_e_tot_at};
+  for (int m = 0; m < 2; ++m) {
+    *mins[m] = mat[m] >= 0 ? mv[m] : NAN;
+    int64_t r = mat[m] < 0 ? 0 : mat[m];
+    for (int d = 0; d < 3; ++d) {
+      ats[m][d] = mat[m] < 0 ? 0 : (int32_t)(r % G.n[d]) + 1;
+      r /= G.n[d];
+    }
+  }
 }
 
 // ------------------------------------------------- initial conditions ----

@@ -203,6 +203,15 @@ class SamrBox:
     def sync(self):
         check(_lib.load().rhy_box_sync(self._h), "rhy_box_sync")
 
+    def diagnostics(self):
+        """rhy_box_diagnostics of the device state, as a dict: NaN cells per
+        component, negative rho / e_tot counts, their minima and 1-based
+        positions, interior sums of rho and e_tot (compensated), and the
+        reference's masked totals (positive values over the ghosted array)."""
+        d = _lib.RhyDiag()
+        check(_lib.load().rhy_box_diagnostics(self._h, C.byref(d)), "rhy_box_diagnostics")
+        return d.as_dict(self.ncmp, self.ndim)
+
     def paint(self, ic):
         """Paint the device state from `ic` (a dict of rhy_ic_t fields); the
         host mirror is not touched (download() to see it)."""

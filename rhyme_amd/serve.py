@@ -14,6 +14,13 @@ CFL dt), so every job's result is bit-identical to running it by itself
 with hydro.run; the reference has no counterpart (it advances one
 configuration per process, src/main.f90:106-177).
 
+Failure detection: with `check_every` = N, every N quanta the device
+diagnostics (rhy_batch_diagnostics, the sanity-check quantities) of each
+resident job are read, and a job whose state holds a NaN or a negative
+density or total energy is ended at once (`failed`), its slot freed; the
+other jobs are not touched.  With `diagnose`, every retired job carries its
+final diagnostics.
+
 Admission: a job is admitted when the bytes of all resident jobs plus its
 own (state planes + flux scratch, batch.box_bytes) stay within the budget and
 its pool's arena has a free block large enough; otherwise it waits, and so
@@ -54,6 +61,8 @@ class Job:
     slot: int = None
     result: np.ndarray = None
     dts: list = field(default_factory=list)
+    diag: dict = None           # rhy_diag_t of the final (or failing) state
+    failed: str = None          # why the job was ended early, if it was
 
     def zones(self):
         return int(np.prod(self.dims))
@@ -125,7 +134,8 @@ class HydroServer:
     """Continuous batching of 2D and 3D jobs under one HBM budget."""
 
     def __init__(self, budget_bytes, precision=FP64, quantum=10, params=None, max_boxes=4096,
-                 tiles=None, keep_results=True, record_dts=False):
+                 tiles=None, keep_results=True, record_dts=False, check_every=0,
+                 diagnose=False):
         self.budget = int(budget_bytes)
         self.precision = precision
         self.quantum = int(quantum)
@@ -138,6 +148,9 @@ class HydroServer:
                 self.pools[nd].set_tiles(*tiles[nd])
         self.keep_results = keep_results
         self.record_dts = record_dts
+        self.check_every = int(check_every)
+        self.diagnose = diagnose
+        self.failures = 0
         self.resident = {2: {}, 3: {}}  # slot -> job
         self.resident_bytes = 0
         self.peak_bytes = 0
@@ -175,8 +188,21 @@ class HydroServer:
         self.peak_bytes = max(self.peak_bytes, self.resident_bytes)
         return True
 
+    @staticmethod
+    def unhealthy(diag):
+        """Why a state fails the sanity checks (None if it does not)."""
+        if any(diag["nan_count"]):
+            return "NaN in components %s" % [c for c, n in enumerate(diag["nan_count"]) if n]
+        if diag["neg_rho"]:
+            return "negative density (%d cells)" % diag["neg_rho"]
+        if diag["neg_e_tot"]:
+            return "negative total energy (%d cells)" % diag["neg_e_tot"]
+        return None
+
     def _retire(self, job, now):
         pool = self.pools[job.ndim]
+        if self.diagnose and job.diag is None:
+            job.diag = pool.diagnostics(job.slot)
         if self.keep_results:
             job.result = pool.download(job.slot)
         pool.remove(job.slot)
@@ -223,6 +249,7 @@ class HydroServer:
             tc = clock()
             self.seconds["quanta"] += tc - tb
             now = tc - t0
+            check = self.check_every and self.quanta % self.check_every == 0
             for nd in busy:
                 q = steps[nd]
                 dts = self.pools[nd].last_dts(q) if self.record_dts else None
@@ -230,7 +257,11 @@ class HydroServer:
                     if dts is not None:
                         job.dts.extend(dts[:, slot].tolist())
                     job.done += q
-                    if job.done >= job.steps:
+                    if check:
+                        job.diag = self.pools[nd].diagnostics(slot)
+                        job.failed = self.unhealthy(job.diag)
+                    if job.failed or job.done >= job.steps:
+                        self.failures += job.failed is not None
                         self._retire(job, now)
                         left -= 1
             self.seconds["retire"] += clock() - tc
@@ -238,13 +269,15 @@ class HydroServer:
 
 
 def latency_report(jobs):
-    """Completion latency (finish - arrival) percentiles and throughput."""
-    lat = np.array([j.finished - j.arrival for j in jobs])
-    wait = np.array([j.admitted - j.arrival for j in jobs])
+    """Completion latency (finish - arrival) percentiles and throughput of
+    the jobs that completed (failed ones are counted apart)."""
+    done = [j for j in jobs if not j.failed]
+    lat = np.array([j.finished - j.arrival for j in done])
+    wait = np.array([j.admitted - j.arrival for j in done])
     span = max(j.finished for j in jobs) - min(j.arrival for j in jobs)
-    zu = float(sum(j.zones() * j.steps for j in jobs))
+    zu = float(sum(j.zones() * j.done for j in jobs))
     return dict(
-        jobs=len(jobs), p50_ms=1e3 * float(np.percentile(lat, 50)),
+        jobs=len(jobs), failed=len(jobs) - len(done), p50_ms=1e3 * float(np.percentile(lat, 50)),
         p99_ms=1e3 * float(np.percentile(lat, 99)), max_ms=1e3 * float(lat.max()),
         wait_p50_ms=1e3 * float(np.percentile(wait, 50)),
         wait_p99_ms=1e3 * float(np.percentile(wait, 99)),

@@ -130,8 +130,9 @@ def test_latency_report_fields():
     for i, j in enumerate(jobs):
         j.admitted = j.arrival + 0.01
         j.finished = j.arrival + 0.1 * (i + 1)
+        j.done = j.steps
     rep = serve.latency_report(jobs)
-    assert rep["jobs"] == 4
+    assert rep["jobs"] == 4 and rep["failed"] == 0
     assert np.isclose(rep["p50_ms"], 250.0) and rep["max_ms"] == pytest.approx(400.0)
     assert rep["zone_updates"] == sum(2 * 8 ** j.ndim for j in jobs)
     span = max(j.finished for j in jobs) - jobs[0].arrival
@@ -181,3 +182,24 @@ def test_paint_blast_properties():
     ghosts = c.copy()
     ghosts[(slice(None),) + inner] = 0
     assert not ghosts.any()
+
+
+def test_failed_jobs_are_reported_apart():
+    jobs = serve.serving_jobs(njobs=4, rate=10.0, seed=0, steps=6, n2d=(8, 8), n3d=(8, 8))
+    for i, j in enumerate(jobs):
+        j.admitted = j.arrival
+        j.finished = j.arrival + 0.1 * (i + 1)
+        j.done = j.steps if i else 2
+    jobs[0].failed = "NaN in components [0]"
+    rep = serve.latency_report(jobs)
+    assert rep["failed"] == 1
+    assert rep["p50_ms"] == pytest.approx(300.0)     # of the three completed jobs
+    assert rep["zone_updates"] == 2 * 8 ** jobs[0].ndim + sum(6 * 8 ** j.ndim for j in jobs[1:])
+
+
+def test_unhealthy_rules():
+    ok = dict(nan_count=[0] * 9, neg_rho=0, neg_e_tot=0)
+    assert serve.HydroServer.unhealthy(ok) is None
+    assert "NaN" in serve.HydroServer.unhealthy(dict(ok, nan_count=[0, 0, 3] + [0] * 6))
+    assert "density" in serve.HydroServer.unhealthy(dict(ok, neg_rho=1))
+    assert "energy" in serve.HydroServer.unhealthy(dict(ok, neg_e_tot=2))
