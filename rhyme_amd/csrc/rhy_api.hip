@@ -59,6 +59,8 @@ struct rhy_box {
   double* partial = nullptr;      // per-block CFL maxima of the last reduction
   int32_t partial_n = 0;
   double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
+  double inflow_copied[6 * RHY_MAX_CMP] = {};  // what the device inflow slots hold
+  bool inflow_valid = false;
   double* dt = nullptr;           // current dt
   double* dt_log = nullptr;       // per-step dt of the last run
   void* diag = nullptr;           // workgroup partials of rhy_box_diagnostics
@@ -256,10 +258,17 @@ int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
       if (d != a) m *= G.n[d];
     maxface = m > maxface ? m : maxface;
   }
-  // inflow states travel in device memory
+  // inflow states travel in device memory; they are copied only when they
+  // change, after the box's queued work (which may still read them) is done,
+  // so nothing refers to the caller's parameters once this call returns
   double* dinf = b->scalars + 16;
-  HIP_TRY(hipMemcpyAsync(dinf, p->cons_inflows, sizeof(double) * 6 * RHY_MAX_CMP,
-                         hipMemcpyHostToDevice, b->stream));
+  if (!b->inflow_valid ||
+      std::memcmp(b->inflow_copied, p->cons_inflows, sizeof(b->inflow_copied)) != 0) {
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    std::memcpy(b->inflow_copied, p->cons_inflows, sizeof(b->inflow_copied));
+    HIP_TRY(hipMemcpy(dinf, b->inflow_copied, sizeof(b->inflow_copied), hipMemcpyHostToDevice));
+    b->inflow_valid = true;
+  }
   TimedLaunch tl;
   timed_begin(b, 2, tl);
   dim3 grid(grid_for(maxface), face_only >= 0 ? 1 : 2 * ND);
