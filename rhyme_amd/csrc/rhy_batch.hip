@@ -668,6 +668,8 @@ int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* c
   Geom G{};
   int rc;
   if ((rc = box_geom(b->ndim, dims, G))) return rc;
+  for (int d = 0; d < b->ndim; ++d)
+    if (!(dx[d] > 0.0)) return bfail(RHY_ERR_ARG, "dx must be positive on active axes");
   int slot = -1;
   for (int i = 0; i < (int)b->slots.size(); ++i)
     if (!b->slots[i].active) {
@@ -678,6 +680,34 @@ int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* c
   const int64_t len = box_elems(G, b->ndim, b->ncmp);
   const int64_t off = arena_alloc(b, len);
   if (off < 0) return bfail(RHY_ERR_OOM, "box does not fit in the arena");
+  // the state first; the slot becomes active only once it is written
+  hipError_t e = hipSuccess;
+  if (cells) {
+    const size_t n = (size_t)G.plane * b->ncmp;
+    if (b->precision == RHY_MIXED) {
+      std::vector<float> tmp(n);
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
+      e = hipMemcpyAsync(static_cast<float*>(b->arena) + off, tmp.data(), sizeof(float) * n,
+                         hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    } else {
+      e = hipMemcpyAsync(static_cast<double*>(b->arena) + off, cells, sizeof(double) * n,
+                         hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    }
+  } else if (ic) {
+    if (b->ndim == 1) launch_paint<1>(b, G, *ic, off);
+    else if (b->ndim == 2) launch_paint<2>(b, G, *ic, off);
+    else launch_paint<3>(b, G, *ic, off);
+    e = hipGetLastError();
+  } else {
+    e = hipMemsetAsync(static_cast<char*>(b->arena) + off * b->esz, 0,
+                       b->esz * (size_t)G.plane * b->ncmp, b->stream);
+  }
+  if (e != hipSuccess) {
+    arena_free(b, off, len);
+    return bfail(RHY_ERR_HIP, std::string("writing the box state: ") + hipGetErrorString(e));
+  }
   Slot& S = b->slots[slot];
   S.active = true;
   S.off = off;
@@ -692,28 +722,6 @@ int add_box(rhy_batch* b, const int32_t* dims, const double* dx, const double* c
   for (int f = 0; f < 6; ++f) S.B.bc.t[f] = bc_types ? bc_types[f] : 1;
   S.needs_bc = true;
   b->dirty = true;
-  if (cells) {
-    const size_t n = (size_t)G.plane * b->ncmp;
-    if (b->precision == RHY_MIXED) {
-      std::vector<float> tmp(n);
-      for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
-      BHIP_TRY(hipMemcpyAsync(static_cast<float*>(b->arena) + off, tmp.data(), sizeof(float) * n,
-                              hipMemcpyHostToDevice, b->stream));
-      BHIP_TRY(hipStreamSynchronize(b->stream));
-    } else {
-      BHIP_TRY(hipMemcpyAsync(static_cast<double*>(b->arena) + off, cells, sizeof(double) * n,
-                              hipMemcpyHostToDevice, b->stream));
-      BHIP_TRY(hipStreamSynchronize(b->stream));
-    }
-  } else if (ic) {
-    if (b->ndim == 1) launch_paint<1>(b, G, *ic, off);
-    else if (b->ndim == 2) launch_paint<2>(b, G, *ic, off);
-    else launch_paint<3>(b, G, *ic, off);
-    BHIP_TRY(hipGetLastError());
-  } else {
-    BHIP_TRY(hipMemsetAsync(static_cast<char*>(b->arena) + off * b->esz, 0,
-                            b->esz * (size_t)G.plane * b->ncmp, b->stream));
-  }
   *slot_out = slot;
   return RHY_OK;
 }
