@@ -131,7 +131,10 @@ def run_alone(job, params, precision=FP64):
 
 
 class HydroServer:
-    """Continuous batching of 2D and 3D jobs under one HBM budget."""
+    """Continuous batching of 2D and 3D jobs under one HBM budget.
+
+    keep_results: True downloads every finished job's state into job.result,
+    False none, a set of job ids only those jobs'."""
 
     def __init__(self, budget_bytes, precision=FP64, quantum=10, params=None, max_boxes=4096,
                  tiles=None, keep_results=True, record_dts=False, check_every=0,
@@ -203,7 +206,7 @@ class HydroServer:
         pool = self.pools[job.ndim]
         if self.diagnose and job.diag is None:
             job.diag = pool.diagnostics(job.slot)
-        if self.keep_results:
+        if self.keep_results is True or (self.keep_results and job.jid in self.keep_results):
             job.result = pool.download(job.slot)
         pool.remove(job.slot)
         del self.resident[job.ndim][job.slot]
