@@ -17,7 +17,9 @@ two stream synchronisations; zone-updates/s = 256^3 * K / elapsed.
 Also reported:
   roofline      algorithmic HBM bytes of the step (2 * 5 * 8 = 80 B per
                 zone-update: one read and one write of the FP64 state) over
-                the measured time of one step's kernels, against 8 TB/s.
+                the measured time of one step's kernels, against 8 TB/s;
+                `traffic` is what the step's kernels actually move per step,
+                from calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes.
   cpu_baseline  the CPU oracle (a restatement of the reference Fortran path,
                 both loop structures) on the host cores, on a bounded sample
                 of the same workload.
@@ -33,6 +35,20 @@ sys.path.insert(0, ROOT)
 
 METRIC = "zone-updates/s at 3D 256^3 (FP64 vs mixed), % HBM roofline; serve p99 latency"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes that one step's kernels move per zone-update at 3D 256^3, from
+# the two rocprofv3 --pmc passes of this bench (FETCH_SIZE, then WRITE_SIZE)
+# summarised in profiles/blast256_fp64_mixed_kernels.md, each scaled by what
+# the counter reads for kernels of known traffic run in the same passes
+# (--copy-probe): FETCH_SIZE reads 0.500 KiB per KiB a 1 GiB scaled copy
+# reads, WRITE_SIZE 1.000 per KiB written.
+PMC_TRAFFIC_PROFILE = "profiles/blast256_fp64_mixed_kernels.md"
+PMC_TRAFFIC_B_PER_ZONE = {"fp64": 624.3, "mixed": 336.5}
+PMC_TRAFFIC_N = 256
+# The flux kernels are bound by VALU issue, not HBM: issue cycles (4 per
+# wave64 VALU instruction, 16 per transcendental) over the SIMD cycles of
+# their run at 2.4 GHz, mean of the x, y, z launches, from the SQ counter
+# passes in the same profile.
+PMC_VALU_ISSUE = {"fp64": 0.76, "mixed": 0.73}
 
 
 def blast_params(hydro, solver):
@@ -200,8 +216,9 @@ def main():
     ap.add_argument("--serve-saturated-rate", type=float, default=1000.0,
                     help="second pass with arrivals faster than service (capacity); 0: skip")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="first run one plain device copy of 1 GiB (torch), the "
-                         "known-size kernel that scales rocprof FETCH/WRITE_SIZE")
+                    help="first run two torch kernels of known traffic, a 1 GiB "
+                         "fill (writes 1 GiB) and a 1 GiB scaled copy (reads 1 GiB, "
+                         "writes 1 GiB), that scale rocprof FETCH/WRITE_SIZE")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -213,7 +230,7 @@ def main():
         src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         torch.cuda.synchronize()
-        dst.copy_(src)
+        torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
     g = gpu_leg(a.n, a.steps, a.warmup, a.seed)
@@ -226,8 +243,13 @@ def main():
     flux_ms = st[0][1] / max(st[0][0], 1)
     bytes_alg = 80.0 * zones
     achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
+    traffic = (PMC_TRAFFIC_B_PER_ZONE["fp64"] * zones if a.n == PMC_TRAFFIC_N else None)
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=achieved / HBM_PEAK_GBS, traffic=None,
+                    frac=achieved / HBM_PEAK_GBS, traffic=traffic,
+                    traffic_unit="HBM bytes per step (FETCH_SIZE, WRITE_SIZE, calibrated)",
+                    traffic_source=PMC_TRAFFIC_PROFILE if traffic else None,
+                    flux_kernel_bound="valu issue",
+                    flux_kernel_valu_issue_frac=PMC_VALU_ISSUE["fp64"],
                     kernel="one step: dt_kernel + 3 x flux_kernel + update_kernel",
                     step_kernel_ms=kern_ms, flux_kernel_avg_ms=flux_ms,
                     update_kernel_avg_ms=st[1][1] / max(st[1][0], 1),
@@ -258,7 +280,10 @@ def main():
             update_kernel_avg_ms=m["stats"][1][1] / max(m["stats"][1][0], 1),
             roofline_achieved_gbs=40.0 * zones / (sum(m["stats"][k][1] for k in (0, 1, 4))
                                                    / max(a.steps, 1) * 1e-3) / 1e9,
-            bytes_per_zone_update=40, state_finite=m["finite"])
+            bytes_per_zone_update=40, state_finite=m["finite"],
+            traffic=(PMC_TRAFFIC_B_PER_ZONE["mixed"] * zones if a.n == PMC_TRAFFIC_N
+                     else None),
+            flux_kernel_valu_issue_frac=PMC_VALU_ISSUE["mixed"])
     if not a.skip_batched:
         line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
     if not a.skip_serve:

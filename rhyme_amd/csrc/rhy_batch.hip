@@ -232,9 +232,7 @@ __global__ __launch_bounds__(256) void batch_dt_kernel(const BBox* __restrict__ 
   __shared__ double wave_max[4];
   const int b = active[blockIdx.x];
   const BBox& B = boxes[b];
-  double v = 0.0;
-  for (int i = B.part_begin + threadIdx.x; i < B.part_end; i += blockDim.x)
-    v = partial[i] > v ? partial[i] : v;
+  double v = strided_max(partial, B.part_begin, B.part_end, (int)threadIdx.x, (int)blockDim.x);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);

@@ -461,6 +461,26 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const R* __r
   block_max_to_partial(v, partial);
 }
 
+// strided_max: the max of p[begin, end) over the threads of a block, thread
+// tid taking every stride-th value from begin + tid, from 0 with the strict
+// `>` update (NaN never replaces the running value).  Eight loads are in
+// flight before they are combined, so a thread does not wait one memory
+// latency per value.
+__device__ __forceinline__ double strided_max(const double* __restrict__ p, int64_t begin,
+                                              int64_t end, int tid, int stride) {
+  double v = 0.0;
+  int64_t i = begin + tid;
+  for (; i + 7 * (int64_t)stride < end; i += 8 * (int64_t)stride) {
+    double x[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) x[u] = p[i + u * (int64_t)stride];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v = x[u] > v ? x[u] : v;
+  }
+  for (; i < end; i += stride) v = p[i] > v ? p[i] : v;
+  return v;
+}
+
 // Final max over the per-block partials (one block of 1024 threads), then
 // dt = courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the product
 // courant*minval(dx) is formed on the host.  Writes dt (and logs it) and the
@@ -469,8 +489,7 @@ __global__ __launch_bounds__(256) void cfl_kernel(Consts k, Geom G, const R* __r
     const double* __restrict__ partial, int32_t n, double cmdx, double* dt_out, double* dt_log,
     double* maxu_out) {
   __shared__ double wave_max[16];
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) v = partial[i] > v ? partial[i] : v;
+  double v = strided_max(partial, 0, n, (int)threadIdx.x, (int)blockDim.x);
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double o = __shfl_xor(v, off, 64);
