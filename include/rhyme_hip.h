@@ -151,11 +151,28 @@ int rhy_box_last_dts(rhy_box_t *box, int32_t n, double *dts);
 
 /* Per-kernel timing with HIP events recorded on the box's stream around every
  * launch while enabled.  rhy_box_kernel_stats returns, for kernel index
- * kidx (0: the fused/flux step kernel, 1: update, 2: bc, 3: cfl, 4: dt), the
- * number of launches timed and their summed milliseconds. */
+ * kidx (0: the flux kernels, 1: update, 2: bc, 3: cfl, 4: dt, 5: the
+ * deferred-face list reset and fixup), the number of launches timed and their
+ * summed milliseconds. */
 int rhy_box_set_profiling(rhy_box_t *box, int32_t on);
 int rhy_box_kernel_stats(rhy_box_t *box, int32_t kidx, int64_t *launches,
                          double *total_ms);
+
+/* ---- deferred faces ----
+ * No counterpart in the reference (its Riemann solve runs the slow paths
+ * inline: libm pow in rhyme_irs_nonlinear_wave_function.f90:20-22 and the
+ * restarts from guesses 2-6 in rhyme_irs_exec.f90:24-42).  The flux kernels
+ * leave faces that need those paths to a fixup kernel, which recomputes them
+ * with the full solver in the same step; results are those of the full
+ * solver bit for bit.
+ * rhy_box_last_deferred (rhy_batch_last_deferred for batches and pools):
+ * faces listed by the last flux phase (waits for the stream).  rhy_debug_defer is a process-wide test hook
+ * for callers that check the fixup path: every > 0 also lists every face
+ * whose low-cell offset is a multiple of `every`, cap >= 0 caps the list (the
+ * fixup then recomputes every face of the step); rhy_debug_defer(0, -1)
+ * restores normal operation.  Not thread-safe with concurrent runs. */
+int rhy_box_last_deferred(rhy_box_t *box, int64_t *n);
+int rhy_debug_defer(int32_t every, int64_t cap);
 
 /* ---- diagnostics (rhyme_sanity_check_perform, rhyme_sanity_check/src/
  * rhyme_sanity_check_perform.f90:34-94, and the totals of rhyme_samr_calc_
@@ -256,6 +273,8 @@ int rhy_batch_last_dts(rhy_batch_t *batch, int32_t n, double *dts);
 const char *rhy_batch_last_error(void);
 /* rhy_box_diagnostics of one box (or pool slot) of a batch */
 int rhy_batch_diagnostics(rhy_batch_t *batch, int32_t box, rhy_diag_t *out);
+/* faces the last flux phase left to the fixup kernel (see rhy_box_last_deferred) */
+int rhy_batch_last_deferred(rhy_batch_t *batch, int64_t *n);
 
 /* ---- pool: a batch whose boxes come and go (continuous batching) ----
  * No counterpart in the reference.  A pool holds a device arena of

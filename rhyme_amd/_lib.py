@@ -34,6 +34,7 @@ EXPORTS = [
     "rhy_batch_last_dts", "rhy_batch_last_error", "rhy_batch_diagnostics",
     "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_add_ic",
     "rhy_pool_remove", "rhy_pool_free_bytes",
+    "rhy_box_last_deferred", "rhy_batch_last_deferred", "rhy_debug_defer",
 ]
 
 
@@ -157,6 +158,8 @@ def load():
         "rhy_flux_batch": (C.c_int, [pp, C.c_int64, dp, ip, dp]),
         "rhy_box_paint": (C.c_int, [vp, C.POINTER(RhyIC)]),
         "rhy_box_diagnostics": (C.c_int, [vp, C.POINTER(RhyDiag)]),
+        "rhy_box_last_deferred": (C.c_int, [vp, C.POINTER(C.c_int64)]),
+        "rhy_debug_defer": (C.c_int, [C.c_int32, C.c_int64]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -164,6 +167,24 @@ def load():
         f.argtypes = args
     _lib = L
     return L
+
+
+class debug_defer:
+    """Test hook (rhy_debug_defer) as a context manager: inside, the flux
+    kernels also leave every `every`-th face (by low-cell offset) to the
+    fixup kernel, and the deferred-face list holds at most `cap` entries
+    (cap >= 0; past it the fixup recomputes every face)."""
+
+    def __init__(self, every, cap=-1):
+        self.every, self.cap = int(every), int(cap)
+
+    def __enter__(self):
+        check(load().rhy_debug_defer(self.every, self.cap), "rhy_debug_defer")
+        return self
+
+    def __exit__(self, *exc):
+        check(load().rhy_debug_defer(0, -1), "rhy_debug_defer")
+        return False
 
 
 def check(rc, what):

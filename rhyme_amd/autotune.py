@@ -63,13 +63,17 @@ def lookup(batch, params):
 
 def tune(batch, params, initial, steps=3, candidates=None, use_cache=True):
     """Pick the fastest tile shape for `batch` running `params`; `initial`
-    is the list of host arrays to restore each box from.  Returns
+    is the list of host arrays to restore each box from.  On return the
+    batch runs with the chosen shape and holds the initial state, whether
+    the shape was timed or taken from the cache.  Returns
     (best_shape, {shape: ms_per_step})."""
     sig = signature(batch, params)
     if use_cache:
         hit = _load().get(sig)
         if hit:
             batch.set_tiles(*hit["tiles"])
+            for i, c in enumerate(initial):
+                batch.upload(i, c)
             return tuple(hit["tiles"]), hit.get("timings", {})
     timings = {}
     for cand in candidates or CANDIDATES:

@@ -29,6 +29,7 @@ _BATCH_SIGS = {
     "rhy_batch_last_dts": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     "rhy_batch_last_error": (C.c_char_p, []),
     "rhy_batch_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_lib.RhyDiag)]),
+    "rhy_batch_last_deferred": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "rhy_pool_box_bytes": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
@@ -143,6 +144,12 @@ class SamrBatch:
         d = _lib.RhyDiag()
         _check(_L().rhy_batch_diagnostics(self._h, i, C.byref(d)), "rhy_batch_diagnostics")
         return d.as_dict(self.ncmp, self.ndim)
+
+    def last_deferred(self):
+        """Faces the last flux phase left to the fixup kernel."""
+        n = C.c_int64(0)
+        _check(_L().rhy_batch_last_deferred(self._h, C.byref(n)), "rhy_batch_last_deferred")
+        return n.value
 
 
 def box_bytes(ndim, dims, ncmp=None, precision=FP64):
@@ -280,3 +287,9 @@ class SamrPool:
         d = _lib.RhyDiag()
         _check(_L().rhy_batch_diagnostics(self._h, slot, C.byref(d)), "rhy_batch_diagnostics")
         return d.as_dict(self.ncmp, self.ndim)
+
+    def last_deferred(self):
+        """Faces the last flux phase left to the fixup kernel."""
+        n = C.c_int64(0)
+        _check(_L().rhy_batch_last_deferred(self._h, C.byref(n)), "rhy_batch_last_deferred")
+        return n.value

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -37,7 +38,9 @@ int g_init_rc = RHY_ERR_NODEV;
 int g_device = -1;
 
 constexpr int kThreads = 256;
-constexpr int kNumKernelIds = 5;  // flux/step, update, bc, cfl, dt
+// workgroups of the fixup kernel (grid-stride over the deferred faces)
+constexpr int kFixupBlocks = 64;
+constexpr int kNumKernelIds = 6;  // flux/step, update, bc, cfl, dt, deferred-face fixup
 
 struct TimedLaunch {
   int kidx;
@@ -64,13 +67,25 @@ struct rhy_box {
   double* dt = nullptr;           // current dt
   double* dt_log = nullptr;       // per-step dt of the last run
   void* diag = nullptr;           // workgroup partials of rhy_box_diagnostics
+  uint32_t* defer_count = nullptr;  // faces the flux kernels left to the fixup
+  uint64_t* defer_items = nullptr;
+  uint32_t defer_cap = 0;
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
   bool profiling = false;
   std::vector<TimedLaunch> pending;
-  int64_t launches[kNumKernelIds] = {0, 0, 0, 0, 0};
-  double total_ms[kNumKernelIds] = {0, 0, 0, 0, 0};
+  int64_t launches[kNumKernelIds] = {0, 0, 0, 0, 0, 0};
+  double total_ms[kNumKernelIds] = {0, 0, 0, 0, 0, 0};
 };
+
+namespace rhy {
+namespace {
+std::atomic<int32_t> g_defer_every{0};
+std::atomic<int64_t> g_defer_cap{-1};
+}  // namespace
+int32_t defer_every_setting() { return g_defer_every.load(); }
+int64_t defer_cap_setting() { return g_defer_cap.load(); }
+}  // namespace rhy
 
 namespace {
 
@@ -112,6 +127,7 @@ Consts make_consts(const rhy_params_t* p) {
   k.e_rho = std::nextafter(p->w_vacuum_rho, INFINITY);
   k.limiter = p->limiter;
   k.n_iter = p->n_iteration;
+  k.defer_every = defer_every_setting();
   return k;
 }
 
@@ -171,8 +187,17 @@ void collect_timings(rhy_box* b) {
 constexpr int kNfX = RHY_NF_X;
 constexpr int kNfYZ = RHY_NF_YZ;
 
+DeferList defer_list(rhy_box* b) {
+  DeferList dl;
+  dl.count = b->defer_count;
+  dl.items = b->defer_items;
+  const int64_t cap = defer_cap_setting();
+  dl.cap = cap >= 0 ? (uint32_t)std::min<int64_t>(cap, b->defer_cap) : b->defer_cap;
+  return dl;
+}
+
 template <int ND, int A, int LIM, typename R, class PW>
-void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx) {
+void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx, const DeferList& dl) {
   const Geom& G = b->geom;
   const int nf = A == 0 ? kNfX : kNfYZ;
   const int64_t nthreads = flux_threads(G, A, nf);
@@ -181,33 +206,49 @@ void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx) {
   timed_begin(b, 0, tl);
   hipLaunchKernelGGL((flux_kernel<ND, A, LIM, R, PW>), dim3(grid_for(nthreads)),
                      dim3(kThreads), 0, b->stream, k, G, static_cast<const R*>(b->cells), F, b->dt,
-                     dx[A], nf);
+                     dx[A], nf, dl);
   timed_end(b, tl);
 }
 
 template <int ND, int LIM, typename R, class PW>
-void launch_flux_lim(rhy_box* b, const Consts& k, const double* dx) {
-  launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx);
-  if constexpr (ND > 1) launch_flux_axis<ND, 1, LIM, R, PW>(b, k, dx);
-  if constexpr (ND > 2) launch_flux_axis<ND, 2, LIM, R, PW>(b, k, dx);
+void launch_flux_lim(rhy_box* b, const Consts& k, const double* dx, const DeferList& dl) {
+  launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx, dl);
+  if constexpr (ND > 1) launch_flux_axis<ND, 1, LIM, R, PW>(b, k, dx, dl);
+  if constexpr (ND > 2) launch_flux_axis<ND, 2, LIM, R, PW>(b, k, dx, dl);
 }
 
-template <int ND, typename R, class PW>
+// The flux kernels run with policy PW (Deferring<P> unless built with libm
+// pow) and list the faces that need P's slow paths; the fixup kernel then
+// recomputes those with P.  The list is emptied before the first axis.
+template <int ND, typename R, class PW, class P>
 void launch_flux_r(rhy_box* b, const Consts& k, const double* dx) {
+  const DeferList dl = defer_list(b);
+  TimedLaunch tr;
+  timed_begin(b, 5, tr);
+  hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 0u);
+  timed_end(b, tr);
   switch (k.limiter) {
-    case 1: launch_flux_lim<ND, 1, R, PW>(b, k, dx); break;
-    case 2: launch_flux_lim<ND, 2, R, PW>(b, k, dx); break;
-    case 3: launch_flux_lim<ND, 3, R, PW>(b, k, dx); break;
-    default: launch_flux_lim<ND, 4, R, PW>(b, k, dx); break;
+    case 1: launch_flux_lim<ND, 1, R, PW>(b, k, dx, dl); break;
+    case 2: launch_flux_lim<ND, 2, R, PW>(b, k, dx, dl); break;
+    case 3: launch_flux_lim<ND, 3, R, PW>(b, k, dx, dl); break;
+    default: launch_flux_lim<ND, 4, R, PW>(b, k, dx, dl); break;
+  }
+  if constexpr (PW::kDefer) {
+    TimedLaunch tf;
+    timed_begin(b, 5, tf);
+    hipLaunchKernelGGL((flux_fixup_kernel<ND, R, P>), dim3(kFixupBlocks), dim3(kThreads), 0,
+                       b->stream, k, b->geom, static_cast<const R*>(b->cells),
+                       static_cast<R*>(b->flux), b->dt, dx[0], dx[1], dx[2], dl);
+    timed_end(b, tf);
   }
 }
 
 template <int ND>
 int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
   if (b->precision == RHY_MIXED)
-    launch_flux_r<ND, float, MixedMath>(b, k, dx);
+    launch_flux_r<ND, float, MixedFlux, MixedMath>(b, k, dx);
   else
-    launch_flux_r<ND, double, PowF64>(b, k, dx);
+    launch_flux_r<ND, double, PowF64Flux, PowF64>(b, k, dx);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
 }
@@ -350,9 +391,14 @@ void free_box_memory(rhy_box* b) {
   if (b->cells) (void)hipFree(b->cells);
   if (b->flux) (void)hipFree(b->flux);
   if (b->partial) (void)hipFree(b->partial);
+  if (b->defer_count) (void)hipFree(b->defer_count);
+  if (b->defer_items) (void)hipFree(b->defer_items);
   b->cells = nullptr;
   b->flux = nullptr;
   b->partial = nullptr;
+  b->defer_count = nullptr;
+  b->defer_items = nullptr;
+  b->defer_cap = 0;
 }
 
 int alloc_box_memory(rhy_box* b) {
@@ -374,6 +420,17 @@ int alloc_box_memory(rhy_box* b) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "reduction scratch allocation failed");
   }
+  // deferred-face list: an eighth of the faces of a step, within
+  // [4096, 2^22] entries (the fixup recomputes every face past that)
+  int64_t faces = 0;
+  for (int a = 0; a < b->ndim; ++a) faces += ncell / b->geom.n[a] * (b->geom.n[a] + 1);
+  b->defer_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(faces / 8, 4096), 1 << 22);
+  if (hipMalloc(&b->defer_count, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->defer_items, sizeof(uint64_t) * (size_t)b->defer_cap) != hipSuccess) {
+    free_box_memory(b);
+    return fail(RHY_ERR_OOM, "deferred-face list allocation failed");
+  }
+  HIP_TRY(hipMemsetAsync(b->defer_count, 0, sizeof(uint32_t), b->stream));
   // ghosts start at 0 (rhyme_samr_bc_init.f90:61-87)
   HIP_TRY(hipMemsetAsync(b->cells, 0, b->esz * (size_t)P * b->ncmp, b->stream));
   return RHY_OK;
@@ -544,6 +601,24 @@ int rhy_bc_set_boundaries(rhy_box_t* b, const rhy_params_t* p) {
   else rc = launch_bc<3>(b, p, -1);
   if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_debug_defer(int32_t every, int64_t cap) {
+  if (every < 0) return fail(RHY_ERR_ARG, "every must be >= 0");
+  g_defer_every.store(every);
+  g_defer_cap.store(cap < 0 ? -1 : cap);
+  return RHY_OK;
+}
+
+int rhy_box_last_deferred(rhy_box_t* b, int64_t* n) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!n) return fail(RHY_ERR_ARG, "n is NULL");
+  uint32_t c = 0;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  HIP_TRY(hipMemcpy(&c, b->defer_count, sizeof(c), hipMemcpyDeviceToHost));
+  *n = c;
   return RHY_OK;
 }
 

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Cons
                                                          int32_t nseg, int32_t ntiles,
                                                          R* __restrict__ arena,
                                                          const double* __restrict__ dt,
-                                                         int32_t nf_x, int32_t nf_yz) {
+                                                         int32_t nf_x, int32_t nf_yz,
+                                                         DeferList dl) {
   for (int32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
     const BTile T = tile_of(segs, nseg, ti);
     if ((int32_t)threadIdx.x >= T.count) continue;
@@ -119,11 +120,51 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Cons
     const R* c = arena + B.cell_off;
     R* F = arena + B.flux_off + (int64_t)T.kind * (ND + 2) * B.G.plane;
     const double d = dt[T.box];
-    if (T.kind == 0) flux_walk<ND, 0, LIM, R, PW>(kb, B.G, c, F, d, B.dx[0], nf_x, t);
+    const uint32_t bx = (uint32_t)T.box;
+    if (T.kind == 0) flux_walk<ND, 0, LIM, R, PW>(kb, B.G, c, F, d, B.dx[0], nf_x, t, dl, bx);
     if constexpr (ND > 1)
-      if (T.kind == 1) flux_walk<ND, 1, LIM, R, PW>(kb, B.G, c, F, d, B.dx[1], nf_yz, t);
+      if (T.kind == 1) flux_walk<ND, 1, LIM, R, PW>(kb, B.G, c, F, d, B.dx[1], nf_yz, t, dl, bx);
     if constexpr (ND > 2)
-      if (T.kind == 2) flux_walk<ND, 2, LIM, R, PW>(kb, B.G, c, F, d, B.dx[2], nf_yz, t);
+      if (T.kind == 2) flux_walk<ND, 2, LIM, R, PW>(kb, B.G, c, F, d, B.dx[2], nf_yz, t, dl, bx);
+  }
+}
+
+// The faces the flux phase listed, recomputed with the full policy PW
+// (each with its box's limiter, dt and dx); if the list overflowed, every
+// face of every active box.
+template <int ND, typename R, class PW>
+__global__ __launch_bounds__(256) void batch_fixup_kernel(Consts k, const BBox* __restrict__ boxes,
+                                                          const int32_t* __restrict__ active,
+                                                          int32_t nactive, R* __restrict__ arena,
+                                                          const double* __restrict__ dt,
+                                                          DeferList dl) {
+  const uint32_t n = *dl.count;
+  if (n == 0) return;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n <= dl.cap) {
+    for (int64_t i = t0; i < (int64_t)n; i += stride) {
+      const uint64_t it = dl.items[i];
+      const int32_t bx = (int32_t)(it >> 42);
+      const BBox& B = boxes[bx];
+      Consts kb = k;
+      kb.limiter = B.limiter;
+      flux_face_fix<ND, R, PW>(kb, B.G, arena + B.cell_off, arena + B.flux_off, dt[bx], B.dx,
+                               (int)((it >> 40) & 3), (int64_t)(it & ((1ull << 40) - 1)));
+    }
+    return;
+  }
+  for (int32_t ai = 0; ai < nactive; ++ai) {
+    const int32_t bx = active[ai];
+    const BBox& B = boxes[bx];
+    Consts kb = k;
+    kb.limiter = B.limiter;
+    for (int a = 0; a < ND; ++a) {
+      const int64_t na = faces_of_axis(B.G, a);
+      for (int64_t q = t0; q < na; q += stride)
+        flux_face_fix<ND, R, PW>(kb, B.G, arena + B.cell_off, arena + B.flux_off, dt[bx], B.dx,
+                                 a, face_low_cell(B.G, a, q));
+    }
   }
 }
 
@@ -261,6 +302,7 @@ __global__ void batch_set_counter_kernel(int32_t* step, int32_t v) {
 unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) boundaries
+constexpr int kFixupBlocks = 64;  // workgroups of the fixup kernel
 
 struct Slot {
   bool active = false;
@@ -302,6 +344,9 @@ struct rhy_batch {
   hipGraphExec_t graph = nullptr;
   uint64_t graph_key = 0;
   DiagPartial* diag = nullptr;  // workgroup partials of rhy_batch_diagnostics
+  uint32_t* defer_count = nullptr;  // faces the flux phase left to the fixup
+  uint64_t* defer_items = nullptr;
+  uint32_t defer_cap = 0;
 };
 
 namespace {
@@ -324,6 +369,7 @@ Consts consts_of(const rhy_params_t* p) {
   k.e_rho = std::nextafter(p->w_vacuum_rho, INFINITY);
   k.limiter = p->limiter;
   k.n_iter = p->n_iteration;
+  k.defer_every = defer_every_setting();
   return k;
 }
 
@@ -449,14 +495,27 @@ unsigned persistent_grid(rhy_batch* b, int32_t ntiles) {
   return std::max(1u, std::min(cap, (unsigned)ntiles));
 }
 
-template <int ND, typename R, class PW>
+DeferList defer_list(rhy_batch* b) {
+  DeferList dl;
+  dl.count = b->defer_count;
+  dl.items = b->defer_items;
+  const int64_t cap = defer_cap_setting();
+  dl.cap = cap >= 0 ? (uint32_t)std::min<int64_t>(cap, b->defer_cap) : b->defer_cap;
+  return dl;
+}
+
+// Flux kernels with policy PW (Deferring<P>), then the fixup of the faces
+// they listed with P; the list is emptied first.
+template <int ND, typename R, class PW, class P>
 void launch_flux_phase(rhy_batch* b, const Consts& k, int lim) {
   const unsigned g = persistent_grid(b, b->nflux_tiles);
   R* a = static_cast<R*>(b->arena);
+  const DeferList dl = defer_list(b);
+  hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 0u);
 #define RHY_BATCH_FLUX(L)                                                                   \
   hipLaunchKernelGGL((batch_flux_kernel<ND, L, R, PW>), dim3(g), dim3(b->block), 0, b->stream, \
                      k, b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles, a, b->dt, \
-                     b->nf_x, b->nf_yz)
+                     b->nf_x, b->nf_yz, dl)
   switch (lim) {
     case 0: RHY_BATCH_FLUX(0); break;
     case 1: RHY_BATCH_FLUX(1); break;
@@ -465,6 +524,10 @@ void launch_flux_phase(rhy_batch* b, const Consts& k, int lim) {
     default: RHY_BATCH_FLUX(4); break;
   }
 #undef RHY_BATCH_FLUX
+  if constexpr (PW::kDefer)
+    hipLaunchKernelGGL((batch_fixup_kernel<ND, R, P>), dim3(kFixupBlocks), dim3(256), 0,
+                       b->stream, k, b->dboxes, b->dactive, (int32_t)b->active.size(), a, b->dt,
+                       dl);
 }
 
 template <int ND, typename R>
@@ -481,12 +544,12 @@ void launch_update_phase(rhy_batch* b, const Consts& k, int order) {
                        b->nupd_tiles, a, b->dt, b->partial);
 }
 
-template <int ND, typename R, class PW>
+template <int ND, typename R, class PW, class P>
 void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim) {
   hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0, b->stream,
                      b->dboxes, b->dactive, b->partial, b->dt, b->dtlog, b->step, b->dtlog_steps,
                      (int32_t)b->slots.size());
-  launch_flux_phase<ND, R, PW>(b, k, lim);
+  launch_flux_phase<ND, R, PW, P>(b, k, lim);
   launch_update_phase<ND, R>(b, k, order);
   hipLaunchKernelGGL(batch_step_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step);
 }
@@ -527,7 +590,7 @@ int prologue(rhy_batch* b, const Consts& k) {
   return RHY_OK;
 }
 
-template <int ND, typename R, class PW>
+template <int ND, typename R, class PW, class P>
 int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph) {
   const Consts k = consts_of(p);
   if (std::memcmp(p, &b->last_params, sizeof(*p)) != 0) {
@@ -577,7 +640,7 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   }
   if ((rc = prologue<ND, R>(b, k))) return rc;
   if (!use_graph) {
-    for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
+    for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW, P>(b, k, p->solver_type, lim);
     BHIP_TRY(hipGetLastError());
     b->last_nsteps = nsteps;
     return RHY_OK;
@@ -590,12 +653,16 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
   };
   mix(p, sizeof(*p));
   mix(&b->dtlog_steps, sizeof(b->dtlog_steps));
+  const int32_t d_every = defer_every_setting();
+  const int64_t d_cap = defer_cap_setting();
+  mix(&d_every, sizeof(d_every));
+  mix(&d_cap, sizeof(d_cap));
   if (!b->graph || b->graph_key != key) {
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     b->graph = nullptr;
     hipGraph_t g = nullptr;
     BHIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step<ND, R, PW>(b, k, p->solver_type, lim);
+    enqueue_step<ND, R, PW, P>(b, k, p->solver_type, lim);
     BHIP_TRY(hipStreamEndCapture(b->stream, &g));
     hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
@@ -643,6 +710,16 @@ rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t aren
       hipMalloc(&b->inflow, sizeof(double) * 6 * RHY_MAX_CMP) != hipSuccess) {
     rhy_batch_destroy(b);
     *rc = bfail(RHY_ERR_OOM, "batch allocation failed");
+    return nullptr;
+  }
+  // deferred-face list: about one entry per 64 arena elements, within
+  // [4096, 2^20] (the fixup recomputes every face past that)
+  b->defer_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(arena_elems / 64, 4096), 1 << 20);
+  if (hipMalloc(&b->defer_count, sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc(&b->defer_items, sizeof(uint64_t) * (size_t)b->defer_cap) != hipSuccess ||
+      hipMemsetAsync(b->defer_count, 0, sizeof(uint32_t), b->stream) != hipSuccess) {
+    rhy_batch_destroy(b);
+    *rc = bfail(RHY_ERR_OOM, "deferred-face list allocation failed");
     return nullptr;
   }
   *rc = RHY_OK;
@@ -836,7 +913,8 @@ int rhy_batch_destroy(rhy_batch_t* b) {
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
-                  (void*)b->inflow, (void*)b->diag})
+                  (void*)b->inflow, (void*)b->diag, (void*)b->defer_count,
+                  (void*)b->defer_items})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -916,20 +994,29 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
   const bool mixed = b->precision == RHY_MIXED;
   switch (b->ndim) {
     case 1:
-      return mixed ? run_t<1, float, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<1, double, PowF64>(b, p, nsteps, use_graph);
+      return mixed ? run_t<1, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
+                   : run_t<1, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
     case 2:
-      return mixed ? run_t<2, float, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<2, double, PowF64>(b, p, nsteps, use_graph);
+      return mixed ? run_t<2, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
+                   : run_t<2, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
     default:
-      return mixed ? run_t<3, float, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<3, double, PowF64>(b, p, nsteps, use_graph);
+      return mixed ? run_t<3, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
+                   : run_t<3, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
   }
 }
 
 int rhy_batch_sync(rhy_batch_t* b) {
   if (!b) return bfail(RHY_ERR_ARG, "batch is NULL");
   BHIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_batch_last_deferred(rhy_batch_t* b, int64_t* n) {
+  if (!b || !n) return bfail(RHY_ERR_ARG, "NULL argument");
+  uint32_t c = 0;
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  BHIP_TRY(hipMemcpy(&c, b->defer_count, sizeof(c), hipMemcpyDeviceToHost));
+  *n = c;
   return RHY_OK;
 }
 

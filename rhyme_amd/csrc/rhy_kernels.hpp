@@ -15,9 +15,12 @@
 
 // Minimum waves per SIMD requested for the flux kernels (caps their VGPRs:
 // 4 -> 128, 3 -> 168, 2 -> 256); 0 leaves the choice to the compiler, which
-// can exceed 256 registers with AGPRs and drop to one wave per SIMD.
+// can exceed 256 registers with AGPRs and drop to one wave per SIMD.  With
+// the slow paths of the Riemann solve deferred to the fixup kernel the flux
+// kernels need 145-167 VGPRs, so 3 waves fit (the batched 3D mixed kernel
+// spills 4 VGPRs for it and is still faster than at 2 waves).
 #ifndef RHY_FLUX_WPE
-#define RHY_FLUX_WPE 2
+#define RHY_FLUX_WPE 3
 #endif
 #if RHY_FLUX_WPE > 0
 #define RHY_FLUX_OCCUPANCY __attribute__((amdgpu_waves_per_eu(RHY_FLUX_WPE)))
@@ -53,6 +56,46 @@ __device__ __forceinline__ void load_state(const R* __restrict__ cells, int64_t 
 struct BcTypes {
   int32_t t[6];
 };
+
+// Test hook settings (rhy_debug_defer): every > 0 defers every such face,
+// cap >= 0 overrides the list capacity.
+int32_t defer_every_setting();
+int64_t defer_cap_setting();
+
+// Faces whose Riemann solve needs a slow path (Deferring<P>), listed by the
+// flux kernels for the fixup kernel: item = box << 42 | axis << 40 | the
+// offset of the face's low cell in its box's planes.  count may pass cap;
+// the fixup then recomputes every face.
+struct DeferList {
+  uint32_t* count;
+  uint64_t* items;
+  uint32_t cap;
+};
+
+__device__ __forceinline__ uint64_t defer_item(uint32_t box, int axis, int64_t of) {
+  return ((uint64_t)box << 42) | ((uint64_t)axis << 40) | (uint64_t)of;
+}
+
+// After a face's Riemann solve: list the face if the lane's flag is up
+// (or, under the test hook k.defer_every, if its offset is a multiple).
+template <class PW>
+__device__ __forceinline__ void defer_take(const Consts& k, const DeferList& dl, uint32_t box,
+                                           int axis, int64_t of) {
+  if constexpr (PW::kDefer) {
+    uint32_t& f = defer_flag();
+    if (k.defer_every > 0 && of % k.defer_every == 0) f = 1u;
+    if (f) {
+      f = 0u;
+      const uint32_t i = atomicAdd(dl.count, 1u);
+      if (i < dl.cap) dl.items[i] = defer_item(box, axis, of);
+    }
+  }
+}
+
+template <class PW>
+__device__ __forceinline__ void defer_clear() {
+  if constexpr (PW::kDefer) defer_flag() = 0u;
+}
 
 // Writes the ghost images of an updated interior cell, i.e. what
 // rhyme_samr_bc_set_<face>_boundary (rhyme_samr_bc/src/…_left_boundary.f90:47-90
@@ -171,8 +214,10 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 template <int ND, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
                                            const R* __restrict__ cells, R* __restrict__ F,
-                                           double dt, double dx, int steps, int64_t t) {
+                                           double dt, double dx, int steps, int64_t t,
+                                           const DeferList& dl, uint32_t box) {
   constexpr int NV = ND + 2;
+  defer_clear<PW>();
   const int64_t rowlen = G.n[0] + 2;
   const int64_t ncells = rowlen * G.n[1] * G.n[2];
   const int lane = (int)(t & 63);
@@ -214,6 +259,7 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
       riemann_flux<ND, 0, PW, R>(k, rp, lw, Fr);
 #pragma unroll
       for (int c = 0; c < NV; ++c) F[o - 1 + c * G.plane] = Fr[c];
+      defer_take<PW>(k, dl, box, 0, o - 1);
     }
   }
 }
@@ -221,12 +267,14 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
 template <int ND, int A, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
-                                          double dt, double dx, int nf_max, int64_t t) {
+                                          double dt, double dx, int nf_max, int64_t t,
+                                          const DeferList& dl, uint32_t box) {
   if constexpr (A == 0) {
-    flux_xrows<ND, LIM, R, PW>(k, G, cells, F, dt, dx, nf_max, t);
+    flux_xrows<ND, LIM, R, PW>(k, G, cells, F, dt, dx, nf_max, t, dl, box);
     return;
   }
   constexpr int NV = ND + 2;
+  defer_clear<PW>();
   const int nA = G.n[A];
   const int nchunk = (nA + 1 + nf_max - 1) / nf_max;
   const int e0 = A == 0 ? nchunk : G.n[0];
@@ -275,6 +323,7 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
         F[of + c * G.plane] = Fr[c];
         rp[c] = rn[c];
       }
+      defer_take<PW>(k, dl, box, A, of);
     }
   } else {
     // FP64 cells: only rp (the right state of the low cell) is carried
@@ -296,6 +345,7 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
         F[of + c * G.plane] = Fr[c];
         rp[c] = rn[c];
       }
+      defer_take<PW>(k, dl, box, A, of);
     }
   }
 }
@@ -316,9 +366,94 @@ __host__ __device__ inline int64_t flux_threads(const Geom& G, int A, int nf) {
 template <int ND, int A, int LIM, typename R, class PW>
 __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(
     Consts k, Geom G, const R* __restrict__ cells, R* __restrict__ F,
-    const double* __restrict__ dtp, double dx, int nf) {
+    const double* __restrict__ dtp, double dx, int nf, DeferList dl) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  flux_walk<ND, A, LIM, R, PW>(k, G, cells, F, *dtp, dx, nf, t);
+  flux_walk<ND, A, LIM, R, PW>(k, G, cells, F, *dtp, dx, nf, t, dl, 0u);
+}
+
+// One face's flux from scratch: the half steps of its two cells (the
+// stencil of four cells along A) and the Riemann solve with policy PW, the
+// computation the flux walks make for it.  of: offset of the low cell.
+template <int ND, int A, typename R, class PW>
+__device__ __forceinline__ void flux_one_face(const Consts& k, const Geom& G,
+                                              const R* __restrict__ cells, R* __restrict__ FA,
+                                              double dt, double dx, int64_t of) {
+  constexpr int NV = ND + 2;
+  const int64_t s = G.s[A];
+  const R hdtdx = (R)(.5 * dt / dx);
+  R um[NV], u0[NV], up[NV], l0[NV], r0[NV], l1[NV], r1[NV], Fr[NV];
+  load_state<ND, R>(cells, G.plane, of - s, um);
+  load_state<ND, R>(cells, G.plane, of, u0);
+  load_state<ND, R>(cells, G.plane, of + s, up);
+  half_step<ND, A, 0, R>(k, um, u0, up, hdtdx, l0, r0);
+  load_state<ND, R>(cells, G.plane, of + 2 * s, um);
+  half_step<ND, A, 0, R>(k, u0, up, um, hdtdx, l1, r1);
+  riemann_flux<ND, A, PW, R>(k, r0, l1, Fr);
+#pragma unroll
+  for (int c = 0; c < NV; ++c) FA[of + c * G.plane] = Fr[c];
+}
+
+// The same for a face given by axis at run time; F holds the NDIM flux
+// plane sets of the box one after the other.
+template <int ND, typename R, class PW>
+__device__ __noinline__ void flux_face_fix(const Consts& k, const Geom& G,
+                                           const R* __restrict__ cells, R* __restrict__ F,
+                                           double dt, const double* dx, int a, int64_t of) {
+  R* FA = F + (int64_t)a * (ND + 2) * G.plane;
+  if (a == 0) flux_one_face<ND, 0, R, PW>(k, G, cells, FA, dt, dx[0], of);
+  if constexpr (ND > 1)
+    if (a == 1) flux_one_face<ND, 1, R, PW>(k, G, cells, FA, dt, dx[1], of);
+  if constexpr (ND > 2)
+    if (a == 2) flux_one_face<ND, 2, R, PW>(k, G, cells, FA, dt, dx[2], of);
+}
+
+// Number of faces of axis a that a step computes, and the low cell of the
+// q-th of them (faces 0..n_a along a, interior along the other axes).
+__device__ __forceinline__ int64_t faces_of_axis(const Geom& G, int a) {
+  int64_t n = (int64_t)G.n[a] + 1;
+  for (int d = 0; d < 3; ++d)
+    if (d != a) n *= G.n[d];
+  return n;
+}
+__device__ __forceinline__ int64_t face_low_cell(const Geom& G, int a, int64_t q) {
+  int idx[3];
+  for (int d = 0; d < 3; ++d) {
+    const int64_t e = d == a ? (int64_t)G.n[d] + 1 : G.n[d];
+    idx[d] = (int)(q % e) + (d == a ? 0 : 1);
+    q /= e;
+  }
+  return goff(G, idx[0], idx[1], idx[2]);
+}
+
+// Recomputes the faces the flux kernels of one box listed, with the full
+// policy PW (grid-stride over the list); if the list overflowed, every face
+// of the box.  Runs after the last flux kernel of the step.
+template <int ND, typename R, class PW>
+__global__ __launch_bounds__(256) void flux_fixup_kernel(Consts k, Geom G,
+                                                         const R* __restrict__ cells,
+                                                         R* __restrict__ F,
+                                                         const double* __restrict__ dtp,
+                                                         double dx0, double dx1, double dx2,
+                                                         DeferList dl) {
+  const uint32_t n = *dl.count;
+  if (n == 0) return;
+  const double dt = *dtp;
+  const double dx[3] = {dx0, dx1, dx2};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n <= dl.cap) {
+    for (int64_t i = t0; i < (int64_t)n; i += stride) {
+      const uint64_t it = dl.items[i];
+      flux_face_fix<ND, R, PW>(k, G, cells, F, dt, dx, (int)((it >> 40) & 3),
+                               (int64_t)(it & ((1ull << 40) - 1)));
+    }
+    return;
+  }
+  for (int a = 0; a < ND; ++a) {
+    const int64_t na = faces_of_axis(G, a);
+    for (int64_t q = t0; q < na; q += stride)
+      flux_face_fix<ND, R, PW>(k, G, cells, F, dt, dx, a, face_low_cell(G, a, q));
+  }
 }
 
 // Conservative update of every interior cell from the stored fluxes.
@@ -510,6 +645,10 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
 }
 
 [[maybe_unused]] static __global__ void set_scalar_kernel(double* dst, double v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+[[maybe_unused]] static __global__ void set_u32_kernel(uint32_t* dst, uint32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
 }
 

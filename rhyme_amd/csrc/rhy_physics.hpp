@@ -32,6 +32,7 @@ struct Consts {
   double e_rho;          // nearest(w_rho, +1), computed on the host
   int32_t limiter;       // slid
   int32_t n_iter;        // irs%n_iteration
+  int32_t defer_every;   // > 0: test hook, defer every such face (by offset)
 };
 
 // tiny() of the arithmetic type (tiny(0d0) / tiny(0e0))
@@ -191,6 +192,7 @@ __device__ __forceinline__ void half_step(const Consts& k, const R* um, const R*
 // IEEE division and sqrt.
 struct PowLib {
   static constexpr bool kFast = false;
+  static constexpr bool kDefer = false;
   __device__ __forceinline__ static double dv(double a, double b) { return a / b; }
   __device__ __forceinline__ static double sq(double x) { return sqrt(x); }
   __device__ __forceinline__ static double pw(double x, double y) { return pow(x, y); }
@@ -209,6 +211,7 @@ struct PowLib {
 // results as two pw calls.
 struct PowFast {
   static constexpr bool kFast = false;
+  static constexpr bool kDefer = false;
   __device__ __forceinline__ static double dv(double a, double b) { return a / b; }
   __device__ __forceinline__ static double sq(double x) { return sqrt(x); }
   static constexpr double LN2_HI = 6.93147180369123816490e-01;
@@ -323,6 +326,66 @@ using PowF64 = PowFast;
 #else
 using PowF64 = PowLib;
 #endif
+
+// Deferred fallbacks.  The slow paths of the Riemann solve -- libm pow for
+// x <= 0, non-finite x or |y log x| >= 700, and the Newton restarts from
+// guesses 2-6 -- are rare, but compiled into a kernel they hold ~90 VGPRs
+// and cut it from 3 to 2 waves per SIMD.  The flux kernels therefore use
+// Deferring<P>: where P would take a slow path it raises the lane's defer
+// flag and returns a placeholder, and the kernel lists the face; a fixup
+// kernel then recomputes every listed face with P itself.  A face's flux
+// depends only on the cells, so the fixed-up flux is the one P gives, bit
+// for bit, and on every other face Deferring<P> computes what P computes.
+//
+// The flag is one LDS word per lane (a lane only touches its own word), so
+// policies need no state; kernels using it have at most 256 threads per
+// workgroup and clear the flag when they start.
+__device__ __forceinline__ uint32_t& defer_flag() {
+  __shared__ uint32_t flags[256];
+  return flags[threadIdx.x];
+}
+__device__ __forceinline__ void defer_mark() { defer_flag() = 1u; }
+
+template <class P>
+struct Deferring : P {
+  static constexpr bool kDefer = true;
+  __device__ __forceinline__ static double pw(double x, double y) {
+    if (!(x > 0.0) || !(x < INFINITY)) {
+      defer_mark();
+      return 1.0;
+    }
+    const double t = y * P::ln(x);
+    if (!(fabs(t) < 700.0)) {
+      defer_mark();
+      return 1.0;
+    }
+    return P::ex(t);
+  }
+  __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
+                                             double& xb) {
+    xa = xb = 1.0;
+    if (!(x > 0.0) || !(x < INFINITY)) {
+      defer_mark();
+      return;
+    }
+    const double l = P::ln(x);
+    const double ta = a * l, tb = b * l;
+    if (!(fabs(ta) < 700.0) || !(fabs(tb) < 700.0)) {
+      defer_mark();
+      return;
+    }
+    xa = P::ex(ta);
+    xb = P::ex(tb);
+  }
+};
+
+// Policies of the flux kernels (their fixup kernels use PowF64 / MixedMath).
+#if RHY_FP64_POW_FAST
+using PowF64Flux = Deferring<PowFast>;
+#else
+using PowF64Flux = PowLib;
+#endif
+using MixedFlux = Deferring<MixedMath>;
 
 // rhyme_irs_nonlinear_wave_function.f90:13-23.  The terms that do not
 // depend on p (A_K, B_K of the shock branch, 2 c_K / (gamma-1) and
@@ -465,9 +528,13 @@ __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv
   const WaveSide R = wave_side<PW>(k, rr, rp, rc);
   double p = newton<PW>(k, guess1<PW>(k, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
   if (!(p > 0.0)) {
-    for (int gi = 1; gi < 6; ++gi) {
-      p = newton<PW>(k, guess_n<PW>(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
-      if (p > 0.0) break;
+    if constexpr (PW::kDefer) {
+      defer_mark();  // guesses 2-6: the fixup kernel's work
+    } else {
+      for (int gi = 1; gi < 6; ++gi) {
+        p = newton<PW>(k, guess_n<PW>(k, gi, lr, lv, lp, lc, rr, rv, rp, rc), L, R, du, lf, rf);
+        if (p > 0.0) break;
+      }
     }
   }
   if (p < 0.0) p = 0.0;

@@ -212,6 +212,13 @@ class SamrBox:
         check(_lib.load().rhy_box_diagnostics(self._h, C.byref(d)), "rhy_box_diagnostics")
         return d.as_dict(self.ncmp, self.ndim)
 
+    def last_deferred(self):
+        """Faces the last flux phase left to the fixup kernel (the slow
+        paths of the Riemann solve, recomputed in the same step)."""
+        n = C.c_int64(0)
+        check(_lib.load().rhy_box_last_deferred(self._h, C.byref(n)), "rhy_box_last_deferred")
+        return n.value
+
     def paint(self, ic):
         """Paint the device state from `ic` (a dict of rhy_ic_t fields); the
         host mirror is not touched (download() to see it)."""

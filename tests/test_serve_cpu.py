@@ -203,3 +203,43 @@ def test_unhealthy_rules():
     assert "NaN" in serve.HydroServer.unhealthy(dict(ok, nan_count=[0, 0, 3] + [0] * 6))
     assert "density" in serve.HydroServer.unhealthy(dict(ok, neg_rho=1))
     assert "energy" in serve.HydroServer.unhealthy(dict(ok, neg_e_tot=2))
+
+
+class _FakeBatch:
+    """Records what autotune.tune does to a batch (no device)."""
+
+    def __init__(self, dims):
+        self.ndim, self.precision, self.dims = 3, 0, dims
+        self.uploads, self.tiles, self.runs = {}, None, 0
+
+    def set_tiles(self, *t):
+        self.tiles = tuple(t)
+
+    def upload(self, i, c):
+        self.uploads[i] = c
+
+    def run(self, params, n, graph=True):
+        self.runs += n
+        self.uploads.clear()          # running changes the state
+
+    def run_async(self, params, n, graph=True):
+        self.run(params, n)
+
+    def sync(self):
+        pass
+
+
+def test_autotune_leaves_initial_state_timed_and_cached(tmp_path, monkeypatch):
+    from rhyme_amd import autotune
+    monkeypatch.setenv("RHYME_AMD_TILE_CACHE", str(tmp_path / "tiles.json"))
+    p = hydro.make_params(3)
+    init = [np.full(3, float(i)) for i in range(4)]
+    b = _FakeBatch([[8, 8, 8]] * 4)
+    best, _ = autotune.tune(b, p, init, steps=1, candidates=autotune.CANDIDATES[:2])
+    assert b.runs > 0 and b.tiles == tuple(best)
+    assert sorted(b.uploads) == [0, 1, 2, 3]
+    # same box set again: the cached shape, and still the initial state
+    b2 = _FakeBatch([[8, 8, 8]] * 4)
+    best2, _ = autotune.tune(b2, p, init, steps=1, candidates=autotune.CANDIDATES[:2])
+    assert b2.runs == 0 and tuple(best2) == tuple(best) and b2.tiles == tuple(best)
+    assert sorted(b2.uploads) == [0, 1, 2, 3]
