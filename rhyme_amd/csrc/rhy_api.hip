@@ -248,7 +248,7 @@ int launch_flux(rhy_box* b, const Consts& k, const double* dx) {
   if (b->precision == RHY_MIXED)
     launch_flux_r<ND, float, MixedFlux, MixedMath>(b, k, dx);
   else
-    launch_flux_r<ND, double, PowF64Flux, PowF64>(b, k, dx);
+    launch_flux_r<ND, double, PowF64Flux, PowF64Fix>(b, k, dx);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
 }

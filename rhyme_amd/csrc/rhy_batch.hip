@@ -995,13 +995,13 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
   switch (b->ndim) {
     case 1:
       return mixed ? run_t<1, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<1, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
+                   : run_t<1, double, PowF64Flux, PowF64Fix>(b, p, nsteps, use_graph);
     case 2:
       return mixed ? run_t<2, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<2, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
+                   : run_t<2, double, PowF64Flux, PowF64Fix>(b, p, nsteps, use_graph);
     default:
       return mixed ? run_t<3, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
-                   : run_t<3, double, PowF64Flux, PowF64>(b, p, nsteps, use_graph);
+                   : run_t<3, double, PowF64Flux, PowF64Fix>(b, p, nsteps, use_graph);
   }
 }
 
