@@ -211,6 +211,20 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
 // warm-up step at the start) and solve the face between the two.  A wave's
 // loads and flux stores are contiguous along x, and each half step is formed
 // once (plus one warm-up step per wave).
+// Lane l of the wave gets lane l-1's value (lane 0 keeps its own): one DPP
+// wave_shr:1 move per 32 bits instead of an LDS permute.
+__device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float lane_below(float x) {
+  return __builtin_bit_cast(float, wave_shr1(__builtin_bit_cast(uint32_t, x)));
+}
+__device__ __forceinline__ double lane_below(double x) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint64_t lo = wave_shr1((uint32_t)b), hi = wave_shr1((uint32_t)(b >> 32));
+  return __builtin_bit_cast(double, lo | (hi << 32));
+}
+
 template <int ND, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
                                            const R* __restrict__ cells, R* __restrict__ F,
@@ -250,7 +264,7 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
     R rp[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) {
-      const R below = __shfl_up(rn[c], 1);
+      const R below = lane_below(rn[c]);
       rp[c] = lane == 0 ? carry[c] : below;
       carry[c] = __shfl(rn[c], 63);
     }
