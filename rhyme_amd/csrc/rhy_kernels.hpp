@@ -241,17 +241,34 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
   R carry[NV];
 #pragma unroll
   for (int c = 0; c < NV; ++c) carry[c] = R(0);
+  // the lane's cell (x index ci, row j, plane kk): located by division at
+  // the warm-up step and the first step, then advanced 64 cells per step
+  int ci = 0, jj = 1, kk = 1;
 #pragma unroll 1
   for (int m = -1; m < steps; ++m) {
     const int64_t g = base + (int64_t)m * 64 + lane;
     const bool valid = g >= 0 && g < ncells;
+    if (m <= 0) {
+      if (valid) {
+        const int64_t row = g / rowlen;
+        ci = (int)(g - row * rowlen);
+        jj = (int)(row % G.n[1]) + 1;
+        kk = (int)(row / G.n[1]) + 1;
+      }
+    } else {
+      ci += 64;
+      while (ci >= rowlen) {
+        ci -= (int)rowlen;
+        if (++jj > G.n[1]) {
+          jj = 1;
+          ++kk;
+        }
+      }
+    }
     R lw[NV], rn[NV];
-    int ci = 0;
     int64_t o = 0;
     if (valid) {
-      const int64_t row = g / rowlen;
-      ci = (int)(g - row * rowlen);
-      o = goff(G, ci, (int)(row % G.n[1]) + 1, (int)(row / G.n[1]) + 1);
+      o = goff(G, ci, jj, kk);
       R um[NV], u0[NV], up[NV];
       load_state<ND, R>(cells, G.plane, o - 1, um);
       load_state<ND, R>(cells, G.plane, o, u0);
