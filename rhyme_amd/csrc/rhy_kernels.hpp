@@ -186,31 +186,6 @@ __device__ __forceinline__ void block_max_to_partial(double v, double* partial) 
   block_max_to_index(v, partial, blockIdx.x);
 }
 
-// Intercell flux on every face normal to axis A that the interior update
-// reads: face f in [0, n_A] sits between cells f and f+1, the other indices
-// run over the interior (the faces-only range the update consumes,
-// rhyme_muscl_hancock_solve_memory_intensive.f90:142-176).  Each thread
-// walks NF consecutive faces along A with a sliding window of cells: the
-// half-step states of every cell on the walk (loop A, :49-98) are formed
-// once and shared by its two faces, then each face gets its Riemann flux
-// (loop B, :100-140), stored at the low cell's offset in F (NV planes).
-//
-// R is the storage/bulk type (double: the FP64 drop-in; float: the mixed
-// form, whose half step and the conversions around the Riemann solve run in
-// FP32 while the star-state solve and the sampling run in FP64 with the
-// MixedMath policy).  LIM is the slope limiter (slid).
-// One thread's walk: nf_max consecutive faces normal to A starting at face
-// chunk `t` of the box (thread t of the axis' index space); returns false
-// if t is past the end.
-// Faces normal to x, walked by whole waves.  The cells of all x rows,
-// including the ghost cell on each side whose half steps the end faces use,
-// are numbered row after row, g = row * (n0 + 2) + c with c = 0..n0+1 the
-// cell's x index; at each of `steps` steps the 64 lanes of a wave take 64
-// consecutive cells, form their half steps, take the right state of the cell
-// below from lane l-1 (lane 0: lane 63's of the previous step, from a
-// warm-up step at the start) and solve the face between the two.  A wave's
-// loads and flux stores are contiguous along x, and each half step is formed
-// once (plus one warm-up step per wave).
 // Lane l of the wave gets lane l-1's value (lane 0 keeps its own): one DPP
 // wave_shr:1 move per 32 bits instead of an LDS permute.
 __device__ __forceinline__ uint32_t wave_shr1(uint32_t v) {
@@ -225,6 +200,31 @@ __device__ __forceinline__ double lane_below(double x) {
   return __builtin_bit_cast(double, lo | (hi << 32));
 }
 
+// Intercell flux on every face normal to axis A that the interior update
+// reads: face f in [0, n_A] sits between cells f and f+1, the other indices
+// run over the interior (the faces-only range the update consumes,
+// rhyme_muscl_hancock_solve_memory_intensive.f90:142-176).  Each thread
+// walks NF consecutive faces along A with a sliding window of cells: the
+// half-step states of every cell on the walk (loop A, :49-98) are formed
+// once and shared by its two faces, then each face gets its Riemann flux
+// (loop B, :100-140), stored at the low cell's offset in F (NV planes).
+//
+// R is the storage/bulk type (double: the FP64 drop-in; float: the mixed
+// form, whose half step and the conversions around the Riemann solve run in
+// FP32 while the star-state solve and the sampling run in FP64 with the
+// MixedMath policy).  LIM is the slope limiter (slid).  PW is the policy of
+// the Riemann solve; with Deferring<P> the faces that need P's slow paths
+// are listed in dl, tagged with `box`, for the fixup kernel.
+//
+// flux_xrows: faces normal to x, walked by whole waves.  The cells of all x rows,
+// including the ghost cell on each side whose half steps the end faces use,
+// are numbered row after row, g = row * (n0 + 2) + c with c = 0..n0+1 the
+// cell's x index; at each of `steps` steps the 64 lanes of a wave take 64
+// consecutive cells, form their half steps, take the right state of the cell
+// below from lane l-1 (lane 0: lane 63's of the previous step, from a
+// warm-up step at the start) and solve the face between the two.  A wave's
+// loads and flux stores are contiguous along x, and each half step is formed
+// once (plus one warm-up step per wave).
 template <int ND, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
                                            const R* __restrict__ cells, R* __restrict__ F,
@@ -295,6 +295,9 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
   }
 }
 
+// flux_walk: one thread's walk of nf_max consecutive faces normal to A
+// (y, z), starting at face chunk `t` of the box (thread t of the axis' index
+// space); nothing if t is past the end.  A == 0 goes to flux_xrows.
 template <int ND, int A, int LIM, typename R, class PW>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
