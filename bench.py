@@ -48,7 +48,7 @@ PMC_TRAFFIC_N = 256
 # wave64 VALU instruction, 16 per transcendental) over the SIMD cycles of
 # their run at 2.4 GHz, mean of the x, y, z launches, from the SQ counter
 # passes in the same profile.
-PMC_VALU_ISSUE = {"fp64": 0.82, "mixed": 0.91}
+PMC_VALU_ISSUE = {"fp64": 0.84, "mixed": 0.92}
 
 
 def blast_params(hydro, solver):
