@@ -121,11 +121,14 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Cons
     R* F = arena + B.flux_off + (int64_t)T.kind * (ND + 2) * B.G.plane;
     const double d = dt[T.box];
     const uint32_t bx = (uint32_t)T.box;
-    if (T.kind == 0) flux_walk<ND, 0, LIM, R, PW>(kb, B.G, c, F, d, B.dx[0], nf_x, t, dl, bx);
+    if (T.kind == 0)
+      flux_walk<ND, 0, LIM, R, PW, false>(kb, B.G, c, F, d, B.dx[0], nf_x, t, dl, bx);
     if constexpr (ND > 1)
-      if (T.kind == 1) flux_walk<ND, 1, LIM, R, PW>(kb, B.G, c, F, d, B.dx[1], nf_yz, t, dl, bx);
+      if (T.kind == 1)
+        flux_walk<ND, 1, LIM, R, PW, false>(kb, B.G, c, F, d, B.dx[1], nf_yz, t, dl, bx);
     if constexpr (ND > 2)
-      if (T.kind == 2) flux_walk<ND, 2, LIM, R, PW>(kb, B.G, c, F, d, B.dx[2], nf_yz, t, dl, bx);
+      if (T.kind == 2)
+        flux_walk<ND, 2, LIM, R, PW, false>(kb, B.G, c, F, d, B.dx[2], nf_yz, t, dl, bx);
   }
 }
 

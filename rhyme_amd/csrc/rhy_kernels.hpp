@@ -298,7 +298,10 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
 // flux_walk: one thread's walk of nf_max consecutive faces normal to A
 // (y, z), starting at face chunk `t` of the box (thread t of the axis' index
 // space); nothing if t is past the end.  A == 0 goes to flux_xrows.
-template <int ND, int A, int LIM, typename R, class PW>
+// WINDOW: FP32 cells through a sliding window (faster for single boxes);
+// otherwise re-read like FP64 cells, which keeps the batched kernel, whose
+// three walks share one register budget, within 3 waves per SIMD unspilled.
+template <int ND, int A, int LIM, typename R, class PW, bool WINDOW = true>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
                                           double dt, double dx, int nf_max, int64_t t,
@@ -332,7 +335,7 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
   load_state<ND, R>(cells, G.plane, o, u0);
   load_state<ND, R>(cells, G.plane, o + s, up);
   half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rp);
-  if constexpr (sizeof(R) == sizeof(float)) {
+  if constexpr (sizeof(R) == sizeof(float) && WINDOW) {
     // FP32 cells: a sliding window carries two cells to the next half step
     // and loads one new cell, issued before the Riemann solve so that its
     // latency hides behind it (cell f0 + m + 3 at face m, i.e. at most cell
@@ -360,10 +363,10 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
       defer_take<PW>(k, dl, box, A, of);
     }
   } else {
-    // FP64 cells: only rp (the right state of the low cell) is carried
-    // across the Riemann solve; the three cells of the next half step are
-    // re-read (cache hits), which keeps the register count near the
-    // single-face one.
+    // FP64 cells (and FP32 without the window): only rp (the right state of
+    // the low cell) is carried across the Riemann solve; the three cells of
+    // the next half step are re-read (cache hits), which keeps the register
+    // count near the single-face one.
 #pragma unroll 1
     for (int m = 0; m < nf; ++m) {
       const int64_t oc = o + (m + 1) * s;
