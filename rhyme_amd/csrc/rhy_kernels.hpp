@@ -17,8 +17,7 @@
 // 4 -> 128, 3 -> 168, 2 -> 256); 0 leaves the choice to the compiler, which
 // can exceed 256 registers with AGPRs and drop to one wave per SIMD.  With
 // the slow paths of the Riemann solve deferred to the fixup kernel the flux
-// kernels need 145-167 VGPRs, so 3 waves fit (the batched 3D mixed kernel
-// spills 4 VGPRs for it and is still faster than at 2 waves).
+// kernels need 145-167 VGPRs, so 3 waves fit without spills.
 #ifndef RHY_FLUX_WPE
 #define RHY_FLUX_WPE 3
 #endif
