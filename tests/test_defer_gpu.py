@@ -10,7 +10,7 @@ solver in the same step.
 * Strong collisions with large density contrasts make the Newton iteration
   from guess 1 overshoot to p* < 0 (rhyme_irs_exec.f90:24-42 then restarts
   from the next guess): such a box defers faces by itself and must still
-  match the oracle.
+  match the oracle, and give the same state inside a batch as alone.
 """
 import numpy as np
 import pytest
@@ -123,3 +123,31 @@ def test_natural_deferral_matches_oracle(precision):
         say("  var", c, "rel L1", e)
         assert e <= tol
     assert np.max(np.abs(dts - rdts) / rdts) <= tol
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_natural_deferral_in_batch_identical_to_alone(precision):
+    """Boxes whose faces defer by themselves, advanced in one batch: the
+    batch's fixup kernel gives every box the state and dts of the box run
+    alone (whose fixup is the single-box kernel)."""
+    boxes = [([64], [1.0 / 64], collision_1d(64)), ([48], [1.0 / 48], collision_1d(48)),
+             ([40], [1.0 / 40], ics.random_smooth(1, [40], seed=3))]
+    p = params(1, hydro.MINMOD, hydro.OUTFLOW)
+    b = SamrBatch(1, [d for d, _, _ in boxes], [x for _, x, _ in boxes], precision=precision)
+    for i, (_, _, c) in enumerate(boxes):
+        b.upload(i, c)
+    b.run(p, 1, graph=True)
+    n1 = b.last_deferred()
+    say("batch precision", precision, "faces deferred on the first step", n1)
+    assert n1 >= 2
+    for i, (_, _, c) in enumerate(boxes):
+        b.upload(i, c)
+    dts = b.run(p, 12, graph=True)
+    for i, (dims, dx, c) in enumerate(boxes):
+        box = hydro.SamrBox(1, dims, cells=c.copy(), precision=precision)
+        box.upload()
+        adts = hydro.run(p, box, dx, 12)
+        np.testing.assert_array_equal(dts[:, i], adts)
+        np.testing.assert_array_equal(b.download(i), box.download())
+        box.close()
+    b.close()
