@@ -124,6 +124,43 @@ def batched_leg(steps, warmup, seed):
     return out
 
 
+def amr_leg(n, steps, warmup, seed):
+    """Two-level hierarchy: the n^3 point blast as the coarse level, the
+    central (n/4)^3 coarse cells refined by 2x2x2 fine boxes of (n/4)^3 cells
+    each (dx / 2) that start as their coarse parents.  One coarse step is the
+    coarse CFL/fine CFL dt, the coarse step, two subcycled fine substeps with
+    ghosts prolonged from the coarse level, and restriction; zone-updates
+    count the coarse cells once and the fine cells twice.  FP64 and mixed."""
+    import numpy as np
+    from rhyme_amd import amr, hydro, ics
+    m = n // 4
+    c0 = n // 2 - m // 2 + 1
+    lo = [[c0 + m // 2 * a, c0 + m // 2 * b, c0 + m // 2 * c]
+          for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+    fdims = [[m, m, m]] * 8
+    cells = ics.blast_3d(n=n, seed=seed)
+    fines = [amr.fine_from_coarse(3, cells, l, d) for l, d in zip(lo, fdims)]
+    p = blast_params(hydro, hydro.MEMORY_INTENSIVE)
+    out = dict(workload="3D %d^3 point blast + 8 fine boxes %d^3 (ratio 2) over the central "
+                        "%d^3 coarse cells, subcycled, superbee, reflective" % (n, m, m))
+    for name, prec in (("fp64", hydro.FP64), ("mixed", hydro.MIXED)):
+        h = amr.AmrHierarchy(3, [n] * 3, [1.0 / n] * 3, lo, fdims, precision=prec)
+        h.upload(cells, fines)
+        if warmup:
+            h.run(p, warmup)
+        t0 = time.perf_counter()
+        dts = h.run(p, steps)
+        el = time.perf_counter() - t0
+        zones = h.zones_per_step()
+        dc, dfs = h.download()
+        finite = bool(np.isfinite(dc).all()) and all(bool(np.isfinite(f).all()) for f in dfs)
+        out[name] = dict(value=zones * steps / el, unit="zone-updates/s",
+                         ms_per_coarse_step=1e3 * el / steps, zones_per_coarse_step=zones,
+                         dt_first=float(dts[0]), dt_last=float(dts[-1]), state_finite=finite)
+        h.close()
+    return out
+
+
 def cpu_leg(n_sample, steps_sample, seed, threads_all):
     """The oracle on the host cores: both reference loop structures with all
     threads, plus memory_intensive on one thread (at a quarter of the edge),
@@ -217,6 +254,8 @@ def main():
     ap.add_argument("--skip-batched", action="store_true",
                     help="omit the batched AMR-boxes leg")
     ap.add_argument("--batched-steps", type=int, default=50)
+    ap.add_argument("--skip-amr", action="store_true", help="omit the two-level hierarchy leg")
+    ap.add_argument("--amr-steps", type=int, default=20)
     ap.add_argument("--skip-serve", action="store_true", help="omit the serving-stream leg")
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals, jobs/s")
@@ -297,6 +336,8 @@ def main():
             flux_kernel_valu_issue_frac=PMC_VALU_ISSUE["mixed"])
     if not a.skip_batched:
         line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
+    if not a.skip_amr:
+        line["amr"] = amr_leg(a.n, a.amr_steps, min(a.warmup, 2), a.seed)
     if not a.skip_serve:
         line["serving"] = serving_leg(a.serve_jobs, a.serve_rate, a.serve_budget_gib,
                                       a.serve_quantum, a.seed)

@@ -114,6 +114,10 @@ int rhy_box_upload(rhy_box_t *box, const double *cells);
 int rhy_box_download(rhy_box_t *box, double *cells);
 /* Wait for all work queued on the box. */
 int rhy_box_sync(rhy_box_t *box);
+/* Device copy of the whole state of src into dst (same dims, ncmp and
+ * precision).  No counterpart in the reference; the two-level hierarchy keeps
+ * the coarse state of the start of a coarse step with it. */
+int rhy_box_copy(rhy_box_t *dst, rhy_box_t *src);
 
 /* ---- the module procedures ---- */
 
@@ -275,6 +279,30 @@ const char *rhy_batch_last_error(void);
 int rhy_batch_diagnostics(rhy_batch_t *batch, int32_t box, rhy_diag_t *out);
 /* faces the last flux phase left to the fixup kernel (see rhy_box_last_deferred) */
 int rhy_batch_last_deferred(rhy_batch_t *batch, int64_t *n);
+
+/* ---- two-level hierarchy (refinement ratio 2) ----
+ * No counterpart in the reference, whose main loop advances every box of
+ * every level alone with the level's dt (src/main.f90:118-136) and has no
+ * coarse-fine coupling.  The coarse level is a box, the fine level a fixed
+ * batch whose boxes use face type 0 ("set by the hierarchy": no ghost
+ * fill).  lo[3*i+d] is the coarse cell (1-based) that holds fine cells 1
+ * and 2 of box i along d; fine extents are even, and every fine box with
+ * its 2 ghost layers lies in the coarse interior.
+ * rhy_batch_cfl_dts: the CFL dt of every box for its current state (into
+ *   dts[nboxes], no step).
+ * rhy_batch_run_dt_async: nsteps steps with the given dt for every box.
+ * rhy_amr_prolong: the fine boxes' face ghosts from their coarse parents,
+ *   piecewise constant; half = 0 takes coarse_old (the start of the coarse
+ *   step), half = 1 the mean 0.5*old + 0.5*new (its middle).
+ * rhy_amr_restrict: every coarse cell under a fine box takes the mean of
+ *   its 2^ndim children (summed with i fastest, times 2^-ndim).
+ * The prolong and restrict calls wait for both sides' streams. */
+int rhy_batch_cfl_dts(rhy_batch_t *batch, const rhy_params_t *p, double *dts);
+int rhy_batch_run_dt_async(rhy_batch_t *batch, const rhy_params_t *p, double dt,
+                           int32_t nsteps);
+int rhy_amr_prolong(rhy_box_t *coarse_new, rhy_box_t *coarse_old, rhy_batch_t *fine,
+                    const int32_t *lo, int32_t half);
+int rhy_amr_restrict(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo);
 
 /* ---- pool: a batch whose boxes come and go (continuous batching) ----
  * No counterpart in the reference.  A pool holds a device arena of

@@ -1,0 +1,127 @@
+"""CPU mirror of the two-level hierarchy (refinement ratio 2).
+
+TEST INFRASTRUCTURE ONLY, like the rest of oracle/.  The reference has no
+coarse-fine coupling: its main loop advances every box of every level alone
+with the level's dt (src/main.f90:118-136).  The hierarchy's semantics are
+therefore defined here, and the device hierarchy (rhyme_amd.amr) must
+reproduce this step bit for bit in FP64:
+
+1. dt0 = min(CFL dt of the coarse box, 2 * min over fine boxes of their CFL
+   dt), each CFL dt the reference's (rhyme_cfl_time_step.f90:3-60) with its
+   level's dx.
+2. The coarse state U_old is kept; the coarse box gets its ghost fill and one
+   MUSCL-Hancock step with dt0 (rhyme_muscl_hancock_solve with the coarse
+   faces' boundary types).
+3. Two fine substeps of dt0 / 2.  Before each, every fine box's face ghosts
+   (2 layers) take the value of the coarse cell that contains them: U_old
+   before the first substep, 0.5 * U_old + 0.5 * U_new before the second
+   (linear in time).  Fine boxes have no boundary fill of their own.
+4. Every coarse cell under a fine box takes the mean of its 2^ndim children
+   (summed with i fastest, then times 2^-ndim).  There is no refluxing, so
+   mass and energy are conserved only up to the coarse-fine flux mismatch.
+
+Placement: lo[d] is the coarse cell (1-based) that holds fine cells 1 and 2
+along d; fine extents are even; every fine box with its ghost layers lies in
+the coarse interior (2 <= lo[d], lo[d] + n_f[d] / 2 <= n_c[d]).
+"""
+import numpy as np
+
+from . import bc_set_boundaries, cfl_time_step, mh_solve
+
+
+def _axis(ndim, d):
+    """Array axis of Fortran index d (arrays are (ncmp, [z,] [y,] x))."""
+    return ndim - d
+
+
+def _fine_dims(cells, ndim):
+    return [cells.shape[_axis(ndim, d)] - 4 for d in range(ndim)]
+
+
+def check_placement(ndim, coarse_dims, fine_dims, lo):
+    for n_f, l in zip(fine_dims, lo):
+        for d in range(ndim):
+            if n_f[d] % 2:
+                raise ValueError("fine extents must be even (ratio 2)")
+            if l[d] < 2 or l[d] + n_f[d] // 2 > coarse_dims[d]:
+                raise ValueError("a fine box or its ghosts leave the coarse interior")
+    # interiors must not overlap (each coarse cell restricted from one box)
+    boxes = [[(l[d], l[d] + n_f[d] // 2 - 1) for d in range(ndim)] for n_f, l in zip(fine_dims, lo)]
+    for a in range(len(boxes)):
+        for b in range(a + 1, len(boxes)):
+            if all(boxes[a][d][0] <= boxes[b][d][1] and boxes[b][d][0] <= boxes[a][d][1]
+                   for d in range(ndim)):
+                raise ValueError("fine boxes %d and %d cover the same coarse cells" % (a, b))
+
+
+def prolong(ndim, fine, lo, cnew, cold, half):
+    """Face ghosts of one fine box from its coarse parents (see step 3)."""
+    nv = ndim + 2
+    n_f = _fine_dims(fine, ndim)
+    for a in range(ndim):
+        for side in (0, 1):
+            for layer in (0, 1):
+                g = n_f[a] + 1 + layer if side else -layer
+                fidx = [None] * ndim
+                cidx = [None] * ndim
+                for d in range(ndim):
+                    f = np.array([g]) if d == a else np.arange(1, n_f[d] + 1)
+                    par = lo[d] + (f + 1) // 2 - 1
+                    fidx[d] = f + 1        # array index of Fortran index f
+                    cidx[d] = par + 1
+                # numpy axes are reversed (x last)
+                fsel = np.ix_(*[fidx[d] for d in reversed(range(ndim))])
+                csel = np.ix_(*[cidx[d] for d in reversed(range(ndim))])
+                for c in range(nv):
+                    o = cold[c][csel]
+                    fine[c][fsel] = 0.5 * o + 0.5 * cnew[c][csel] if half else o
+
+
+def restrict(ndim, fine, lo, coarse):
+    """Coarse cells under one fine box from their children (see step 4)."""
+    nv = ndim + 2
+    n_f = _fine_dims(fine, ndim)
+    m = [n // 2 for n in n_f]
+    w = {1: 0.5, 2: 0.25, 3: 0.125}[ndim]
+    csel = tuple(slice(lo[d] + 1, lo[d] + 1 + m[d]) for d in reversed(range(ndim)))
+    for c in range(nv):
+        s = np.zeros([m[d] for d in reversed(range(ndim))])
+        for z in range(2 if ndim > 2 else 1):
+            for y in range(2 if ndim > 1 else 1):
+                for x in range(2):
+                    off = (x, y, z)
+                    fsel = tuple(slice(2 + off[d], 2 + off[d] + 2 * m[d], 2)
+                                 for d in reversed(range(ndim)))
+                    s = s + fine[c][fsel]
+        coarse[c][csel] = s * w
+
+
+def fine_params(p):
+    """The fine level's parameters: the coarse ones with every face type 0
+    (ghosts set by the hierarchy)."""
+    q = type(p).from_buffer_copy(p)
+    for f in range(6):
+        q.bc_types[f] = 0
+    return q
+
+
+def step(pc, pf, coarse_dims, coarse, dx0, fines, lo):
+    """One coarse step of the hierarchy, in place; returns dt0.  pc / pf are
+    oracle parameters of the coarse and fine levels, fines the fine boxes'
+    host arrays, lo their placements."""
+    ndim = len(coarse_dims)
+    dx1 = [x / 2.0 for x in dx0]
+    fdims = [_fine_dims(f, ndim) for f in fines]
+    dt0 = cfl_time_step(pc, coarse_dims, coarse, dx0)
+    for fd, f in zip(fdims, fines):
+        dt0 = min(dt0, 2.0 * cfl_time_step(pf, fd, f, dx1))
+    old = coarse.copy()
+    bc_set_boundaries(pc, coarse_dims, coarse)
+    mh_solve(pc, coarse_dims, coarse, dx0, dt0)
+    for half in (0, 1):
+        for fd, f, l in zip(fdims, fines, lo):
+            prolong(ndim, f, l, coarse, old, half)
+            mh_solve(pf, fd, f, dx1, 0.5 * dt0)
+    for f, l in zip(fines, lo):
+        restrict(ndim, f, l, coarse)
+    return dt0

@@ -1,0 +1,136 @@
+"""Two-level SAMR hierarchy on the device (refinement ratio 2, subcycled).
+
+The reference has no coarse-fine coupling: its main loop advances every box
+of every level alone with the level's dt (src/main.f90:118-136).  Here a
+coarse box and a fixed batch of fine boxes are coupled as follows, per
+coarse step:
+
+1. dt0 = min(coarse CFL dt, 2 * min fine CFL dt), each with its level's dx.
+2. The coarse state is kept (U_old); the coarse box gets its ghost fill and
+   one MUSCL-Hancock step with dt0.
+3. Two fine substeps of dt0 / 2, each after the fine boxes' face ghosts are
+   set from their coarse parents, piecewise constant in space: U_old before
+   the first, 0.5 * U_old + 0.5 * U_new before the second.
+4. Every coarse cell under a fine box takes the mean of its 2^ndim children.
+
+No refluxing: mass and energy are conserved up to the coarse-fine flux
+mismatch.  Every kernel is the library's (the box and batch steps, and
+rhy_amr_prolong / rhy_amr_restrict); the orchestration is on the host, with a
+wait between levels.  oracle/amr.py states the same step on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib, hydro
+from .batch import SamrBatch, _L, _check
+
+EXTERNAL = 0  # face type of the fine boxes: ghosts set by the hierarchy
+
+
+def fine_from_coarse(ndim, coarse_cells, lo, fine_dims):
+    """Host array of a fine box (ghosts included) whose every cell holds its
+    coarse parent's state: a fine level that starts as the coarse one."""
+    idx = []
+    for d in reversed(range(ndim)):
+        f = np.arange(-1, fine_dims[d] + 3)
+        idx.append(lo[d] + (f + 1) // 2 - 1 + 1)  # array index of the parent cell
+    return np.ascontiguousarray(coarse_cells[(slice(None),) + np.ix_(*idx)])
+
+
+def fine_params(params):
+    """The fine level's parameters: params with every face type EXTERNAL."""
+    q = type(params).from_buffer_copy(params)
+    for f in range(6):
+        q.bc_types[f] = EXTERNAL
+    return q
+
+
+class AmrHierarchy:
+    """A coarse box (base_dims cells of base_dx) and fine boxes (fine_dims[i]
+    cells of base_dx / 2) placed at fine_lo[i], the coarse cell (1-based) that
+    holds fine cells 1 and 2 along each axis."""
+
+    def __init__(self, ndim, base_dims, base_dx, fine_lo, fine_dims, precision=hydro.FP64,
+                 ncmp=None):
+        self.ndim = ndim
+        self.base_dims = list(base_dims)[:ndim]
+        self.dx0 = [float(x) for x in list(base_dx)[:ndim]]
+        self.dx1 = [x / 2.0 for x in self.dx0]
+        self.fine_dims = [list(d)[:ndim] for d in fine_dims]
+        self.fine_lo = [list(l)[:ndim] for l in fine_lo]
+        self._check_placement()
+        self.ncmp = ncmp if ncmp is not None else hydro.ncmp_for(ndim)
+        self.coarse = hydro.SamrBox(ndim, self.base_dims, self.ncmp, precision)
+        self.coarse_old = hydro.SamrBox(ndim, self.base_dims, self.ncmp, precision)
+        self.fine = SamrBatch(ndim, self.fine_dims, [self.dx1] * len(self.fine_dims),
+                              ncmp=self.ncmp, precision=precision)
+        lo = np.ones((len(self.fine_dims), 3), dtype=np.int32)
+        for i, l in enumerate(self.fine_lo):
+            lo[i, :ndim] = l
+        self._lo = np.ascontiguousarray(lo.ravel())
+
+    def _check_placement(self):
+        nd = self.ndim
+        for n_f, l in zip(self.fine_dims, self.fine_lo):
+            for d in range(nd):
+                if n_f[d] % 2:
+                    raise ValueError("fine extents must be even (ratio 2)")
+                if l[d] < 2 or l[d] + n_f[d] // 2 > self.base_dims[d]:
+                    raise ValueError("a fine box or its ghosts leave the coarse interior")
+        cover = [[(l[d], l[d] + n_f[d] // 2 - 1) for d in range(nd)]
+                 for n_f, l in zip(self.fine_dims, self.fine_lo)]
+        for a in range(len(cover)):
+            for b in range(a + 1, len(cover)):
+                if all(cover[a][d][0] <= cover[b][d][1] and cover[b][d][0] <= cover[a][d][1]
+                       for d in range(nd)):
+                    raise ValueError("fine boxes %d and %d cover the same coarse cells" % (a, b))
+
+    def close(self):
+        self.fine.close()
+        self.coarse.close()
+        self.coarse_old.close()
+
+    def upload(self, coarse_cells, fine_cells):
+        self.coarse.cells[...] = coarse_cells
+        self.coarse.upload()
+        for i, c in enumerate(fine_cells):
+            self.fine.upload(i, c)
+
+    def download(self):
+        return self.coarse.download().copy(), [self.fine.download(i).copy()
+                                                for i in range(len(self.fine_dims))]
+
+    def zones_per_step(self):
+        """Zone-updates of one coarse step: the coarse cells once, the fine
+        cells twice (two substeps)."""
+        return int(np.prod(self.base_dims)) + 2 * sum(int(np.prod(d)) for d in self.fine_dims)
+
+    def _prolong(self, half):
+        _check(_L().rhy_amr_prolong(self.coarse.handle, self.coarse_old.handle, self.fine._h,
+                                    self._lo.ctypes.data_as(C.POINTER(C.c_int32)), half),
+               "rhy_amr_prolong")
+
+    def _restrict(self):
+        _check(_L().rhy_amr_restrict(self.coarse.handle, self.fine._h,
+                                     self._lo.ctypes.data_as(C.POINTER(C.c_int32))),
+               "rhy_amr_restrict")
+
+    def step(self, params):
+        """One coarse step; returns dt0."""
+        fp = fine_params(params)
+        dt0 = hydro.rhyme_cfl_time_step(params, self.coarse, self.dx0)
+        dt0 = min(dt0, 2.0 * float(self.fine.cfl_dts(fp).min()))
+        _lib.check(_lib.load().rhy_box_copy(self.coarse_old.handle, self.coarse.handle),
+                   "rhy_box_copy")
+        hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
+        hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
+        for half in (0, 1):
+            self._prolong(half)
+            self.fine.run_dt_async(fp, 0.5 * dt0, 1)
+            self.fine.sync()
+        self._restrict()
+        return dt0
+
+    def run(self, params, nsteps):
+        return np.array([self.step(params) for _ in range(nsteps)])

@@ -30,6 +30,13 @@ _BATCH_SIGS = {
     "rhy_batch_last_error": (C.c_char_p, []),
     "rhy_batch_diagnostics": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(_lib.RhyDiag)]),
     "rhy_batch_last_deferred": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "rhy_batch_cfl_dts": (C.c_int, [C.c_void_p, C.POINTER(_lib.RhyParams),
+                                    C.POINTER(C.c_double)]),
+    "rhy_batch_run_dt_async": (C.c_int, [C.c_void_p, C.POINTER(_lib.RhyParams), C.c_double,
+                                         C.c_int32]),
+    "rhy_amr_prolong": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
+                                  C.c_int32]),
+    "rhy_amr_restrict": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "rhy_pool_box_bytes": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
@@ -150,6 +157,17 @@ class SamrBatch:
         n = C.c_int64(0)
         _check(_L().rhy_batch_last_deferred(self._h, C.byref(n)), "rhy_batch_last_deferred")
         return n.value
+
+    def cfl_dts(self, params):
+        """The CFL dt of every box for its current state (no step)."""
+        out = np.zeros(self.n)
+        _check(_L().rhy_batch_cfl_dts(self._h, C.byref(params), dptr(out)), "rhy_batch_cfl_dts")
+        return out
+
+    def run_dt_async(self, params, dt, nsteps):
+        """nsteps steps with the given dt for every box (enqueue only)."""
+        _check(_L().rhy_batch_run_dt_async(self._h, C.byref(params), float(dt), nsteps),
+               "rhy_batch_run_dt_async")
 
 
 def box_bytes(ndim, dims, ncmp=None, precision=FP64):

@@ -302,6 +302,96 @@ __global__ void batch_set_counter_kernel(int32_t* step, int32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step = v;
 }
 
+// The given dt for every active box (the fine level of a hierarchy steps
+// with the dt its coarse level sets), logged like batch_dt_kernel's.
+__global__ __launch_bounds__(256) void batch_set_dt_kernel(const int32_t* __restrict__ active,
+                                                           int32_t nactive, double value,
+                                                           double* __restrict__ dt,
+                                                           double* __restrict__ dtlog,
+                                                           const int32_t* __restrict__ step,
+                                                           int32_t log_steps, int32_t log_stride) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= nactive) return;
+  const int b = active[i];
+  dt[b] = value;
+  const int32_t s = *step;
+  if (dtlog && s < log_steps) dtlog[(int64_t)s * log_stride + b] = value;
+}
+
+// Two-level hierarchy, refinement ratio 2.  lo[d] is the coarse cell (1-based)
+// that holds fine cells 1 and 2 along d, so fine cell f lies in coarse cell
+// lo[d] + (f + 1) / 2 - 1 (f >= -1).
+__device__ __forceinline__ int coarse_parent(int lo, int f) { return lo + (f + 1) / 2 - 1; }
+
+// Face ghosts (2 layers, faces only: what the flux walks read) of one fine
+// box from its coarse parents, piecewise constant in space; in time the
+// coarse state at the start of the coarse step (half == 0) or the mean of
+// the start and end states (half == 1, the middle of the coarse step).
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict__ fine, Geom GC,
+                                                          const R* __restrict__ cnew,
+                                                          const R* __restrict__ cold, int lo0,
+                                                          int lo1, int lo2, int half) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lo[3] = {lo0, lo1, lo2};
+  for (int a = 0; a < ND; ++a) {
+    int64_t m = 4;  // 2 sides x 2 layers
+    for (int d = 0; d < 3; ++d)
+      if (d != a) m *= GF.n[d];
+    if (t >= m) {
+      t -= m;
+      continue;
+    }
+    int idx[3];
+    int64_t q = t;
+    const int sl = (int)(q % 4);
+    q /= 4;
+    for (int d = 0; d < 3; ++d) {
+      if (d == a) continue;
+      idx[d] = (int)(q % GF.n[d]) + 1;
+      q /= GF.n[d];
+    }
+    const int side = sl >> 1, layer = sl & 1;
+    idx[a] = side ? GF.n[a] + 1 + layer : -layer;
+    int pc[3] = {1, 1, 1};
+    for (int d = 0; d < ND; ++d) pc[d] = coarse_parent(lo[d], idx[d]);
+    const int64_t of = goff(GF, idx[0], idx[1], idx[2]);
+    const int64_t oc = goff(GC, pc[0], pc[1], pc[2]);
+#pragma unroll
+    for (int c = 0; c < ND + 2; ++c) {
+      const R o = cold[oc + c * GC.plane];
+      fine[of + c * GF.plane] = half ? R(0.5) * o + R(0.5) * cnew[oc + c * GC.plane] : o;
+    }
+    return;
+  }
+}
+
+// The coarse cells a fine box covers take the mean of their 2^ND children:
+// the children summed i fastest, then times 2^-ND.
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_restrict_kernel(Geom GF, const R* __restrict__ fine,
+                                                           Geom GC, R* __restrict__ coarse,
+                                                           int lo0, int lo1, int lo2) {
+  const int m0 = GF.n[0] / 2, m1 = ND > 1 ? GF.n[1] / 2 : 1, m2 = ND > 2 ? GF.n[2] / 2 : 1;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)m0 * m1 * m2) return;
+  const int c0 = (int)(t % m0), c1 = (int)(t / m0 % m1), c2 = (int)(t / ((int64_t)m0 * m1));
+  const int64_t oc = goff(GC, lo0 + c0, ND > 1 ? lo1 + c1 : 1, ND > 2 ? lo2 + c2 : 1);
+  const R w = ND == 3 ? R(0.125) : ND == 2 ? R(0.25) : R(0.5);
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) {
+    R sum = R(0);
+    for (int z = 0; z < (ND > 2 ? 2 : 1); ++z)
+      for (int y = 0; y < (ND > 1 ? 2 : 1); ++y)
+        for (int x = 0; x < 2; ++x) {
+          const int64_t of = goff(GF, 2 * c0 + 1 + x, ND > 1 ? 2 * c1 + 1 + y : 1,
+                                  ND > 2 ? 2 * c2 + 1 + z : 1);
+          sum = sum + fine[of + c * GF.plane];
+        }
+    coarse[oc + c * GC.plane] = sum * w;
+  }
+}
+
 unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) boundaries
@@ -548,10 +638,15 @@ void launch_update_phase(rhy_batch* b, const Consts& k, int order) {
 }
 
 template <int ND, typename R, class PW, class P>
-void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim) {
-  hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0, b->stream,
-                     b->dboxes, b->dactive, b->partial, b->dt, b->dtlog, b->step, b->dtlog_steps,
-                     (int32_t)b->slots.size());
+void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim, double fixed_dt = 0.0) {
+  if (fixed_dt > 0.0)
+    hipLaunchKernelGGL(batch_set_dt_kernel, dim3(cdiv((int64_t)b->active.size(), 256)), dim3(256),
+                       0, b->stream, b->dactive, (int32_t)b->active.size(), fixed_dt, b->dt,
+                       b->dtlog, b->step, b->dtlog_steps, (int32_t)b->slots.size());
+  else
+    hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0,
+                       b->stream, b->dboxes, b->dactive, b->partial, b->dt, b->dtlog, b->step,
+                       b->dtlog_steps, (int32_t)b->slots.size());
   launch_flux_phase<ND, R, PW, P>(b, k, lim);
   launch_update_phase<ND, R>(b, k, order);
   hipLaunchKernelGGL(batch_step_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->step);
@@ -593,8 +688,12 @@ int prologue(rhy_batch* b, const Consts& k) {
   return RHY_OK;
 }
 
+// mode 0: nsteps steps, each box with its own CFL dt; 1: nsteps steps with
+// dt = fixed_dt for every box (no graph); 2: only the CFL dt of every box
+// for the current state, into the batch's dt array (no step).
 template <int ND, typename R, class PW, class P>
-int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph) {
+int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph, int mode = 0,
+          double fixed_dt = 0.0) {
   const Consts k = consts_of(p);
   if (std::memcmp(p, &b->last_params, sizeof(*p)) != 0) {
     // new parameters: inflow states, ghosts and wave speeds all follow them
@@ -642,6 +741,20 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
     b->dtlog_steps = nsteps;
   }
   if ((rc = prologue<ND, R>(b, k))) return rc;
+  if (mode == 2) {
+    hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0,
+                       b->stream, b->dboxes, b->dactive, b->partial, b->dt, (double*)nullptr,
+                       b->step, 0, (int32_t)b->slots.size());
+    BHIP_TRY(hipGetLastError());
+    return RHY_OK;
+  }
+  if (mode == 1) {
+    for (int32_t s = 0; s < nsteps; ++s)
+      enqueue_step<ND, R, PW, P>(b, k, p->solver_type, lim, fixed_dt);
+    BHIP_TRY(hipGetLastError());
+    b->last_nsteps = nsteps;
+    return RHY_OK;
+  }
   if (!use_graph) {
     for (int32_t s = 0; s < nsteps; ++s) enqueue_step<ND, R, PW, P>(b, k, p->solver_type, lim);
     BHIP_TRY(hipGetLastError());
@@ -1006,6 +1119,141 @@ int rhy_batch_run_async(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, i
       return mixed ? run_t<3, float, MixedFlux, MixedMath>(b, p, nsteps, use_graph)
                    : run_t<3, double, PowF64Flux, PowF64Fix>(b, p, nsteps, use_graph);
   }
+}
+
+static int run_checks(rhy_batch_t* b, const rhy_params_t* p) {
+  if (!b || !p) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (p->ndim != b->ndim) return bfail(RHY_ERR_ARG, "params.ndim does not match the batch");
+  if (!b->pool && (p->limiter < 1 || p->limiter > 4))
+    return bfail(RHY_ERR_ARG, "unknown slope limiter");
+  if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
+    return bfail(RHY_ERR_SOLVER, "Unknown solver type");
+  return RHY_OK;
+}
+
+static int run_mode(rhy_batch_t* b, const rhy_params_t* p, int32_t nsteps, int mode, double dt) {
+  const bool mixed = b->precision == RHY_MIXED;
+  switch (b->ndim) {
+    case 1:
+      return mixed ? run_t<1, float, MixedFlux, MixedMath>(b, p, nsteps, 0, mode, dt)
+                   : run_t<1, double, PowF64Flux, PowF64Fix>(b, p, nsteps, 0, mode, dt);
+    case 2:
+      return mixed ? run_t<2, float, MixedFlux, MixedMath>(b, p, nsteps, 0, mode, dt)
+                   : run_t<2, double, PowF64Flux, PowF64Fix>(b, p, nsteps, 0, mode, dt);
+    default:
+      return mixed ? run_t<3, float, MixedFlux, MixedMath>(b, p, nsteps, 0, mode, dt)
+                   : run_t<3, double, PowF64Flux, PowF64Fix>(b, p, nsteps, 0, mode, dt);
+  }
+}
+
+int rhy_batch_run_dt_async(rhy_batch_t* b, const rhy_params_t* p, double dt, int32_t nsteps) {
+  int rc;
+  if ((rc = run_checks(b, p))) return rc;
+  if (!(dt > 0.0) || nsteps < 0) return bfail(RHY_ERR_ARG, "dt must be positive, nsteps >= 0");
+  return run_mode(b, p, nsteps, 1, dt);
+}
+
+int rhy_batch_cfl_dts(rhy_batch_t* b, const rhy_params_t* p, double* dts) {
+  int rc;
+  if ((rc = run_checks(b, p))) return rc;
+  if (!dts) return bfail(RHY_ERR_ARG, "dts is NULL");
+  if ((rc = run_mode(b, p, 0, 2, 0.0))) return rc;
+  BHIP_TRY(hipMemcpyAsync(dts, b->dt, sizeof(double) * b->slots.size(), hipMemcpyDeviceToHost,
+                          b->stream));
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+// Coarse parents of every fine box (ghosts included) must be interior
+// coarse cells: 2 <= lo[d] and lo[d] + n_f[d] / 2 <= n_c[d], n_f[d] even.
+static int amr_check(rhy_batch_t* f, const BoxView& c, const int32_t* lo) {
+  if (!f || !lo) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (f->pool) return bfail(RHY_ERR_ARG, "the fine level is a fixed batch, not a pool");
+  if (c.ndim != f->ndim || c.precision != f->precision)
+    return bfail(RHY_ERR_ARG, "coarse box and fine batch differ in ndim or precision");
+  for (int i = 0; i < (int)f->slots.size(); ++i) {
+    const Slot& S = f->slots[i];
+    if (!S.active) return bfail(RHY_ERR_ARG, "every fine box must be active");
+    for (int d = 0; d < f->ndim; ++d) {
+      const int nf = S.B.G.n[d], l = lo[3 * i + d];
+      if (nf % 2) return bfail(RHY_ERR_SHAPE, "fine extents must be even (ratio 2)");
+      if (l < 2 || l + nf / 2 > c.geom.n[d])
+        return bfail(RHY_ERR_SHAPE, "a fine box or its ghosts leave the coarse interior");
+    }
+  }
+  return RHY_OK;
+}
+
+int rhy_amr_prolong(rhy_box_t* cnew, rhy_box_t* cold, rhy_batch_t* f, const int32_t* lo,
+                    int32_t half) {
+  BoxView vn, vo;
+  int rc;
+  if (!cnew || !cold) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(cnew, &vn)) || (rc = box_view(cold, &vo)))
+    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if (std::memcmp(&vn.geom, &vo.geom, sizeof(Geom)) != 0 || vn.precision != vo.precision)
+    return bfail(RHY_ERR_SHAPE, "the two coarse states differ in shape or precision");
+  if ((rc = amr_check(f, vn, lo))) return rc;
+  for (int i = 0; i < (int)f->slots.size(); ++i) {
+    const BBox& B = f->slots[i].B;
+    int64_t n = 0;
+    for (int a = 0; a < f->ndim; ++a) {
+      int64_t m = 4;
+      for (int d = 0; d < 3; ++d)
+        if (d != a) m *= B.G.n[d];
+      n += m;
+    }
+    const int* l = lo + 3 * i;
+#define RHY_PROLONG(ND, R)                                                                      \
+  hipLaunchKernelGGL((amr_prolong_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0, f->stream, \
+                     B.G, static_cast<R*>(f->arena) + B.cell_off, vn.geom,                     \
+                     static_cast<const R*>(vn.cells), static_cast<const R*>(vo.cells), l[0],   \
+                     l[1], l[2], half ? 1 : 0)
+    if (f->precision == RHY_MIXED) {
+      if (f->ndim == 1) RHY_PROLONG(1, float);
+      else if (f->ndim == 2) RHY_PROLONG(2, float);
+      else RHY_PROLONG(3, float);
+    } else {
+      if (f->ndim == 1) RHY_PROLONG(1, double);
+      else if (f->ndim == 2) RHY_PROLONG(2, double);
+      else RHY_PROLONG(3, double);
+    }
+#undef RHY_PROLONG
+  }
+  BHIP_TRY(hipGetLastError());
+  BHIP_TRY(hipStreamSynchronize(f->stream));
+  return RHY_OK;
+}
+
+int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
+  BoxView vc;
+  int rc;
+  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if ((rc = amr_check(f, vc, lo))) return rc;
+  for (int i = 0; i < (int)f->slots.size(); ++i) {
+    const BBox& B = f->slots[i].B;
+    int64_t n = 1;
+    for (int d = 0; d < f->ndim; ++d) n *= B.G.n[d] / 2;
+    const int* l = lo + 3 * i;
+#define RHY_RESTRICT(ND, R)                                                                      \
+  hipLaunchKernelGGL((amr_restrict_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0, f->stream, \
+                     B.G, static_cast<const R*>(f->arena) + B.cell_off, vc.geom,                \
+                     static_cast<R*>(vc.cells), l[0], l[1], l[2])
+    if (f->precision == RHY_MIXED) {
+      if (f->ndim == 1) RHY_RESTRICT(1, float);
+      else if (f->ndim == 2) RHY_RESTRICT(2, float);
+      else RHY_RESTRICT(3, float);
+    } else {
+      if (f->ndim == 1) RHY_RESTRICT(1, double);
+      else if (f->ndim == 2) RHY_RESTRICT(2, double);
+      else RHY_RESTRICT(3, double);
+    }
+#undef RHY_RESTRICT
+  }
+  BHIP_TRY(hipGetLastError());
+  BHIP_TRY(hipStreamSynchronize(f->stream));
+  return RHY_OK;
 }
 
 int rhy_batch_sync(rhy_batch_t* b) {

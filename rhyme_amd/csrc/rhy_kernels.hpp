@@ -61,6 +61,15 @@ struct BcTypes {
 int32_t defer_every_setting();
 int64_t defer_cap_setting();
 
+// A box's device state as the hierarchy kernels see it (box_view waits for
+// the box's stream).
+struct BoxView {
+  void* cells;
+  Geom geom;
+  int32_t ndim, ncmp, precision;
+};
+int box_view(rhy_box_t* b, BoxView* v);
+
 // Faces whose Riemann solve needs a slow path (Deferring<P>), listed by the
 // flux kernels for the fixup kernel: item = box << 42 | axis << 40 | the
 // offset of the face's low cell in its box's planes.  count may pass cap;

@@ -1,0 +1,187 @@
+"""Two-level hierarchy on the device (rhyme_amd.amr) against its CPU mirror
+(oracle/amr.py): ratio-2 prolongation of the fine boxes' face ghosts (linear
+in time over the two substeps), subcycled fine steps, restriction.  The
+reference has no coarse-fine coupling (src/main.f90:118-136 advances every
+box alone), so oracle/amr.py defines the semantics; the kernels under it are
+the single-box and batch steps whose parity the other tests establish."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import amr as oamr
+from rhyme_amd import _lib, amr, hydro, ics
+from rhyme_amd.batch import SamrBatch, _L
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def params(ndim, limiter, bc, courant=0.3):
+    return hydro.make_params(
+        ndim, rp=hydro.RiemannProblem(irs=hydro.Irs(n_iteration=100, tolerance=1e-6)),
+        sl=hydro.SlopeLimiter(type=limiter), cfl=hydro.Cfl(courant),
+        bc=hydro.SamrBc(types=[bc] * 6))
+
+
+def inject(ndim, coarse, lo, fdims, seed):
+    """A fine box (ghosts included) filled from its coarse parents, every
+    conserved variable of a cell scaled by one factor in [0.97, 1.03] (same
+    velocity, pressure scaled) so that the levels differ."""
+    ncmp = coarse.shape[0]
+    fine = np.zeros(hydro.box_shape(ndim, fdims, ncmp))
+    idx = []
+    for d in reversed(range(ndim)):
+        f = np.arange(-1, fdims[d] + 3)
+        idx.append(lo[d] + (f + 1) // 2 - 1 + 1)     # array index of the parent
+    fine[...] = coarse[(slice(None),) + np.ix_(*idx)]
+    s = 1.0 + 0.03 * np.random.default_rng(seed).uniform(-1, 1, size=fine.shape[1:])
+    fine[:ndim + 2] *= s
+    return fine
+
+
+def rel_l1(a, b, ndim):
+    inner = (slice(None),) + (slice(2, -2),) * ndim
+    a, b = a[inner], b[inner]
+    return [float(np.abs(a[c] - b[c]).sum() / max(np.abs(b[c]).sum(), 1e-300))
+            for c in range(ndim + 2)]
+
+
+CASES = {
+    "1d": (1, [96], [[24], [16]], [[10], [60]], hydro.VAN_LEER, hydro.OUTFLOW, 12),
+    "2d": (2, [40, 32], [[24, 16], [8, 12]], [[4, 6], [24, 18]], hydro.SUPERBEE,
+           hydro.PERIODIC, 8),
+    "3d": (3, [20, 18, 16], [[12, 8, 10], [6, 6, 6]], [[3, 4, 3], [12, 11, 10]],
+           hydro.MINMOD, hydro.OUTFLOW, 4),
+}
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_hierarchy_matches_oracle(case, precision):
+    ndim, cdims, fdims, lo, lim, bc, nsteps = CASES[case]
+    say("hierarchy", case, "precision", precision, "coarse", cdims, "fine", fdims, "lo", lo)
+    coarse = ics.random_smooth(ndim, cdims, seed=7 + ndim)
+    fines = [inject(ndim, coarse, l, d, 100 + i) for i, (l, d) in enumerate(zip(lo, fdims))]
+    p = params(ndim, lim, bc)
+    dx0 = [1.0 / cdims[0]] * ndim
+    h = amr.AmrHierarchy(ndim, cdims, dx0, lo, fdims, precision=precision)
+    h.upload(coarse, fines)
+    dts = h.run(p, nsteps)
+    dc, dfs = h.download()
+    h.close()
+    rc, rfs = coarse.copy(), [f.copy() for f in fines]
+    pc = oracle.params_from(p)
+    pf = oamr.fine_params(pc)
+    rdts = np.array([oamr.step(pc, pf, cdims, rc, dx0, rfs, lo) for _ in range(nsteps)])
+    tol = 1e-12 if precision == hydro.FP64 else 1e-5
+    e_dt = float(np.max(np.abs(dts - rdts) / rdts))
+    say("  dt rel err", e_dt)
+    assert e_dt <= tol
+    e = rel_l1(dc, rc, ndim)
+    say("  coarse rel L1", e)
+    assert max(e) <= tol
+    for i, (a, b) in enumerate(zip(dfs, rfs)):
+        e = rel_l1(a, b, ndim)
+        say("  fine", i, "rel L1", e)
+        assert max(e) <= tol
+    # restriction ran last: covered coarse cells are the mean of the device's fine cells
+    again = dc.copy()
+    for f, l in zip(dfs, lo):
+        oamr.restrict(ndim, f, l, again)
+    if precision == hydro.FP64:
+        assert np.array_equal(again, dc)
+    else:
+        # FP32 sums of 2^ndim children: a few ulp of the variable's scale
+        for c in range(ndim + 2):
+            np.testing.assert_allclose(again[c], dc[c], rtol=0,
+                                       atol=1e-6 * np.abs(dc[c]).max())
+
+
+def test_hierarchy_blast_fp64():
+    """A 2D blast whose shock crosses both fine boxes (strong shocks, faces
+    that defer to the full solver)."""
+    ndim, cdims = 2, [48, 48]
+    fdims, lo = [[20, 24], [16, 16]], [[14, 12], [30, 30]]
+    coarse = ics.blast(ndim, cdims, seed=3)
+    fines = [inject(ndim, coarse, l, d, 200 + i) for i, (l, d) in enumerate(zip(lo, fdims))]
+    p = params(ndim, hydro.VAN_ALBADA, hydro.OUTFLOW)
+    dx0 = [1.0 / 48] * 2
+    h = amr.AmrHierarchy(ndim, cdims, dx0, lo, fdims)
+    h.upload(coarse, fines)
+    dts = h.run(p, 10)
+    dc, dfs = h.download()
+    h.close()
+    rc, rfs = coarse.copy(), [f.copy() for f in fines]
+    pc = oracle.params_from(p)
+    rdts = np.array([oamr.step(pc, oamr.fine_params(pc), cdims, rc, dx0, rfs, lo)
+                     for _ in range(10)])
+    assert np.max(np.abs(dts - rdts) / rdts) <= 1e-12
+    assert max(rel_l1(dc, rc, ndim)) <= 1e-12
+    for a, b in zip(dfs, rfs):
+        e = rel_l1(a, b, ndim)
+        say("blast fine rel L1", e)
+        assert max(e) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_uniform_state_stays_uniform_large(precision):
+    """At a full-size coarse box (128^3 with four 64^3 fine boxes) a uniform
+    state is a fixed point of every kernel: both levels stay bit-identical."""
+    ndim, cdims = 3, [128, 128, 128]
+    fdims = [[64, 64, 64]] * 4
+    lo = [[4, 4, 4], [70, 4, 4], [4, 70, 40], [70, 70, 90]]
+    ncmp = hydro.ncmp_for(ndim)
+
+    def uniform(dims):
+        # rho 1, u 0.25, p 1 (gamma 5/3): E = 1.5 + 0.03125, exact in FP32 too
+        c = np.zeros(hydro.box_shape(ndim, dims, ncmp))
+        c[0] = 1.0
+        c[1] = 0.25
+        c[ndim + 1] = 1.53125
+        return c
+    h = amr.AmrHierarchy(ndim, cdims, [1.0 / 128] * 3, lo, fdims, precision=precision)
+    h.upload(uniform(cdims), [uniform(d) for d in fdims])
+    dts = h.run(params(ndim, hydro.SUPERBEE, hydro.PERIODIC), 3)
+    dc, dfs = h.download()
+    h.close()
+    say("uniform 128^3 + 4 x 64^3, precision", precision, "dts", dts)
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    assert np.array_equal(dc[inner], uniform(cdims)[inner])
+    for f, d in zip(dfs, fdims):
+        assert np.array_equal(f[inner], uniform(d)[inner])
+    assert np.all(dts == dts[0])
+
+
+def test_batch_cfl_dts_match_single_boxes():
+    boxes = ics.amr_boxes(seed=11, nboxes=5, base=24, lo=6, hi=20)
+    p = params(3, hydro.MINMOD, hydro.OUTFLOW)
+    b = SamrBatch(3, [d for d, _, _ in boxes], [x for _, x, _ in boxes])
+    for i, (_, _, c) in enumerate(boxes):
+        b.upload(i, c)
+    dts = b.cfl_dts(p)
+    for i, (dims, dx, c) in enumerate(boxes):
+        box = hydro.SamrBox(3, dims, cells=c.copy())
+        box.upload()
+        assert dts[i] == hydro.rhyme_cfl_time_step(p, box, dx)
+        box.close()
+    b.close()
+
+
+def test_bad_placement_rejected_by_library():
+    ndim, cdims = 2, [16, 16]
+    h = amr.AmrHierarchy(ndim, cdims, [1.0 / 16] * 2, [[2, 2]], [[8, 8]])
+    L = _L()
+    for bad in ([1, 2, 1], [2, 13, 1]):
+        lo = np.array(bad, dtype=np.int32)
+        rc = L.rhy_amr_prolong(h.coarse.handle, h.coarse_old.handle, h.fine._h,
+                               lo.ctypes.data_as(C.POINTER(C.c_int32)), 0)
+        assert rc == _lib.RHY_ERR_SHAPE
+        rc = L.rhy_amr_restrict(h.coarse.handle, h.fine._h,
+                                lo.ctypes.data_as(C.POINTER(C.c_int32)))
+        assert rc == _lib.RHY_ERR_SHAPE
+    h.close()
