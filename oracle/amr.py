@@ -15,7 +15,9 @@ reproduce this step bit for bit in FP64:
 3. Two fine substeps of dt0 / 2.  Before each, every fine box's face ghosts
    (2 layers) take the value of the coarse cell that contains them: U_old
    before the first substep, 0.5 * U_old + 0.5 * U_new before the second
-   (linear in time).  Fine boxes have no boundary fill of their own.
+   (linear in time).  Then the face ghosts that lie in another fine box's
+   interior take that box's cells instead (same level and time).  Fine boxes
+   have no boundary fill of their own.
 4. Every coarse cell under a fine box takes the mean of its 2^ndim children
    (summed with i fastest, then times 2^-ndim).  There is no refluxing, so
    mass and energy are conserved only up to the coarse-fine flux mismatch.
@@ -77,6 +79,39 @@ def prolong(ndim, fine, lo, cnew, cold, half):
                     fine[c][fsel] = 0.5 * o + 0.5 * cnew[c][csel] if half else o
 
 
+def fill_siblings(ndim, fines, lo):
+    """Face ghosts of each fine box that lie in another fine box's interior
+    take that box's cells (see step 3).  Box i's cell f lies at fine-level
+    index 2 (lo_i - 1) + f; interiors do not overlap."""
+    nv = ndim + 2
+    dims = [_fine_dims(f, ndim) for f in fines]
+    for i, fi in enumerate(fines):
+        bi = [2 * (lo[i][d] - 1) for d in range(ndim)]
+        for a in range(ndim):
+            for side in (0, 1):
+                for j, fj in enumerate(fines):
+                    if j == i:
+                        continue
+                    bj = [2 * (lo[j][d] - 1) for d in range(ndim)]
+                    g0, g1 = [], []
+                    for d in range(ndim):
+                        s0, s1 = bi[d] + 1, bi[d] + dims[i][d]
+                        if d == a:
+                            s0 = bi[d] + dims[i][d] + 1 if side else bi[d] - 1
+                            s1 = s0 + 1
+                        g0.append(max(s0, bj[d] + 1))
+                        g1.append(min(s1, bj[d] + dims[j][d]))
+                    if any(g0[d] > g1[d] for d in range(ndim)):
+                        continue
+                    # array index of Fortran index f is f + 1; numpy axes reversed
+                    dsel = tuple(slice(g0[d] - bi[d] + 1, g1[d] - bi[d] + 2)
+                                 for d in reversed(range(ndim)))
+                    ssel = tuple(slice(g0[d] - bj[d] + 1, g1[d] - bj[d] + 2)
+                                 for d in reversed(range(ndim)))
+                    for c in range(nv):
+                        fi[c][dsel] = fj[c][ssel]
+
+
 def restrict(ndim, fine, lo, coarse):
     """Coarse cells under one fine box from their children (see step 4)."""
     nv = ndim + 2
@@ -119,8 +154,10 @@ def step(pc, pf, coarse_dims, coarse, dx0, fines, lo):
     bc_set_boundaries(pc, coarse_dims, coarse)
     mh_solve(pc, coarse_dims, coarse, dx0, dt0)
     for half in (0, 1):
-        for fd, f, l in zip(fdims, fines, lo):
+        for f, l in zip(fines, lo):
             prolong(ndim, f, l, coarse, old, half)
+        fill_siblings(ndim, fines, lo)
+        for fd, f in zip(fdims, fines):
             mh_solve(pf, fd, f, dx1, 0.5 * dt0)
     for f, l in zip(fines, lo):
         restrict(ndim, f, l, coarse)

@@ -366,6 +366,23 @@ __global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict
   }
 }
 
+// Cells of a rectangular region copied from one box to another (fine ghosts
+// that lie in a sibling's interior): ext[d] cells from src index slo[d] to
+// dst index dlo[d], the ND + 2 hydro components.
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_copy_region_kernel(Geom GD, R* __restrict__ dst,
+                                                              Geom GS, const R* __restrict__ src,
+                                                              int3 dlo, int3 slo, int3 ext) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (int64_t)ext.x * ext.y * ext.z) return;
+  const int x = (int)(t % ext.x), y = (int)(t / ext.x % ext.y),
+            z = (int)(t / ((int64_t)ext.x * ext.y));
+  const int64_t od = goff(GD, dlo.x + x, dlo.y + y, dlo.z + z);
+  const int64_t os = goff(GS, slo.x + x, slo.y + y, slo.z + z);
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) dst[od + c * GD.plane] = src[os + c * GS.plane];
+}
+
 // The coarse cells a fine box covers take the mean of their 2^ND children:
 // the children summed i fastest, then times 2^-ND.
 template <int ND, typename R>
@@ -1219,6 +1236,65 @@ int rhy_amr_prolong(rhy_box_t* cnew, rhy_box_t* cold, rhy_batch_t* f, const int3
       else RHY_PROLONG(3, double);
     }
 #undef RHY_PROLONG
+  }
+  // Face ghosts inside a sibling's interior take the sibling's cells instead
+  // (same level, same time).  In fine-level indices box i's cell f lies at
+  // 2 (lo_i - 1) + f; interiors never overlap, so each ghost has at most one
+  // sibling source, and the copies run after every coarse fill above.
+  const int nd = f->ndim, nb = (int)f->slots.size();
+  for (int i = 0; i < nb; ++i) {
+    const BBox& D = f->slots[i].B;
+    for (int a = 0; a < nd; ++a)
+      for (int side = 0; side < 2; ++side)
+        for (int j = 0; j < nb; ++j) {
+          if (j == i) continue;
+          const BBox& S = f->slots[j].B;
+          int g0[3], g1[3];  // the face slab in fine-level indices, intersected with j
+          bool empty = false;
+          for (int d = 0; d < 3; ++d) {
+            const int bi = d < nd ? 2 * (lo[3 * i + d] - 1) : 0;
+            const int bj = d < nd ? 2 * (lo[3 * j + d] - 1) : 0;
+            int s0 = bi + 1, s1 = bi + D.G.n[d];
+            if (d == a) {
+              s0 = side ? bi + D.G.n[d] + 1 : bi - 1;
+              s1 = s0 + 1;
+            }
+            g0[d] = std::max(s0, bj + 1);
+            g1[d] = std::min(s1, bj + S.G.n[d]);
+            empty = empty || g0[d] > g1[d];
+          }
+          if (empty) continue;
+          int3 dlo, slo, ext;
+          int* dl = &dlo.x;
+          int* sl = &slo.x;
+          int* ex = &ext.x;
+          for (int d = 0; d < 3; ++d) {
+            const int bi = d < nd ? 2 * (lo[3 * i + d] - 1) : 0;
+            const int bj = d < nd ? 2 * (lo[3 * j + d] - 1) : 0;
+            dl[d] = g0[d] - bi;
+            sl[d] = g0[d] - bj;
+            ex[d] = g1[d] - g0[d] + 1;
+            // dst within its ghost layers, src within its interior
+            if (dl[d] < 1 - D.G.g[d] || dl[d] + ex[d] - 1 > D.G.n[d] + D.G.g[d] || sl[d] < 1 ||
+                sl[d] + ex[d] - 1 > S.G.n[d])
+              return bfail(RHY_ERR_SHAPE, "sibling ghost region outside a box");
+          }
+          const int64_t n = (int64_t)ext.x * ext.y * ext.z;
+#define RHY_SIBLING(ND, R)                                                                      \
+  hipLaunchKernelGGL((amr_copy_region_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0,        \
+                     f->stream, D.G, static_cast<R*>(f->arena) + D.cell_off, S.G,              \
+                     static_cast<const R*>(f->arena) + S.cell_off, dlo, slo, ext)
+          if (f->precision == RHY_MIXED) {
+            if (nd == 1) RHY_SIBLING(1, float);
+            else if (nd == 2) RHY_SIBLING(2, float);
+            else RHY_SIBLING(3, float);
+          } else {
+            if (nd == 1) RHY_SIBLING(1, double);
+            else if (nd == 2) RHY_SIBLING(2, double);
+            else RHY_SIBLING(3, double);
+          }
+#undef RHY_SIBLING
+        }
   }
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));

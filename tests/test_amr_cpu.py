@@ -118,3 +118,73 @@ def test_uniform_state_stays_uniform(ndim):
     assert np.array_equal(coarse[inner], uniform(cdims)[inner])
     for f, d in zip(fines, fdims):
         assert np.array_equal(f[inner], uniform(d)[inner])
+
+
+def test_fill_siblings_against_loops():
+    """2D, three fine boxes: two share a face along x, the third touches the
+    second along y.  Every face ghost cell inside another box's interior
+    takes that cell; every other cell is left alone."""
+    ndim = 2
+    fdims = [[8, 6], [6, 6], [4, 4]]
+    lo = [[3, 3], [7, 3], [8, 6]]
+    oamr.check_placement(ndim, [16, 16], fdims, lo)
+    ncmp = hydro.ncmp_for(ndim)
+    g = np.random.default_rng(5)
+    fines = [g.uniform(1, 2, size=hydro.box_shape(ndim, d, ncmp)) for d in fdims]
+    before = [f.copy() for f in fines]
+    oamr.fill_siblings(ndim, fines, lo)
+
+    def owner(gx, gy, skip):
+        for j, (d, l) in enumerate(zip(fdims, lo)):
+            fx, fy = gx - 2 * (l[0] - 1), gy - 2 * (l[1] - 1)
+            if j != skip and 1 <= fx <= d[0] and 1 <= fy <= d[1]:
+                return j, fx, fy
+        return None
+    copied = 0
+    for i, (d, l) in enumerate(zip(fdims, lo)):
+        for fy in range(-1, d[1] + 3):
+            for fx in range(-1, d[0] + 3):
+                face = (1 <= fx <= d[0]) != (1 <= fy <= d[1])   # face ghost, not edge
+                o = owner(2 * (l[0] - 1) + fx, 2 * (l[1] - 1) + fy, i) if face else None
+                for c in range(ncmp):
+                    got = fines[i][c][fy + 1, fx + 1]
+                    if o is not None and c < ndim + 2:
+                        j, sx, sy = o
+                        assert got == before[j][c][sy + 1, sx + 1]
+                        copied += c == 0
+                    else:
+                        assert got == before[i][c][fy + 1, fx + 1]
+    # box 0 <-> box 1 share a 6-cell x face, box 1 <-> box 2 a 4-cell y face
+    assert copied == 2 * (2 * 6) + 2 * (2 * 4)
+
+
+@pytest.mark.parametrize("ndim", [1, 2, 3])
+def test_split_fine_box_identical(ndim):
+    """A fine box cut in two along x gives the same hierarchy step, bit for
+    bit, as the whole box: the shared face's ghosts come from the sibling."""
+    cdims = [24, 16, 12][:ndim]
+    whole, lo = [16, 12, 8][:ndim], [5, 3, 3][:ndim]
+    left = [8] + whole[1:]
+    right = [8] + whole[1:]
+    lo_r = [lo[0] + 4] + lo[1:]
+    p = mk(ndim, hydro.OUTFLOW)
+    pf = oamr.fine_params(p)
+    dx0 = [1.0 / cdims[0]] * ndim
+    coarse = ics.random_smooth(ndim, cdims, seed=20 + ndim)
+    fw = np.zeros(hydro.box_shape(ndim, whole, coarse.shape[0]))
+    idx = [lo[d] + (np.arange(-1, whole[d] + 3) + 1) // 2 for d in reversed(range(ndim))]
+    fw[...] = coarse[(slice(None),) + np.ix_(*idx)]
+    fw[:ndim + 2] *= 1.0 + 0.02 * np.random.default_rng(1).uniform(-1, 1, fw.shape[1:])
+    fl = np.ascontiguousarray(fw[..., :left[0] + 4])
+    fr = np.ascontiguousarray(fw[..., right[0]:])
+    c1, c2 = coarse.copy(), coarse.copy()
+    fines2 = [fl, fr]
+    for _ in range(3):
+        d1 = oamr.step(p, pf, cdims, c1, dx0, [fw], [lo])
+        d2 = oamr.step(p, pf, cdims, c2, dx0, fines2, [lo, lo_r])
+        assert d1 == d2
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    assert np.array_equal(c1[inner], c2[inner])
+    inner_w = (slice(0, ndim + 2),) + (slice(2, -2),) * (ndim - 1)
+    assert np.array_equal(fw[inner_w + (slice(2, 2 + left[0]),)], fl[inner_w + (slice(2, -2),)])
+    assert np.array_equal(fw[inner_w + (slice(2 + left[0], -2),)], fr[inner_w + (slice(2, -2),)])

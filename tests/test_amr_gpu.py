@@ -57,6 +57,12 @@ CASES = {
            hydro.PERIODIC, 8),
     "3d": (3, [20, 18, 16], [[12, 8, 10], [6, 6, 6]], [[3, 4, 3], [12, 11, 10]],
            hydro.MINMOD, hydro.OUTFLOW, 4),
+    # fine boxes that touch: ghosts from siblings across the shared faces
+    "2d_adjacent": (2, [32, 32], [[12, 16], [8, 16], [12, 8]], [[6, 8], [12, 8], [8, 16]],
+                    hydro.VAN_ALBADA, hydro.PERIODIC, 6),
+    "3d_adjacent": (3, [20, 20, 20], [[8, 8, 8]] * 4, [[5, 5, 5], [9, 5, 5], [5, 9, 5],
+                                                       [9, 9, 9]], hydro.VAN_LEER,
+                    hydro.OUTFLOW, 4),
 }
 
 
@@ -100,6 +106,44 @@ def test_hierarchy_matches_oracle(case, precision):
         for c in range(ndim + 2):
             np.testing.assert_allclose(again[c], dc[c], rtol=0,
                                        atol=1e-6 * np.abs(dc[c]).max())
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_split_fine_box_identical(precision):
+    """A 3D fine box cut in two along x, and each half cut again along y:
+    the device hierarchy gives the whole box's states and dts bit for bit
+    (the shared faces' ghosts come from the siblings)."""
+    ndim, cdims = 3, [24, 20, 16]
+    whole, lo = [16, 12, 8], [5, 4, 4]
+    coarse = ics.random_smooth(ndim, cdims, seed=31)
+    fw = inject(ndim, coarse, lo, whole, 77)
+    parts, plo = [], []
+    for x0 in (0, 8):
+        for y0 in (0, 6):
+            parts.append(np.ascontiguousarray(fw[:, :, y0:y0 + 10, x0:x0 + 12]))
+            plo.append([lo[0] + x0 // 2, lo[1] + y0 // 2, lo[2]])
+    p = params(ndim, hydro.SUPERBEE, hydro.OUTFLOW)
+    dx0 = [1.0 / cdims[0]] * ndim
+
+    def go(fdims, lo_list, fines):
+        h = amr.AmrHierarchy(ndim, cdims, dx0, lo_list, fdims, precision=precision)
+        h.upload(coarse, fines)
+        dts = h.run(p, 5)
+        out = h.download()
+        h.close()
+        return dts, out
+    d1, (c1, f1) = go([whole], [lo], [fw])
+    d2, (c2, f2) = go([[8, 6, 8]] * 4, plo, parts)
+    say("split box, precision", precision, "dts", d1[:2])
+    np.testing.assert_array_equal(d1, d2)
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    np.testing.assert_array_equal(c1[inner], c2[inner])
+    k = 0
+    for x0 in (0, 8):
+        for y0 in (0, 6):
+            np.testing.assert_array_equal(
+                f1[0][:ndim + 2, 2:-2, 2 + y0:8 + y0, 2 + x0:10 + x0], f2[k][inner])
+            k += 1
 
 
 def test_hierarchy_blast_fp64():
