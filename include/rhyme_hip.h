@@ -114,11 +114,6 @@ int rhy_box_upload(rhy_box_t *box, const double *cells);
 int rhy_box_download(rhy_box_t *box, double *cells);
 /* Wait for all work queued on the box. */
 int rhy_box_sync(rhy_box_t *box);
-/* Device copy of the whole state of src into dst (same dims, ncmp and
- * precision).  No counterpart in the reference; the two-level hierarchy keeps
- * the coarse state of the start of a coarse step with it. */
-int rhy_box_copy(rhy_box_t *dst, rhy_box_t *src);
-
 /* ---- the module procedures ---- */
 
 /* rhyme_samr_bc_set_boundaries (rhyme_samr_bc/src/rhyme_samr_bc_set_boundaries.f90:3-22):
@@ -292,16 +287,19 @@ int rhy_batch_last_deferred(rhy_batch_t *batch, int64_t *n);
  *   dts[nboxes], no step).
  * rhy_batch_run_dt_async: nsteps steps with the given dt for every box.
  * rhy_amr_prolong: the fine boxes' face ghosts from their coarse parents,
- *   piecewise constant; half = 0 takes coarse_old (the start of the coarse
- *   step), half = 1 the mean 0.5*old + 0.5*new (its middle).
+ *   piecewise constant.  half = 0 sets each ghost to its parent: call it
+ *   before the coarse step.  half = 1 sets each ghost to the mean of its
+ *   value and its parent's: call it after the coarse step and the first fine
+ *   substep (fine steps with face type 0 leave face ghosts alone), so the
+ *   ghost becomes 0.5 * U_old + 0.5 * U_new.  Both then copy the face ghosts
+ *   that lie in another fine box's interior from that box.
  * rhy_amr_restrict: every coarse cell under a fine box takes the mean of
  *   its 2^ndim children (summed with i fastest, times 2^-ndim).
  * The prolong and restrict calls wait for both sides' streams. */
 int rhy_batch_cfl_dts(rhy_batch_t *batch, const rhy_params_t *p, double *dts);
 int rhy_batch_run_dt_async(rhy_batch_t *batch, const rhy_params_t *p, double dt,
                            int32_t nsteps);
-int rhy_amr_prolong(rhy_box_t *coarse_new, rhy_box_t *coarse_old, rhy_batch_t *fine,
-                    const int32_t *lo, int32_t half);
+int rhy_amr_prolong(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, int32_t half);
 int rhy_amr_restrict(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo);
 
 /* ---- pool: a batch whose boxes come and go (continuous batching) ----

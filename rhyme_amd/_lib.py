@@ -35,7 +35,7 @@ EXPORTS = [
     "rhy_pool_create", "rhy_pool_box_bytes", "rhy_pool_add", "rhy_pool_add_ic",
     "rhy_pool_remove", "rhy_pool_free_bytes",
     "rhy_box_last_deferred", "rhy_batch_last_deferred", "rhy_debug_defer",
-    "rhy_box_copy", "rhy_batch_cfl_dts", "rhy_batch_run_dt_async", "rhy_amr_prolong",
+    "rhy_batch_cfl_dts", "rhy_batch_run_dt_async", "rhy_amr_prolong",
     "rhy_amr_restrict",
 ]
 
@@ -162,7 +162,6 @@ def load():
         "rhy_box_diagnostics": (C.c_int, [vp, C.POINTER(RhyDiag)]),
         "rhy_box_last_deferred": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "rhy_debug_defer": (C.c_int, [C.c_int32, C.c_int64]),
-        "rhy_box_copy": (C.c_int, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -6,23 +6,26 @@ coarse box and a fixed batch of fine boxes are coupled as follows, per
 coarse step:
 
 1. dt0 = min(coarse CFL dt, 2 * min fine CFL dt), each with its level's dx.
-2. The coarse state is kept (U_old); the coarse box gets its ghost fill and
-   one MUSCL-Hancock step with dt0.
+2. The coarse box gets its ghost fill and one MUSCL-Hancock step with dt0.
 3. Two fine substeps of dt0 / 2, each after the fine boxes' face ghosts are
    set from their coarse parents, piecewise constant in space: U_old before
-   the first, 0.5 * U_old + 0.5 * U_new before the second.
+   the first, 0.5 * U_old + 0.5 * U_new before the second; ghosts inside
+   another fine box's interior are copied from that box instead.
 4. Every coarse cell under a fine box takes the mean of its 2^ndim children.
 
 No refluxing: mass and energy are conserved up to the coarse-fine flux
 mismatch.  Every kernel is the library's (the box and batch steps, and
-rhy_amr_prolong / rhy_amr_restrict); the orchestration is on the host, with a
-wait between levels.  oracle/amr.py states the same step on the CPU.
+rhy_amr_prolong / rhy_amr_restrict); the orchestration is on the host.  The
+ghosts for the first substep are set before the coarse step, so no copy of
+U_old is kept (the second fill interpolates in place), and the first fine
+substep runs on the batch's stream while the coarse step runs on the box's.
+oracle/amr.py states the same step on the CPU.
 """
 import ctypes as C
 
 import numpy as np
 
-from . import _lib, hydro
+from . import hydro
 from .batch import SamrBatch, _L, _check
 
 EXTERNAL = 0  # face type of the fine boxes: ghosts set by the hierarchy
@@ -62,7 +65,6 @@ class AmrHierarchy:
         self._check_placement()
         self.ncmp = ncmp if ncmp is not None else hydro.ncmp_for(ndim)
         self.coarse = hydro.SamrBox(ndim, self.base_dims, self.ncmp, precision)
-        self.coarse_old = hydro.SamrBox(ndim, self.base_dims, self.ncmp, precision)
         self.fine = SamrBatch(ndim, self.fine_dims, [self.dx1] * len(self.fine_dims),
                               ncmp=self.ncmp, precision=precision)
         lo = np.ones((len(self.fine_dims), 3), dtype=np.int32)
@@ -89,7 +91,6 @@ class AmrHierarchy:
     def close(self):
         self.fine.close()
         self.coarse.close()
-        self.coarse_old.close()
 
     def upload(self, coarse_cells, fine_cells):
         self.coarse.cells[...] = coarse_cells
@@ -107,7 +108,7 @@ class AmrHierarchy:
         return int(np.prod(self.base_dims)) + 2 * sum(int(np.prod(d)) for d in self.fine_dims)
 
     def _prolong(self, half):
-        _check(_L().rhy_amr_prolong(self.coarse.handle, self.coarse_old.handle, self.fine._h,
+        _check(_L().rhy_amr_prolong(self.coarse.handle, self.fine._h,
                                     self._lo.ctypes.data_as(C.POINTER(C.c_int32)), half),
                "rhy_amr_prolong")
 
@@ -121,15 +122,13 @@ class AmrHierarchy:
         fp = fine_params(params)
         dt0 = hydro.rhyme_cfl_time_step(params, self.coarse, self.dx0)
         dt0 = min(dt0, 2.0 * float(self.fine.cfl_dts(fp).min()))
-        _lib.check(_lib.load().rhy_box_copy(self.coarse_old.handle, self.coarse.handle),
-                   "rhy_box_copy")
+        self._prolong(0)                              # U_old: before the coarse step
+        self.fine.run_dt_async(fp, 0.5 * dt0, 1)      # overlaps the coarse step
         hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
         hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
-        for half in (0, 1):
-            self._prolong(half)
-            self.fine.run_dt_async(fp, 0.5 * dt0, 1)
-            self.fine.sync()
-        self._restrict()
+        self._prolong(1)                              # after substep 1 (stream order)
+        self.fine.run_dt_async(fp, 0.5 * dt0, 1)
+        self._restrict()                              # waits for the batch's stream
         return dt0
 
     def run(self, params, nsteps):

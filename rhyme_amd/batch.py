@@ -34,8 +34,7 @@ _BATCH_SIGS = {
                                     C.POINTER(C.c_double)]),
     "rhy_batch_run_dt_async": (C.c_int, [C.c_void_p, C.POINTER(_lib.RhyParams), C.c_double,
                                          C.c_int32]),
-    "rhy_amr_prolong": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32),
-                                  C.c_int32]),
+    "rhy_amr_prolong": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
     "rhy_amr_restrict": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
     "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
                                   C.POINTER(C.c_void_p)]),

@@ -584,19 +584,6 @@ int rhy_box_download(rhy_box_t* b, double* cells) {
   return RHY_OK;
 }
 
-int rhy_box_copy(rhy_box_t* dst, rhy_box_t* src) {
-  int rc;
-  if ((rc = check_box(dst)) || (rc = check_box(src))) return rc;
-  if (dst->ndim != src->ndim || dst->ncmp != src->ncmp || dst->precision != src->precision ||
-      std::memcmp(dst->dims, src->dims, sizeof(dst->dims)) != 0)
-    return fail(RHY_ERR_SHAPE, "boxes differ in shape, components or precision");
-  HIP_TRY(hipStreamSynchronize(src->stream));
-  HIP_TRY(hipMemcpyAsync(dst->cells, src->cells, dst->esz * (size_t)dst->geom.plane * dst->ncmp,
-                         hipMemcpyDeviceToDevice, dst->stream));
-  HIP_TRY(hipStreamSynchronize(dst->stream));
-  return RHY_OK;
-}
-
 int rhy_box_sync(rhy_box_t* b) {
   int rc;
   if ((rc = check_box(b))) return rc;

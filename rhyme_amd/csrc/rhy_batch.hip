@@ -324,13 +324,14 @@ __global__ __launch_bounds__(256) void batch_set_dt_kernel(const int32_t* __rest
 __device__ __forceinline__ int coarse_parent(int lo, int f) { return lo + (f + 1) / 2 - 1; }
 
 // Face ghosts (2 layers, faces only: what the flux walks read) of one fine
-// box from its coarse parents, piecewise constant in space; in time the
-// coarse state at the start of the coarse step (half == 0) or the mean of
-// the start and end states (half == 1, the middle of the coarse step).
+// box from its coarse parents, piecewise constant in space.  half == 0: the
+// parent's value (called before the coarse step, so the start of the coarse
+// step).  half == 1: the mean of the ghost's value and the parent's (called
+// after the coarse step and the first fine substep, which leaves the fine
+// boxes' face ghosts as they were), i.e. 0.5 * U_old + 0.5 * U_new.
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict__ fine, Geom GC,
-                                                          const R* __restrict__ cnew,
-                                                          const R* __restrict__ cold, int lo0,
+                                                          const R* __restrict__ coarse, int lo0,
                                                           int lo1, int lo2, int half) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lo[3] = {lo0, lo1, lo2};
@@ -359,8 +360,9 @@ __global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict
     const int64_t oc = goff(GC, pc[0], pc[1], pc[2]);
 #pragma unroll
     for (int c = 0; c < ND + 2; ++c) {
-      const R o = cold[oc + c * GC.plane];
-      fine[of + c * GF.plane] = half ? R(0.5) * o + R(0.5) * cnew[oc + c * GC.plane] : o;
+      const R v = coarse[oc + c * GC.plane];
+      R* g = fine + of + c * GF.plane;
+      *g = half ? R(0.5) * *g + R(0.5) * v : v;
     }
     return;
   }
@@ -675,7 +677,7 @@ void enqueue_step(rhy_batch* b, const Consts& k, int order, int lim, double fixe
 // was just set get a fill, and the partials are recomputed only when the
 // tile table, the parameters or a state changed since the last step.
 template <int ND, typename R>
-int prologue(rhy_batch* b, const Consts& k) {
+int prologue(rhy_batch* b, const Consts& k, bool need_partials) {
   R* a = static_cast<R*>(b->arena);
   for (int i : b->active) {
     Slot& S = b->slots[i];
@@ -692,7 +694,7 @@ int prologue(rhy_batch* b, const Consts& k) {
     hipLaunchKernelGGL((bc_kernel<ND, R>), dim3(cdiv(maxface, 256), 2 * ND), dim3(256), 0,
                        b->stream, B.G, B.bc, b->inflow, b->ncmp, a + B.cell_off, -1);
   }
-  if (!b->pool || !b->partials_valid) {
+  if (need_partials && (!b->pool || !b->partials_valid)) {
     const unsigned g = persistent_grid(b, b->nupd_tiles);
     hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
                        b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
@@ -757,7 +759,8 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
       return bfail(RHY_ERR_OOM, "dt log allocation failed");
     b->dtlog_steps = nsteps;
   }
-  if ((rc = prologue<ND, R>(b, k))) return rc;
+  // steps with a given dt need no CFL partials; their updates rewrite them
+  if ((rc = prologue<ND, R>(b, k, !(mode == 1 && nsteps > 0)))) return rc;
   if (mode == 2) {
     hipLaunchKernelGGL(batch_dt_kernel, dim3((unsigned)b->active.size()), dim3(256), 0,
                        b->stream, b->dboxes, b->dactive, b->partial, b->dt, (double*)nullptr,
@@ -1201,15 +1204,11 @@ static int amr_check(rhy_batch_t* f, const BoxView& c, const int32_t* lo) {
   return RHY_OK;
 }
 
-int rhy_amr_prolong(rhy_box_t* cnew, rhy_box_t* cold, rhy_batch_t* f, const int32_t* lo,
-                    int32_t half) {
-  BoxView vn, vo;
+int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_t half) {
+  BoxView vn;
   int rc;
-  if (!cnew || !cold) return bfail(RHY_ERR_ARG, "NULL argument");
-  if ((rc = box_view(cnew, &vn)) || (rc = box_view(cold, &vo)))
-    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
-  if (std::memcmp(&vn.geom, &vo.geom, sizeof(Geom)) != 0 || vn.precision != vo.precision)
-    return bfail(RHY_ERR_SHAPE, "the two coarse states differ in shape or precision");
+  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vn))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
   if ((rc = amr_check(f, vn, lo))) return rc;
   for (int i = 0; i < (int)f->slots.size(); ++i) {
     const BBox& B = f->slots[i].B;
@@ -1224,8 +1223,7 @@ int rhy_amr_prolong(rhy_box_t* cnew, rhy_box_t* cold, rhy_batch_t* f, const int3
 #define RHY_PROLONG(ND, R)                                                                      \
   hipLaunchKernelGGL((amr_prolong_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0, f->stream, \
                      B.G, static_cast<R*>(f->arena) + B.cell_off, vn.geom,                     \
-                     static_cast<const R*>(vn.cells), static_cast<const R*>(vo.cells), l[0],   \
-                     l[1], l[2], half ? 1 : 0)
+                     static_cast<const R*>(vn.cells), l[0], l[1], l[2], half ? 1 : 0)
     if (f->precision == RHY_MIXED) {
       if (f->ndim == 1) RHY_PROLONG(1, float);
       else if (f->ndim == 2) RHY_PROLONG(2, float);

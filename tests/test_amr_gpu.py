@@ -222,7 +222,7 @@ def test_bad_placement_rejected_by_library():
     L = _L()
     for bad in ([1, 2, 1], [2, 13, 1]):
         lo = np.array(bad, dtype=np.int32)
-        rc = L.rhy_amr_prolong(h.coarse.handle, h.coarse_old.handle, h.fine._h,
+        rc = L.rhy_amr_prolong(h.coarse.handle, h.fine._h,
                                lo.ctypes.data_as(C.POINTER(C.c_int32)), 0)
         assert rc == _lib.RHY_ERR_SHAPE
         rc = L.rhy_amr_restrict(h.coarse.handle, h.fine._h,
