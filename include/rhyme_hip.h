@@ -114,6 +114,21 @@ int rhy_box_upload(rhy_box_t *box, const double *cells);
 int rhy_box_download(rhy_box_t *box, double *cells);
 /* Wait for all work queued on the box. */
 int rhy_box_sync(rhy_box_t *box);
+/* ---- snapshot payloads (the Chombo writer's data and the restart reader) ----
+ * rhyme_chombo_write_level_data (rhyme_chombo/src/rhyme_chombo_write_level_data.f90:41-71)
+ * writes per level one FP32 dataset "data:datatype=0": box after box, each
+ * box's components one after another, each component's interior cells with
+ * i fastest, converted with real() (round to nearest even).  The restart
+ * reader (rhyme_initial_condition/src/rhyme_initial_condition_load_rhyme.f90:57-76)
+ * widens them back into the boxes.  These calls produce and consume that
+ * payload on the device; the HDF5 container (groups, attributes, the boxes
+ * table) stays with the host's HDF5 library.
+ * rhy_box_snapshot_pack: data[ncmp * nx * ny * nz] from the box's interior.
+ * rhy_box_snapshot_unpack: the box's interior from such data (ghosts
+ *   untouched; fill them before stepping). */
+int rhy_box_snapshot_pack(rhy_box_t *box, float *data);
+int rhy_box_snapshot_unpack(rhy_box_t *box, const float *data);
+
 /* ---- the module procedures ---- */
 
 /* rhyme_samr_bc_set_boundaries (rhyme_samr_bc/src/rhyme_samr_bc_set_boundaries.f90:3-22):
@@ -301,6 +316,14 @@ int rhy_batch_run_dt_async(rhy_batch_t *batch, const rhy_params_t *p, double dt,
                            int32_t nsteps);
 int rhy_amr_prolong(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, int32_t half);
 int rhy_amr_restrict(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo);
+
+/* Snapshot payload of a whole level (a batch or pool): the active boxes in
+ * slot order, each laid out as rhy_box_snapshot_pack lays out one box; the
+ * length is in floats.  Unpacking marks every box for a ghost fill before
+ * its next step. */
+int rhy_batch_snapshot_length(rhy_batch_t *batch, int64_t *n);
+int rhy_batch_snapshot_pack(rhy_batch_t *batch, float *data);
+int rhy_batch_snapshot_unpack(rhy_batch_t *batch, const float *data);
 
 /* ---- pool: a batch whose boxes come and go (continuous batching) ----
  * No counterpart in the reference.  A pool holds a device arena of

@@ -37,6 +37,8 @@ EXPORTS = [
     "rhy_box_last_deferred", "rhy_batch_last_deferred", "rhy_debug_defer",
     "rhy_batch_cfl_dts", "rhy_batch_run_dt_async", "rhy_amr_prolong",
     "rhy_amr_restrict",
+    "rhy_box_snapshot_pack", "rhy_box_snapshot_unpack", "rhy_batch_snapshot_length",
+    "rhy_batch_snapshot_pack", "rhy_batch_snapshot_unpack",
 ]
 
 
@@ -162,6 +164,8 @@ def load():
         "rhy_box_diagnostics": (C.c_int, [vp, C.POINTER(RhyDiag)]),
         "rhy_box_last_deferred": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "rhy_debug_defer": (C.c_int, [C.c_int32, C.c_int64]),
+        "rhy_box_snapshot_pack": (C.c_int, [vp, C.POINTER(C.c_float)]),
+        "rhy_box_snapshot_unpack": (C.c_int, [vp, C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -197,6 +201,11 @@ def check(rc, what):
 
 def dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def iptr(a):

@@ -36,6 +36,9 @@ _BATCH_SIGS = {
                                          C.c_int32]),
     "rhy_amr_prolong": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_int32]),
     "rhy_amr_restrict": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]),
+    "rhy_batch_snapshot_length": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "rhy_batch_snapshot_pack": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "rhy_batch_snapshot_unpack": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "rhy_pool_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int64, C.c_int32,
                                   C.POINTER(C.c_void_p)]),
     "rhy_pool_box_bytes": (C.c_int, [C.c_int32, C.POINTER(C.c_int32), C.c_int32, C.c_int32,
@@ -157,6 +160,25 @@ class SamrBatch:
         _check(_L().rhy_batch_last_deferred(self._h, C.byref(n)), "rhy_batch_last_deferred")
         return n.value
 
+    def snapshot(self):
+        """FP32 snapshot payload of the level (rhy_batch_snapshot_pack): the
+        boxes in order, each as SamrBox.snapshot lays it out."""
+        n = C.c_int64(0)
+        _check(_L().rhy_batch_snapshot_length(self._h, C.byref(n)), "rhy_batch_snapshot_length")
+        out = np.empty(n.value, dtype=np.float32)
+        _check(_L().rhy_batch_snapshot_pack(self._h, _lib.fptr(out)), "rhy_batch_snapshot_pack")
+        return out
+
+    def load_snapshot(self, data):
+        """Every box's interior from a level payload (rhy_batch_snapshot_unpack)."""
+        n = C.c_int64(0)
+        _check(_L().rhy_batch_snapshot_length(self._h, C.byref(n)), "rhy_batch_snapshot_length")
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        if data.size != n.value:
+            raise ValueError("payload holds %d values, the level %d" % (data.size, n.value))
+        _check(_L().rhy_batch_snapshot_unpack(self._h, _lib.fptr(data)),
+               "rhy_batch_snapshot_unpack")
+
     def cfl_dts(self, params):
         """The CFL dt of every box for its current state (no step)."""
         out = np.zeros(self.n)
@@ -277,6 +299,9 @@ class SamrPool:
         cells = np.ascontiguousarray(cells, dtype=np.float64)
         assert cells.shape == box_shape(self.ndim, self.boxes[slot], self.ncmp)
         _check(_L().rhy_batch_upload(self._h, slot, dptr(cells)), "rhy_batch_upload")
+
+    snapshot = SamrBatch.snapshot            # the resident boxes in slot order
+    load_snapshot = SamrBatch.load_snapshot
 
     def set_tiles(self, block=256, nf_x=8, nf_yz=8, grid_cap=0):
         _check(_L().rhy_batch_set_tiles(self._h, block, nf_x, nf_yz, grid_cap),

@@ -70,6 +70,7 @@ struct rhy_box {
   uint32_t* defer_count = nullptr;  // faces the flux kernels left to the fixup
   uint64_t* defer_items = nullptr;
   uint32_t defer_cap = 0;
+  float* snap = nullptr;          // staging of the FP32 snapshot payload (lazy)
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
   bool profiling = false;
@@ -393,6 +394,8 @@ void free_box_memory(rhy_box* b) {
   if (b->partial) (void)hipFree(b->partial);
   if (b->defer_count) (void)hipFree(b->defer_count);
   if (b->defer_items) (void)hipFree(b->defer_items);
+  if (b->snap) (void)hipFree(b->snap);
+  b->snap = nullptr;
   b->cells = nullptr;
   b->flux = nullptr;
   b->partial = nullptr;
@@ -620,6 +623,52 @@ int rhy_box_last_deferred(rhy_box_t* b, int64_t* n) {
   HIP_TRY(hipMemcpy(&c, b->defer_count, sizeof(c), hipMemcpyDeviceToHost));
   *n = c;
   return RHY_OK;
+}
+
+// FP32 snapshot payload of the interior (pack) or from it (unpack), through a
+// device staging buffer that lives as long as the box's storage.
+static int box_snapshot(rhy_box_t* b, float* out, const float* in) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!out && !in) return fail(RHY_ERR_ARG, "data is NULL");
+  const int64_t blen = (int64_t)b->geom.n[0] * b->geom.n[1] * b->geom.n[2];
+  const size_t bytes = sizeof(float) * (size_t)blen * b->ncmp;
+  if (!b->snap && hipMalloc(&b->snap, bytes) != hipSuccess) {
+    b->snap = nullptr;
+    return fail(RHY_ERR_OOM, "snapshot staging allocation failed");
+  }
+  const dim3 grid(grid_for(blen), (unsigned)b->ncmp);
+  if (in) {
+    HIP_TRY(hipMemcpyAsync(b->snap, in, bytes, hipMemcpyHostToDevice, b->stream));
+    if (b->precision == RHY_MIXED)
+      hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
+                         b->geom, static_cast<float*>(b->cells), b->snap);
+    else
+      hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
+                         b->geom, static_cast<double*>(b->cells), b->snap);
+    HIP_TRY(hipGetLastError());
+  } else {
+    if (b->precision == RHY_MIXED)
+      hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
+                         b->geom, static_cast<const float*>(b->cells), b->snap);
+    else
+      hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
+                         b->geom, static_cast<const double*>(b->cells), b->snap);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, b->snap, bytes, hipMemcpyDeviceToHost, b->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_box_snapshot_pack(rhy_box_t* b, float* data) {
+  if (!data) return fail(RHY_ERR_ARG, "data is NULL");
+  return box_snapshot(b, data, nullptr);
+}
+
+int rhy_box_snapshot_unpack(rhy_box_t* b, const float* data) {
+  if (!data) return fail(RHY_ERR_ARG, "data is NULL");
+  return box_snapshot(b, nullptr, data);
 }
 
 constexpr int kDiagBlocks = 256;

@@ -459,6 +459,8 @@ struct rhy_batch {
   uint32_t* defer_count = nullptr;  // faces the flux phase left to the fixup
   uint64_t* defer_items = nullptr;
   uint32_t defer_cap = 0;
+  float* snap = nullptr;  // staging of the FP32 snapshot payload (lazy, grows)
+  int64_t snap_elems = 0;
 };
 
 namespace {
@@ -1050,7 +1052,7 @@ int rhy_batch_destroy(rhy_batch_t* b) {
   for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow, (void*)b->diag, (void*)b->defer_count,
-                  (void*)b->defer_items})
+                  (void*)b->defer_items, (void*)b->snap})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1085,6 +1087,84 @@ int rhy_batch_upload(rhy_batch_t* b, int32_t box, const double* cells) {
                           sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
   BHIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
+}
+
+// Snapshot payload of the whole level: the active boxes in slot order, each
+// its components' interior cells (i fastest) one component after another, as
+// FP32 -- a level's data:datatype=0 dataset (rhyme_chombo_write_level_data.f90:41-71);
+// unpack is the restart reader's half (rhyme_initial_condition_load_rhyme.f90:57-76).
+// (The slots are walked directly: the active list is only rebuilt by a run.)
+static int64_t snapshot_elems(const rhy_batch_t* b) {
+  int64_t n = 0;
+  for (const Slot& S : b->slots) {
+    if (!S.active) continue;
+    const Geom& G = S.B.G;
+    n += (int64_t)G.n[0] * G.n[1] * G.n[2] * b->ncmp;
+  }
+  return n;
+}
+
+static int batch_snapshot(rhy_batch_t* b, float* out, const float* in) {
+  if (!b || (!out && !in)) return bfail(RHY_ERR_ARG, "NULL argument");
+  const int64_t total = snapshot_elems(b);
+  if (total == 0) return RHY_OK;
+  if (total > b->snap_elems) {
+    BHIP_TRY(hipStreamSynchronize(b->stream));
+    if (b->snap) (void)hipFree(b->snap);
+    b->snap = nullptr;
+    b->snap_elems = 0;
+    if (hipMalloc(&b->snap, sizeof(float) * (size_t)total) != hipSuccess) {
+      b->snap = nullptr;
+      return bfail(RHY_ERR_OOM, "snapshot staging allocation failed");
+    }
+    b->snap_elems = total;
+  }
+  if (in)
+    BHIP_TRY(hipMemcpyAsync(b->snap, in, sizeof(float) * (size_t)total, hipMemcpyHostToDevice,
+                            b->stream));
+  int64_t ofs = 0;
+  for (Slot& S : b->slots) {
+    if (!S.active) continue;
+    const Geom& G = S.B.G;
+    const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
+    const dim3 grid(cdiv(blen, 256), (unsigned)b->ncmp);
+    if (in) {
+      S.needs_bc = true;
+      if (b->precision == RHY_MIXED)
+        hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(256), 0, b->stream, G,
+                           static_cast<float*>(b->arena) + S.B.cell_off, b->snap + ofs);
+      else
+        hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(256), 0, b->stream, G,
+                           static_cast<double*>(b->arena) + S.B.cell_off, b->snap + ofs);
+    } else {
+      if (b->precision == RHY_MIXED)
+        hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(256), 0, b->stream, G,
+                           static_cast<const float*>(b->arena) + S.B.cell_off, b->snap + ofs);
+      else
+        hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(256), 0, b->stream, G,
+                           static_cast<const double*>(b->arena) + S.B.cell_off, b->snap + ofs);
+    }
+    ofs += blen * b->ncmp;
+  }
+  BHIP_TRY(hipGetLastError());
+  if (in) b->partials_valid = false;
+  if (out)
+    BHIP_TRY(hipMemcpyAsync(out, b->snap, sizeof(float) * (size_t)total, hipMemcpyDeviceToHost,
+                            b->stream));
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_batch_snapshot_length(rhy_batch_t* b, int64_t* n) {
+  if (!b || !n) return bfail(RHY_ERR_ARG, "NULL argument");
+  *n = snapshot_elems(b);
+  return RHY_OK;
+}
+
+int rhy_batch_snapshot_pack(rhy_batch_t* b, float* data) { return batch_snapshot(b, data, nullptr); }
+
+int rhy_batch_snapshot_unpack(rhy_batch_t* b, const float* data) {
+  return batch_snapshot(b, nullptr, data);
 }
 
 int rhy_batch_download(rhy_batch_t* b, int32_t box, double* cells) {

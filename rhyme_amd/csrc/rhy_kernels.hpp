@@ -1093,4 +1093,34 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(Consts k, int64_t n,
   for (int c = 0; c < NV; ++c) f[q * NV + c] = Fo[c];
 }
 
+// Snapshot payload of one box: every component's interior cells, i fastest,
+// component after component, as FP32 (round to nearest even) -- the layout
+// of a level's data:datatype=0 dataset in the reference's Chombo writer
+// (rhyme_chombo_write_level_data.f90:41-71).  blockIdx.y is the component.
+template <typename R>
+__global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, const R* __restrict__ cells,
+                                                            float* __restrict__ out) {
+  const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= blen) return;
+  const int v = (int)blockIdx.y;
+  const int i = (int)(t % G.n[0]), j = (int)(t / G.n[0] % G.n[1]),
+            k = (int)(t / ((int64_t)G.n[0] * G.n[1]));
+  out[v * blen + t] = (float)cells[goff(G, i + 1, j + 1, k + 1) + v * G.plane];
+}
+
+// The reverse (the restart reader, rhyme_initial_condition_load_rhyme.f90:57-76):
+// interior cells from an FP32 payload, widened exactly; ghosts untouched.
+template <typename R>
+__global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, R* __restrict__ cells,
+                                                              const float* __restrict__ in) {
+  const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= blen) return;
+  const int v = (int)blockIdx.y;
+  const int i = (int)(t % G.n[0]), j = (int)(t / G.n[0] % G.n[1]),
+            k = (int)(t / ((int64_t)G.n[0] * G.n[1]));
+  cells[goff(G, i + 1, j + 1, k + 1) + v * G.plane] = (R)in[v * blen + t];
+}
+
 }  // namespace rhy

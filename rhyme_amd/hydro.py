@@ -194,6 +194,25 @@ class SamrBox:
         check(_lib.load().rhy_box_download(self._h, dptr(self.cells)), "rhy_box_download")
         return self.cells
 
+    def snapshot(self):
+        """FP32 snapshot payload of the box (rhy_box_snapshot_pack): every
+        component's interior cells, i fastest, one component after another."""
+        n = self.ncmp * int(np.prod(self.dims))
+        out = np.empty(n, dtype=np.float32)
+        check(_lib.load().rhy_box_snapshot_pack(self._h, _lib.fptr(out)),
+              "rhy_box_snapshot_pack")
+        return out
+
+    def load_snapshot(self, data):
+        """The interior from an FP32 snapshot payload (rhy_box_snapshot_unpack);
+        ghosts are left as they are."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        if data.size != self.ncmp * int(np.prod(self.dims)):
+            raise ValueError("payload holds %d values, the box %d" %
+                             (data.size, self.ncmp * int(np.prod(self.dims))))
+        check(_lib.load().rhy_box_snapshot_unpack(self._h, _lib.fptr(data)),
+              "rhy_box_snapshot_unpack")
+
     def interior(self, var=None):
         sl = tuple(slice(2, 2 + n) for n in self.dims[::-1])
         if var is None:
