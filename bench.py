@@ -161,6 +161,34 @@ def amr_leg(n, steps, warmup, seed):
     return out
 
 
+def snapshot_leg(n, seed, reps=5):
+    """Snapshot payload of the n^3 blast (FP64 state): the FP32 interior,
+    component after component, packed on the device and copied to pageable
+    host memory (rhy_box_snapshot_pack), then unpacked into a box again.
+    Whole calls are timed (kernel, PCIe copy and waits); the kernel alone is
+    in the rocprofv3 summary."""
+    import numpy as np
+    from rhyme_amd import hydro, ics
+    box = hydro.SamrBox(3, [n] * 3, cells=ics.blast_3d(n=n, seed=seed))
+    box.upload()
+    data = box.snapshot()
+    pack, unpack = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        data = box.snapshot()
+        pack.append(time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        box.load_snapshot(data)
+        unpack.append(time.perf_counter() - t0)
+    box.close()
+    mb = data.nbytes / 1e6
+    return dict(payload_mb=mb, components=int(data.size // n ** 3),
+                pack_call_ms=1e3 * float(np.median(pack)),
+                unpack_call_ms=1e3 * float(np.median(unpack)),
+                pack_call_gbs=mb / 1e3 / float(np.median(pack)),
+                note="median of %d calls, pageable host buffer, PCIe-inclusive" % reps)
+
+
 def cpu_leg(n_sample, steps_sample, seed, threads_all):
     """The oracle on the host cores: both reference loop structures with all
     threads, plus memory_intensive on one thread (at a quarter of the edge),
@@ -256,6 +284,8 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=50)
     ap.add_argument("--skip-amr", action="store_true", help="omit the two-level hierarchy leg")
     ap.add_argument("--amr-steps", type=int, default=20)
+    ap.add_argument("--skip-snapshot", action="store_true",
+                    help="omit the snapshot payload pack/unpack leg")
     ap.add_argument("--skip-serve", action="store_true", help="omit the serving-stream leg")
     ap.add_argument("--serve-jobs", type=int, default=1000)
     ap.add_argument("--serve-rate", type=float, default=40.0, help="Poisson arrivals, jobs/s")
@@ -338,6 +368,8 @@ def main():
         line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
     if not a.skip_amr:
         line["amr"] = amr_leg(a.n, a.amr_steps, min(a.warmup, 2), a.seed)
+    if not a.skip_snapshot:
+        line["snapshot"] = snapshot_leg(a.n, a.seed)
     if not a.skip_serve:
         line["serving"] = serving_leg(a.serve_jobs, a.serve_rate, a.serve_budget_gib,
                                       a.serve_quantum, a.seed)

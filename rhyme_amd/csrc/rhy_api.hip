@@ -637,23 +637,23 @@ static int box_snapshot(rhy_box_t* b, float* out, const float* in) {
     b->snap = nullptr;
     return fail(RHY_ERR_OOM, "snapshot staging allocation failed");
   }
-  const dim3 grid(grid_for(blen), (unsigned)b->ncmp);
+  const dim3 grid = snapshot_grid(b->geom);
   if (in) {
     HIP_TRY(hipMemcpyAsync(b->snap, in, bytes, hipMemcpyHostToDevice, b->stream));
     if (b->precision == RHY_MIXED)
       hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, static_cast<float*>(b->cells), b->snap);
+                         b->geom, b->ncmp, static_cast<float*>(b->cells), b->snap);
     else
       hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, static_cast<double*>(b->cells), b->snap);
+                         b->geom, b->ncmp, static_cast<double*>(b->cells), b->snap);
     HIP_TRY(hipGetLastError());
   } else {
     if (b->precision == RHY_MIXED)
       hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, static_cast<const float*>(b->cells), b->snap);
+                         b->geom, b->ncmp, static_cast<const float*>(b->cells), b->snap);
     else
       hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, static_cast<const double*>(b->cells), b->snap);
+                         b->geom, b->ncmp, static_cast<const double*>(b->cells), b->snap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, b->snap, bytes, hipMemcpyDeviceToHost, b->stream));
   }

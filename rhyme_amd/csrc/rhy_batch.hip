@@ -323,18 +323,47 @@ __global__ __launch_bounds__(256) void batch_set_dt_kernel(const int32_t* __rest
 // lo[d] + (f + 1) / 2 - 1 (f >= -1).
 __device__ __forceinline__ int coarse_parent(int lo, int f) { return lo + (f + 1) / 2 - 1; }
 
-// Face ghosts (2 layers, faces only: what the flux walks read) of one fine
-// box from its coarse parents, piecewise constant in space.  half == 0: the
-// parent's value (called before the coarse step, so the start of the coarse
-// step).  half == 1: the mean of the ghost's value and the parent's (called
-// after the coarse step and the first fine substep, which leaves the fine
-// boxes' face ghosts as they were), i.e. 0.5 * U_old + 0.5 * U_new.
+// One work item of a hierarchy launch, which covers all fine boxes at once:
+// a fine box (prolongation, restriction) or a region of fine ghosts inside
+// a sibling (copy).  Item i takes threads [first_i, first_{i+1}).
+struct AmrItem {
+  int64_t first;
+  int64_t dst_off;  // arena offset of the fine box written (prolong, copy) or read (restrict)
+  int64_t src_off;  // arena offset of the sibling read (copy)
+  Geom GD, GS;      // their geometries
+  int32_t lo[3];    // placement (prolong, restrict); first dst cell (copy)
+  int32_t slo[3];   // first src cell (copy)
+  int32_t ext[3];   // region extent (copy)
+};
+
+__device__ __forceinline__ int amr_item_of(const AmrItem* __restrict__ items, int n, int64_t t) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (items[mid].first <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Face ghosts (2 layers, faces only: what the flux walks read) of every fine
+// box from their coarse parents, piecewise constant in space.  half == 0:
+// the parent's value (called before the coarse step, so the start of the
+// coarse step).  half == 1: the mean of the ghost's value and the parent's
+// (called after the coarse step and the first fine substep, which leaves
+// the fine boxes' face ghosts as they were), i.e. 0.5 * U_old + 0.5 * U_new.
 template <int ND, typename R>
-__global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict__ fine, Geom GC,
-                                                          const R* __restrict__ coarse, int lo0,
-                                                          int lo1, int lo2, int half) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lo[3] = {lo0, lo1, lo2};
+__global__ __launch_bounds__(256) void amr_prolong_kernel(const AmrItem* __restrict__ items,
+                                                          int nitems, int64_t total,
+                                                          R* __restrict__ arena, Geom GC,
+                                                          const R* __restrict__ coarse,
+                                                          int half) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
+  const Geom& GF = it.GD;
+  R* fine = arena + it.dst_off;
+  int64_t t = t0 - it.first;
   for (int a = 0; a < ND; ++a) {
     int64_t m = 4;  // 2 sides x 2 layers
     for (int d = 0; d < 3; ++d)
@@ -355,7 +384,7 @@ __global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict
     const int side = sl >> 1, layer = sl & 1;
     idx[a] = side ? GF.n[a] + 1 + layer : -layer;
     int pc[3] = {1, 1, 1};
-    for (int d = 0; d < ND; ++d) pc[d] = coarse_parent(lo[d], idx[d]);
+    for (int d = 0; d < ND; ++d) pc[d] = coarse_parent(it.lo[d], idx[d]);
     const int64_t of = goff(GF, idx[0], idx[1], idx[2]);
     const int64_t oc = goff(GC, pc[0], pc[1], pc[2]);
 #pragma unroll
@@ -368,34 +397,42 @@ __global__ __launch_bounds__(256) void amr_prolong_kernel(Geom GF, R* __restrict
   }
 }
 
-// Cells of a rectangular region copied from one box to another (fine ghosts
-// that lie in a sibling's interior): ext[d] cells from src index slo[d] to
-// dst index dlo[d], the ND + 2 hydro components.
+// Regions of fine ghosts that lie in a sibling's interior, copied from it:
+// ext[d] cells from src cell slo[d] to dst cell lo[d], the ND + 2 hydro
+// components.
 template <int ND, typename R>
-__global__ __launch_bounds__(256) void amr_copy_region_kernel(Geom GD, R* __restrict__ dst,
-                                                              Geom GS, const R* __restrict__ src,
-                                                              int3 dlo, int3 slo, int3 ext) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)ext.x * ext.y * ext.z) return;
-  const int x = (int)(t % ext.x), y = (int)(t / ext.x % ext.y),
-            z = (int)(t / ((int64_t)ext.x * ext.y));
-  const int64_t od = goff(GD, dlo.x + x, dlo.y + y, dlo.z + z);
-  const int64_t os = goff(GS, slo.x + x, slo.y + y, slo.z + z);
+__global__ __launch_bounds__(256) void amr_copy_kernel(const AmrItem* __restrict__ items,
+                                                       int nitems, int64_t total,
+                                                       R* __restrict__ arena) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
+  const int64_t t = t0 - it.first;
+  const int x = (int)(t % it.ext[0]), y = (int)(t / it.ext[0] % it.ext[1]),
+            z = (int)(t / ((int64_t)it.ext[0] * it.ext[1]));
+  const int64_t od = it.dst_off + goff(it.GD, it.lo[0] + x, it.lo[1] + y, it.lo[2] + z);
+  const int64_t os = it.src_off + goff(it.GS, it.slo[0] + x, it.slo[1] + y, it.slo[2] + z);
 #pragma unroll
-  for (int c = 0; c < ND + 2; ++c) dst[od + c * GD.plane] = src[os + c * GS.plane];
+  for (int c = 0; c < ND + 2; ++c) arena[od + c * it.GD.plane] = arena[os + c * it.GS.plane];
 }
 
-// The coarse cells a fine box covers take the mean of their 2^ND children:
-// the children summed i fastest, then times 2^-ND.
+// The coarse cells every fine box covers take the mean of their 2^ND
+// children: the children summed i fastest, then times 2^-ND.
 template <int ND, typename R>
-__global__ __launch_bounds__(256) void amr_restrict_kernel(Geom GF, const R* __restrict__ fine,
-                                                           Geom GC, R* __restrict__ coarse,
-                                                           int lo0, int lo1, int lo2) {
-  const int m0 = GF.n[0] / 2, m1 = ND > 1 ? GF.n[1] / 2 : 1, m2 = ND > 2 ? GF.n[2] / 2 : 1;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (int64_t)m0 * m1 * m2) return;
+__global__ __launch_bounds__(256) void amr_restrict_kernel(const AmrItem* __restrict__ items,
+                                                           int nitems, int64_t total,
+                                                           const R* __restrict__ arena, Geom GC,
+                                                           R* __restrict__ coarse) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
+  const Geom& GF = it.GD;
+  const R* fine = arena + it.dst_off;
+  const int64_t t = t0 - it.first;
+  const int m0 = GF.n[0] / 2, m1 = ND > 1 ? GF.n[1] / 2 : 1;
   const int c0 = (int)(t % m0), c1 = (int)(t / m0 % m1), c2 = (int)(t / ((int64_t)m0 * m1));
-  const int64_t oc = goff(GC, lo0 + c0, ND > 1 ? lo1 + c1 : 1, ND > 2 ? lo2 + c2 : 1);
+  const int64_t oc = goff(GC, it.lo[0] + c0, ND > 1 ? it.lo[1] + c1 : 1,
+                          ND > 2 ? it.lo[2] + c2 : 1);
   const R w = ND == 3 ? R(0.125) : ND == 2 ? R(0.25) : R(0.5);
 #pragma unroll
   for (int c = 0; c < ND + 2; ++c) {
@@ -461,6 +498,9 @@ struct rhy_batch {
   uint32_t defer_cap = 0;
   float* snap = nullptr;  // staging of the FP32 snapshot payload (lazy, grows)
   int64_t snap_elems = 0;
+  AmrItem* amr_items = nullptr;      // work items of the hierarchy launches (lazy, grows)
+  int64_t amr_cap = 0;
+  std::vector<AmrItem> amr_host;     // their host copy, kept until the stream is idle
 };
 
 namespace {
@@ -1052,7 +1092,7 @@ int rhy_batch_destroy(rhy_batch_t* b) {
   for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
                   (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow, (void*)b->diag, (void*)b->defer_count,
-                  (void*)b->defer_items, (void*)b->snap})
+                  (void*)b->defer_items, (void*)b->snap, (void*)b->amr_items})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1127,21 +1167,21 @@ static int batch_snapshot(rhy_batch_t* b, float* out, const float* in) {
     if (!S.active) continue;
     const Geom& G = S.B.G;
     const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
-    const dim3 grid(cdiv(blen, 256), (unsigned)b->ncmp);
+    const dim3 grid = snapshot_grid(G);
     if (in) {
       S.needs_bc = true;
       if (b->precision == RHY_MIXED)
-        hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(256), 0, b->stream, G,
+        hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(256), 0, b->stream, G, b->ncmp,
                            static_cast<float*>(b->arena) + S.B.cell_off, b->snap + ofs);
       else
-        hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(256), 0, b->stream, G,
+        hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(256), 0, b->stream, G, b->ncmp,
                            static_cast<double*>(b->arena) + S.B.cell_off, b->snap + ofs);
     } else {
       if (b->precision == RHY_MIXED)
-        hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(256), 0, b->stream, G,
+        hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(256), 0, b->stream, G, b->ncmp,
                            static_cast<const float*>(b->arena) + S.B.cell_off, b->snap + ofs);
       else
-        hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(256), 0, b->stream, G,
+        hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(256), 0, b->stream, G, b->ncmp,
                            static_cast<const double*>(b->arena) + S.B.cell_off, b->snap + ofs);
     }
     ofs += blen * b->ncmp;
@@ -1284,42 +1324,56 @@ static int amr_check(rhy_batch_t* f, const BoxView& c, const int32_t* lo) {
   return RHY_OK;
 }
 
+// The host items to the device (one copy, in stream order).
+static int amr_upload(rhy_batch_t* f) {
+  const int64_t n = (int64_t)f->amr_host.size();
+  if (n > f->amr_cap) {
+    BHIP_TRY(hipStreamSynchronize(f->stream));
+    if (f->amr_items) (void)hipFree(f->amr_items);
+    f->amr_items = nullptr;
+    f->amr_cap = 0;
+    if (hipMalloc(&f->amr_items, sizeof(AmrItem) * (size_t)n) != hipSuccess) {
+      f->amr_items = nullptr;
+      return bfail(RHY_ERR_OOM, "hierarchy work items allocation failed");
+    }
+    f->amr_cap = n;
+  }
+  if (n)
+    BHIP_TRY(hipMemcpyAsync(f->amr_items, f->amr_host.data(), sizeof(AmrItem) * (size_t)n,
+                            hipMemcpyHostToDevice, f->stream));
+  return RHY_OK;
+}
+
 int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_t half) {
   BoxView vn;
   int rc;
   if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
   if ((rc = box_view(coarse, &vn))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
   if ((rc = amr_check(f, vn, lo))) return rc;
-  for (int i = 0; i < (int)f->slots.size(); ++i) {
+  const int nd = f->ndim, nb = (int)f->slots.size();
+  f->amr_host.clear();
+  // items [0, nb): one per fine box, its face ghost cells
+  int64_t nprol = 0;
+  for (int i = 0; i < nb; ++i) {
     const BBox& B = f->slots[i].B;
-    int64_t n = 0;
-    for (int a = 0; a < f->ndim; ++a) {
+    AmrItem it{};
+    it.first = nprol;
+    it.dst_off = B.cell_off;
+    it.GD = B.G;
+    for (int d = 0; d < 3; ++d) it.lo[d] = lo[3 * i + d];
+    f->amr_host.push_back(it);
+    for (int a = 0; a < nd; ++a) {
       int64_t m = 4;
       for (int d = 0; d < 3; ++d)
         if (d != a) m *= B.G.n[d];
-      n += m;
+      nprol += m;
     }
-    const int* l = lo + 3 * i;
-#define RHY_PROLONG(ND, R)                                                                      \
-  hipLaunchKernelGGL((amr_prolong_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0, f->stream, \
-                     B.G, static_cast<R*>(f->arena) + B.cell_off, vn.geom,                     \
-                     static_cast<const R*>(vn.cells), l[0], l[1], l[2], half ? 1 : 0)
-    if (f->precision == RHY_MIXED) {
-      if (f->ndim == 1) RHY_PROLONG(1, float);
-      else if (f->ndim == 2) RHY_PROLONG(2, float);
-      else RHY_PROLONG(3, float);
-    } else {
-      if (f->ndim == 1) RHY_PROLONG(1, double);
-      else if (f->ndim == 2) RHY_PROLONG(2, double);
-      else RHY_PROLONG(3, double);
-    }
-#undef RHY_PROLONG
   }
-  // Face ghosts inside a sibling's interior take the sibling's cells instead
-  // (same level, same time).  In fine-level indices box i's cell f lies at
-  // 2 (lo_i - 1) + f; interiors never overlap, so each ghost has at most one
-  // sibling source, and the copies run after every coarse fill above.
-  const int nd = f->ndim, nb = (int)f->slots.size();
+  // items [nb, ...): face ghosts inside a sibling's interior take the
+  // sibling's cells (same level, same time).  In fine-level indices box i's
+  // cell f lies at 2 (lo_i - 1) + f; interiors never overlap, so each ghost
+  // has at most one sibling source; the copies run after the coarse fill.
+  int64_t ncopy = 0;
   for (int i = 0; i < nb; ++i) {
     const BBox& D = f->slots[i].B;
     for (int a = 0; a < nd; ++a)
@@ -1342,38 +1396,54 @@ int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_
             empty = empty || g0[d] > g1[d];
           }
           if (empty) continue;
-          int3 dlo, slo, ext;
-          int* dl = &dlo.x;
-          int* sl = &slo.x;
-          int* ex = &ext.x;
+          AmrItem it{};
+          it.first = ncopy;
+          it.dst_off = D.cell_off;
+          it.src_off = S.cell_off;
+          it.GD = D.G;
+          it.GS = S.G;
+          int64_t vol = 1;
           for (int d = 0; d < 3; ++d) {
             const int bi = d < nd ? 2 * (lo[3 * i + d] - 1) : 0;
             const int bj = d < nd ? 2 * (lo[3 * j + d] - 1) : 0;
-            dl[d] = g0[d] - bi;
-            sl[d] = g0[d] - bj;
-            ex[d] = g1[d] - g0[d] + 1;
+            it.lo[d] = g0[d] - bi;
+            it.slo[d] = g0[d] - bj;
+            it.ext[d] = g1[d] - g0[d] + 1;
             // dst within its ghost layers, src within its interior
-            if (dl[d] < 1 - D.G.g[d] || dl[d] + ex[d] - 1 > D.G.n[d] + D.G.g[d] || sl[d] < 1 ||
-                sl[d] + ex[d] - 1 > S.G.n[d])
+            if (it.lo[d] < 1 - D.G.g[d] || it.lo[d] + it.ext[d] - 1 > D.G.n[d] + D.G.g[d] ||
+                it.slo[d] < 1 || it.slo[d] + it.ext[d] - 1 > S.G.n[d])
               return bfail(RHY_ERR_SHAPE, "sibling ghost region outside a box");
+            vol *= it.ext[d];
           }
-          const int64_t n = (int64_t)ext.x * ext.y * ext.z;
-#define RHY_SIBLING(ND, R)                                                                      \
-  hipLaunchKernelGGL((amr_copy_region_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0,        \
-                     f->stream, D.G, static_cast<R*>(f->arena) + D.cell_off, S.G,              \
-                     static_cast<const R*>(f->arena) + S.cell_off, dlo, slo, ext)
-          if (f->precision == RHY_MIXED) {
-            if (nd == 1) RHY_SIBLING(1, float);
-            else if (nd == 2) RHY_SIBLING(2, float);
-            else RHY_SIBLING(3, float);
-          } else {
-            if (nd == 1) RHY_SIBLING(1, double);
-            else if (nd == 2) RHY_SIBLING(2, double);
-            else RHY_SIBLING(3, double);
-          }
-#undef RHY_SIBLING
+          f->amr_host.push_back(it);
+          ncopy += vol;
         }
   }
+  if ((rc = amr_upload(f))) return rc;
+  const int ncitems = (int)f->amr_host.size() - nb;
+  const AmrItem* items = f->amr_items;
+#define RHY_AMR_FILL(ND, R)                                                                    \
+  do {                                                                                         \
+    R* arena = static_cast<R*>(f->arena);                                                      \
+    hipLaunchKernelGGL((amr_prolong_kernel<ND, R>), dim3(cdiv(nprol, 256)), dim3(256), 0,     \
+                       f->stream, items, nb, nprol, arena, vn.geom,                            \
+                       static_cast<const R*>(vn.cells), half ? 1 : 0);                         \
+    if (ncopy)                                                                                 \
+      hipLaunchKernelGGL((amr_copy_kernel<ND, R>), dim3(cdiv(ncopy, 256)), dim3(256), 0,      \
+                         f->stream, items + nb, ncitems, ncopy, arena);                        \
+  } while (0)
+  if (nb) {
+    if (f->precision == RHY_MIXED) {
+      if (nd == 1) RHY_AMR_FILL(1, float);
+      else if (nd == 2) RHY_AMR_FILL(2, float);
+      else RHY_AMR_FILL(3, float);
+    } else {
+      if (nd == 1) RHY_AMR_FILL(1, double);
+      else if (nd == 2) RHY_AMR_FILL(2, double);
+      else RHY_AMR_FILL(3, double);
+    }
+  }
+#undef RHY_AMR_FILL
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));
   return RHY_OK;
@@ -1385,26 +1455,38 @@ int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
   if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
   if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
   if ((rc = amr_check(f, vc, lo))) return rc;
-  for (int i = 0; i < (int)f->slots.size(); ++i) {
+  const int nd = f->ndim, nb = (int)f->slots.size();
+  f->amr_host.clear();
+  int64_t total = 0;
+  for (int i = 0; i < nb; ++i) {
     const BBox& B = f->slots[i].B;
+    AmrItem it{};
+    it.first = total;
+    it.dst_off = B.cell_off;
+    it.GD = B.G;
+    for (int d = 0; d < 3; ++d) it.lo[d] = lo[3 * i + d];
+    f->amr_host.push_back(it);
     int64_t n = 1;
-    for (int d = 0; d < f->ndim; ++d) n *= B.G.n[d] / 2;
-    const int* l = lo + 3 * i;
+    for (int d = 0; d < nd; ++d) n *= B.G.n[d] / 2;
+    total += n;
+  }
+  if ((rc = amr_upload(f))) return rc;
 #define RHY_RESTRICT(ND, R)                                                                      \
-  hipLaunchKernelGGL((amr_restrict_kernel<ND, R>), dim3(cdiv(n, 256)), dim3(256), 0, f->stream, \
-                     B.G, static_cast<const R*>(f->arena) + B.cell_off, vc.geom,                \
-                     static_cast<R*>(vc.cells), l[0], l[1], l[2])
+  hipLaunchKernelGGL((amr_restrict_kernel<ND, R>), dim3(cdiv(total, 256)), dim3(256), 0,        \
+                     f->stream, f->amr_items, nb, total, static_cast<const R*>(f->arena),        \
+                     vc.geom, static_cast<R*>(vc.cells))
+  if (total) {
     if (f->precision == RHY_MIXED) {
-      if (f->ndim == 1) RHY_RESTRICT(1, float);
-      else if (f->ndim == 2) RHY_RESTRICT(2, float);
+      if (nd == 1) RHY_RESTRICT(1, float);
+      else if (nd == 2) RHY_RESTRICT(2, float);
       else RHY_RESTRICT(3, float);
     } else {
-      if (f->ndim == 1) RHY_RESTRICT(1, double);
-      else if (f->ndim == 2) RHY_RESTRICT(2, double);
+      if (nd == 1) RHY_RESTRICT(1, double);
+      else if (nd == 2) RHY_RESTRICT(2, double);
       else RHY_RESTRICT(3, double);
     }
-#undef RHY_RESTRICT
   }
+#undef RHY_RESTRICT
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));
   return RHY_OK;

@@ -1096,31 +1096,54 @@ __global__ __launch_bounds__(256) void flux_batch_kernel(Consts k, int64_t n,
 // Snapshot payload of one box: every component's interior cells, i fastest,
 // component after component, as FP32 (round to nearest even) -- the layout
 // of a level's data:datatype=0 dataset in the reference's Chombo writer
-// (rhyme_chombo_write_level_data.f90:41-71).  blockIdx.y is the component.
+// (rhyme_chombo_write_level_data.f90:41-71).  A workgroup covers 256 cells
+// of one interior row (snapshot_grid); each thread moves all components of
+// its cell, so every thread has ncmp loads in flight.
+inline dim3 snapshot_grid(const Geom& G) {
+  const unsigned chunks = (unsigned)((G.n[0] + 255) / 256);
+  return dim3(chunks * (unsigned)G.n[1] * (unsigned)G.n[2]);
+}
+
 template <typename R>
-__global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, const R* __restrict__ cells,
+__global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, int ncmp,
+                                                            const R* __restrict__ cells,
                                                             float* __restrict__ out) {
+  const int chunks = (G.n[0] + 255) >> 8;
+  const int row = (int)blockIdx.x / chunks;
+  const int i = ((int)blockIdx.x - row * chunks) * 256 + (int)threadIdx.x;
+  if (i >= G.n[0]) return;
+  const int j = row % G.n[1], k = row / G.n[1];
   const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= blen) return;
-  const int v = (int)blockIdx.y;
-  const int i = (int)(t % G.n[0]), j = (int)(t / G.n[0] % G.n[1]),
-            k = (int)(t / ((int64_t)G.n[0] * G.n[1]));
-  out[v * blen + t] = (float)cells[goff(G, i + 1, j + 1, k + 1) + v * G.plane];
+  const int64_t src = goff(G, i + 1, j + 1, k + 1), dst = (int64_t)row * G.n[0] + i;
+  R v[RHY_MAX_CMP];
+#pragma unroll
+  for (int c = 0; c < RHY_MAX_CMP; ++c)
+    if (c < ncmp) v[c] = cells[src + c * G.plane];
+#pragma unroll
+  for (int c = 0; c < RHY_MAX_CMP; ++c)
+    if (c < ncmp) out[c * blen + dst] = (float)v[c];
 }
 
 // The reverse (the restart reader, rhyme_initial_condition_load_rhyme.f90:57-76):
 // interior cells from an FP32 payload, widened exactly; ghosts untouched.
 template <typename R>
-__global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, R* __restrict__ cells,
+__global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, int ncmp,
+                                                              R* __restrict__ cells,
                                                               const float* __restrict__ in) {
+  const int chunks = (G.n[0] + 255) >> 8;
+  const int row = (int)blockIdx.x / chunks;
+  const int i = ((int)blockIdx.x - row * chunks) * 256 + (int)threadIdx.x;
+  if (i >= G.n[0]) return;
+  const int j = row % G.n[1], k = row / G.n[1];
   const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= blen) return;
-  const int v = (int)blockIdx.y;
-  const int i = (int)(t % G.n[0]), j = (int)(t / G.n[0] % G.n[1]),
-            k = (int)(t / ((int64_t)G.n[0] * G.n[1]));
-  cells[goff(G, i + 1, j + 1, k + 1) + v * G.plane] = (R)in[v * blen + t];
+  const int64_t dst = goff(G, i + 1, j + 1, k + 1), src = (int64_t)row * G.n[0] + i;
+  float v[RHY_MAX_CMP];
+#pragma unroll
+  for (int c = 0; c < RHY_MAX_CMP; ++c)
+    if (c < ncmp) v[c] = in[c * blen + src];
+#pragma unroll
+  for (int c = 0; c < RHY_MAX_CMP; ++c)
+    if (c < ncmp) cells[dst + c * G.plane] = (R)v[c];
 }
 
 }  // namespace rhy
