@@ -205,6 +205,21 @@ def cpu_leg(n_sample, steps_sample, seed, threads_all):
     }
 
 
+def _host_cpu():
+    """nproc and the CPU model of the host the baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(nproc=os.cpu_count(), model=model,
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"))
+
+
 def _cpu_subprocess(n, steps, seed, solver, threads):
     """One warm-up main-loop step of the blast from its initial condition,
     then `steps` single steps each timed on its own; the rate is the median
@@ -390,7 +405,7 @@ def main():
             value=m["zone_updates_per_s"], unit="zone-updates/s", cores=m["threads"],
             kind="port", sample=m["sample"] + " (memory_intensive, all threads)",
             cpu_intensive=c["cpu_intensive"],
-            one_thread=c["memory_intensive_1thread"])
+            one_thread=c["memory_intensive_1thread"], host=_host_cpu())
     print(json.dumps(line), flush=True)
     return 0
 
