@@ -78,10 +78,21 @@ __device__ __forceinline__ R sp_int_e(const R* u) {
   return rdiv<R>(u[ND + 1], u[0]) - sp_kin_e<ND, R>(u);
 }
 
-// rhyme_ideal_gas_pressure.f90:9
+// rhyme_ideal_gas_pressure.f90:9.  FP64 keeps the reference's
+// rho * (e / rho - kin) order.  The mixed form (FP32) computes the same
+// quantity as (E - |m|^2 / (2 rho)) * (gamma - 1): a gas at rest with
+// uniform E then has bitwise uniform p whatever its density, so such
+// interfaces take the exact contact solution in the Riemann solve.
 template <int ND, typename R = double>
 __device__ __forceinline__ R pressure(const Consts& k, const R* u) {
-  return u[0] * sp_int_e<ND, R>(u) * R(k.g - 1.0);
+  if constexpr (sizeof(R) == sizeof(float)) {
+    R s = R(0);
+#pragma unroll
+    for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
+    return (u[ND + 1] - rdiv<R>(R(.5) * s, u[0])) * R(k.g - 1.0);
+  } else {
+    return u[0] * sp_int_e<ND, R>(u) * R(k.g - 1.0);
+  }
 }
 
 // rhyme_ideal_gas_speed_of_sound.f90:9 with temperature_per_mu.f90:9-10
@@ -387,6 +398,12 @@ struct Deferring : P {
 #ifndef RHY_FP64_FAST_RIEMANN
 #define RHY_FP64_FAST_RIEMANN 0
 #endif
+// Build option RHY_CONTACT_EXACT (default 1): interfaces with equal
+// pressures and normal velocities take the exact contact star state
+// (irs_exec) instead of the Newton iteration.
+#ifndef RHY_CONTACT_EXACT
+#define RHY_CONTACT_EXACT 1
+#endif
 #if RHY_FP64_FAST_RIEMANN
 using PowF64Flux = Deferring<MixedMath>;
 using PowF64Fix = MixedMath;
@@ -535,6 +552,19 @@ template <class PW = PowLib>
 __device__ __forceinline__ double irs_exec(const Consts& k, double lr, double lv, double lp,
                                            double lc, double rr, double rv, double rp, double rc,
                                            double& lf, double& rf) {
+  // Equal pressures and normal velocities: the interface carries at most a
+  // contact, whose exact star state is p* = p and u* = u (f_L(p) = f_R(p) = 0).
+  // The reference's Newton iteration reaches the same state to within a few
+  // ulps; taking it exactly skips the guesses and the iteration on every
+  // face of a region at uniform pressure and velocity.  A build with
+  // -DRHY_CONTACT_EXACT=0 iterates on these faces as well.
+#if RHY_CONTACT_EXACT
+  if (lp == rp && lv == rv) {
+    lf = 0.0;
+    rf = 0.0;
+    return lp;
+  }
+#endif
   const double du = rv - lv;
   const WaveSide L = wave_side<PW>(k, lr, lp, lc);
   const WaveSide R = wave_side<PW>(k, rr, rp, rc);
