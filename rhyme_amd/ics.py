@@ -46,8 +46,8 @@ def cell_centres(dims):
 
 
 def two_state_1d(n=1024, seed=0, gamma=5.0 / 3.0, ncmp=None):
-    """Config 1: discontinuity at x = 0.5 (between cells n/2 and n/2+1), u = 0,
-    rho and p on each side drawn from U[0.1, 1]."""
+    """1D two-state workload: discontinuity at x = 0.5 (between cells n/2 and
+    n/2+1), u = 0, rho and p on each side drawn from U[0.1, 1]."""
     g = rng(seed)
     rl, pl, rr, pr = g.uniform(0.1, 1.0, size=4)
     ncmp = ncmp or ncmp_for(1)
@@ -58,10 +58,10 @@ def two_state_1d(n=1024, seed=0, gamma=5.0 / 3.0, ncmp=None):
 
 
 def spheres_2d(n=1024, seed=0, gamma=5.0 / 3.0, nspheres=16, amp=0.05, lam=32.0, ncmp=None):
-    """Config 2: canvas rho = p = 1 at rest; harmonic perturbation of p along x
-    with amplitude amp and wavelength lam cells; then nspheres sharp discs at
-    seeded centres, radius U[8, 32] cells, rho and p U[0.1, 10]; later discs
-    overwrite earlier ones."""
+    """2D spheres workload: canvas rho = p = 1 at rest; harmonic perturbation
+    of p along x with amplitude amp and wavelength lam cells; then nspheres
+    sharp discs at seeded centres, radius U[8, 32] cells, rho and p
+    U[0.1, 10]; later discs overwrite earlier ones."""
     g = rng(seed)
     ncmp = ncmp or ncmp_for(2)
     i = np.arange(1, n + 1, dtype=np.float64)
@@ -82,7 +82,7 @@ def spheres_2d(n=1024, seed=0, gamma=5.0 / 3.0, nspheres=16, amp=0.05, lam=32.0,
 
 def blast_3d(n=256, seed=0, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, dims=None,
              ncmp=None):
-    """Config 3: rho = 1 + jitter*U[-1, 1], p = p0, at rest; unit energy
+    """3D point blast: rho = 1 + jitter*U[-1, 1], p = p0, at rest; unit energy
     deposited as pressure in the cells whose centre lies within `radius`
     cells of the box centre: p = (gamma-1) * 1 / (N_in * dx^3)."""
     dims = list(dims) if dims is not None else [n, n, n]

@@ -71,13 +71,13 @@ def fsum_interior(ndim, a, var):
 
 
 @pytest.fixture(scope="module")
-def config2():
+def spheres1024():
     n = 1024
     return n, ics.spheres_2d(n=n, seed=2, nspheres=16)
 
 
-def test_config2_oracle_prefix(config2):
-    n, cells = config2
+def test_spheres_oracle_prefix(spheres1024):
+    n, cells = spheres1024
     p = params(2, hydro.MINMOD, hydro.PERIODIC)
     nsteps = 10
     say("2D 1024^2 vs oracle,", nsteps, "steps, oracle threads", oracle.num_threads())
@@ -92,8 +92,8 @@ def test_config2_oracle_prefix(config2):
 
 
 @pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
-def test_config2_conservation_and_symmetry(config2, precision):
-    n, cells = config2
+def test_spheres1024_conservation_and_symmetry(spheres1024, precision):
+    n, cells = spheres1024
     p = params(2, hydro.MINMOD, hydro.PERIODIC)
     nsteps = 500
     say("2D 1024^2 periodic,", nsteps, "steps, precision", precision)
@@ -110,14 +110,14 @@ def test_config2_conservation_and_symmetry(config2, precision):
 
 
 @pytest.fixture(scope="module")
-def config3():
+def blast256():
     n = 256
     return n, ics.blast_3d(n=n, seed=3)
 
 
 @pytest.fixture(scope="module")
-def config3_oracle(config3):
-    n, cells = config3
+def blast_oracle(blast256):
+    n, cells = blast256
     p = params(3, hydro.SUPERBEE, hydro.REFLECTIVE)
     nsteps = 2
     say("3D 256^3 blast on the oracle,", nsteps, "steps, threads", oracle.num_threads())
@@ -127,9 +127,9 @@ def config3_oracle(config3):
 
 
 @pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
-def test_config3_oracle_prefix(config3, config3_oracle, precision):
-    n, cells = config3
-    nsteps, ref, rdts = config3_oracle
+def test_blast_oracle_prefix(blast256, blast_oracle, precision):
+    n, cells = blast256
+    nsteps, ref, rdts = blast_oracle
     p = params(3, hydro.SUPERBEE, hydro.REFLECTIVE)
     say("3D 256^3 blast vs oracle,", nsteps, "steps, precision", precision)
     out, dts = run_device(3, [n] * 3, cells, p, nsteps, precision)
@@ -142,8 +142,8 @@ def test_config3_oracle_prefix(config3, config3_oracle, precision):
 
 
 @pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
-def test_config3_conservation_and_symmetry(config3, precision):
-    n, cells = config3
+def test_blast256_conservation_and_symmetry(blast256, precision):
+    n, cells = blast256
     p = params(3, hydro.SUPERBEE, hydro.REFLECTIVE)
     nsteps = 100
     say("3D 256^3 blast,", nsteps, "steps, precision", precision)

@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU oracle on the
 same seeded inputs.
 
-Tolerance (north_star): FP64 results within 1e-12 relative L1 per conserved
+Tolerance: FP64 results within 1e-12 relative L1 per conserved
 variable.  Unit kernels without pow() (flux, limiter, half step, BC, CFL) are
 required to be bit-identical; Riemann states may differ only by pow()'s last
 bit between ocml (device) and glibc (host).
@@ -195,9 +195,9 @@ def interior(ndim, a):
     return a[(slice(0, ndim + 2),) + sl]
 
 
-def test_config1_1d_trajectory():
-    """1D 1024 cells, van Leer, outflow, 200 steps."""
-    say("config 1 trajectory")
+def test_two_state_1d_trajectory():
+    """1D two-state, 1024 cells, van Leer, outflow, 200 steps."""
+    say("1D two-state trajectory")
     cells, _ = ics.two_state_1d(1024, seed=1)
     p = params(1, limiter=hydro.VAN_LEER, bc=[hydro.OUTFLOW] * 6, tol=1e-6, nit=100)
     out, ref, dts, rdts = run_both(1, [1024], cells, p, [1.0 / 1024], 200)
