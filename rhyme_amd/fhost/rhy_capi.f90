@@ -231,6 +231,73 @@ module rhy_capi
          type(c_ptr), value :: pool
          integer(c_int32_t), value :: slot
       end function rhy_pool_remove
+
+      ! snapshot payloads: the FP32 data of rhyme_chombo_write_level_data and
+      ! the restart reader (ncmp * product(dims) values per box)
+      integer(c_int) function rhy_box_snapshot_pack(box, data) &
+         bind(C, name="rhy_box_snapshot_pack")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: box
+         real(c_float), intent(out) :: data(*)
+      end function rhy_box_snapshot_pack
+
+      integer(c_int) function rhy_box_snapshot_unpack(box, data) &
+         bind(C, name="rhy_box_snapshot_unpack")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: box
+         real(c_float), intent(in) :: data(*)
+      end function rhy_box_snapshot_unpack
+
+      integer(c_int) function rhy_batch_snapshot_length(batch, n) &
+         bind(C, name="rhy_batch_snapshot_length")
+         import :: c_int, c_int64_t, c_ptr
+         type(c_ptr), value :: batch
+         integer(c_int64_t), intent(out) :: n
+      end function rhy_batch_snapshot_length
+
+      integer(c_int) function rhy_batch_snapshot_pack(batch, data) &
+         bind(C, name="rhy_batch_snapshot_pack")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: batch
+         real(c_float), intent(out) :: data(*)
+      end function rhy_batch_snapshot_pack
+
+      integer(c_int) function rhy_batch_snapshot_unpack(batch, data) &
+         bind(C, name="rhy_batch_snapshot_unpack")
+         import :: c_int, c_ptr, c_float
+         type(c_ptr), value :: batch
+         real(c_float), intent(in) :: data(*)
+      end function rhy_batch_snapshot_unpack
+
+      ! two-level hierarchy (ratio 2): lo(3, nboxes) places the fine boxes
+      integer(c_int) function rhy_batch_cfl_dts(batch, p, dts) bind(C, name="rhy_batch_cfl_dts")
+         import :: c_int, c_double, c_ptr, rhy_params_t
+         type(c_ptr), value :: batch
+         type(rhy_params_t), intent(in) :: p
+         real(c_double), intent(out) :: dts(*)
+      end function rhy_batch_cfl_dts
+
+      integer(c_int) function rhy_batch_run_dt_async(batch, p, dt, nsteps) &
+         bind(C, name="rhy_batch_run_dt_async")
+         import :: c_int, c_int32_t, c_double, c_ptr, rhy_params_t
+         type(c_ptr), value :: batch
+         type(rhy_params_t), intent(in) :: p
+         real(c_double), value :: dt
+         integer(c_int32_t), value :: nsteps
+      end function rhy_batch_run_dt_async
+
+      integer(c_int) function rhy_amr_prolong(coarse, fine, lo, half) bind(C, name="rhy_amr_prolong")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: coarse, fine
+         integer(c_int32_t), intent(in) :: lo(3, *)
+         integer(c_int32_t), value :: half
+      end function rhy_amr_prolong
+
+      integer(c_int) function rhy_amr_restrict(coarse, fine, lo) bind(C, name="rhy_amr_restrict")
+         import :: c_int, c_int32_t, c_ptr
+         type(c_ptr), value :: coarse, fine
+         integer(c_int32_t), intent(in) :: lo(3, *)
+      end function rhy_amr_restrict
    end interface
 
 contains
