@@ -348,6 +348,39 @@ contains
       end do
    end subroutine rhyme_samr_download
 
+   ! FP32 payload of a level, box after box, as rhyme_chombo_write_level_data
+   ! fills chws%data (rhyme_chombo_write_level_data.f90:41-71), packed on the
+   ! device.  data must hold NCMP * sum(product(dims)) values; ok is false
+   ! (and msg set) when a box has no device copy or the call fails.
+   subroutine rhyme_samr_level_snapshot(level, data, ok, msg)
+      type(samr_level_t), intent(inout), target :: level
+      real(kind=4), intent(out), target :: data(:)
+      logical, intent(out) :: ok
+      character(len=:), allocatable, intent(out) :: msg
+      type(c_ptr) :: h
+      integer(c_int) :: rc
+      integer :: b, offset, n
+      ok = .false.
+      msg = ''
+      offset = 1
+      do b = 1, level%nboxes
+         n = NCMP*product(level%boxes(b)%dims)
+         if (offset + n - 1 > size(data)) then
+            msg = 'snapshot buffer too small'
+            return
+         end if
+         h = rhy_box_handle(level%boxes(b), msg)
+         if (.not. c_associated(h)) return
+         rc = rhy_box_snapshot_pack(h, data(offset:offset + n - 1))
+         if (rc /= RHY_OK) then
+            msg = rhy_error_text()
+            return
+         end if
+         offset = offset + n
+      end do
+      ok = .true.
+   end subroutine rhyme_samr_level_snapshot
+
    subroutine rhy_device_release_all()
       integer :: l, b
       integer(c_int) :: rc

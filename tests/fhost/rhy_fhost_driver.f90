@@ -5,7 +5,8 @@
 !
 ! usage: driver <namelist file> <cells in> <cells out>
 !   cells in/out: the box's cells(:, ..., 1:NCMP) array as raw FP64 in
-!   Fortran order, ghosts included; cells out is followed by the nsteps dts.
+!   Fortran order, ghosts included; cells out is followed by the nsteps dts
+!   and by the level's FP32 snapshot payload (rhyme_samr_level_snapshot).
 program rhy_fhost_driver
    use rhyme_units
    use rhyme_logger
@@ -40,6 +41,9 @@ program rhy_fhost_driver
    character(len=1024) :: nml_file, in_file, out_file
    integer :: u, s, l, b
    real(kind=8), allocatable :: dts(:)
+   real(kind=4), allocatable :: payload(:)
+   logical :: ok
+   character(len=:), allocatable :: msg
 
    call get_command_argument(1, nml_file)
    call get_command_argument(2, in_file)
@@ -101,9 +105,13 @@ program rhy_fhost_driver
    end do
 
    call rhyme_samr_download(samr)
+   allocate (payload(size(samr%levels(0)%boxes(1)%cells, dim=NDIM + 1)*product(dims(1:NDIM))))
+   call rhyme_samr_level_snapshot(samr%levels(0), payload, ok, msg)
+   if (.not. ok) call logger%err('snapshot: '//msg)
    open (newunit=u, file=trim(out_file), access='stream', form='unformatted', status='replace')
    write (u) samr%levels(0)%boxes(1)%cells
    write (u) dts
+   write (u) payload
    close (u)
    call rhy_device_release_all()
    if (logger%nerr > 0) stop 3

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
+from oracle import chombo as ochombo
 from rhyme_amd import hydro, ics
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -48,10 +49,13 @@ def run_driver(nd, dims, cells, nsteps, p, precision=0):
         r = subprocess.run([exe(nd), nml, cin, cout], capture_output=True, text=True,
                            timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
-        raw = np.fromfile(cout, dtype=np.float64)
-    out = raw[:cells.size].reshape(cells.shape)
-    dts = raw[cells.size:]
-    return out, dts
+        raw = np.fromfile(cout, dtype=np.uint8)
+    nb = 8 * (cells.size + nsteps)
+    f64 = raw[:nb].view(np.float64)
+    out = f64[:cells.size].reshape(cells.shape)
+    dts = f64[cells.size:]
+    payload = raw[nb:].view(np.float32)
+    return out, dts, payload
 
 
 def test_fhost_built():
@@ -104,7 +108,7 @@ def test_fhost_main_loop_vs_oracle(nd, dims, limiter, bc, solver):
     else:
         cells = ics.random_smooth(nd, dims, seed=6, gamma=g)
     nsteps = 10
-    out, dts = run_driver(nd, dims, cells.copy(), nsteps, p)
+    out, dts, payload = run_driver(nd, dims, cells.copy(), nsteps, p)
     ref = cells.copy()
     q = _kb_amu_params(p)
     rdts = oracle.run(q, dims, ref, [1.0 / n for n in dims], nsteps)
@@ -114,3 +118,7 @@ def test_fhost_main_loop_vs_oracle(nd, dims, limiter, bc, solver):
         e = np.abs(a[c] - b[c]).sum() / np.abs(b[c]).sum()
         assert e <= 1e-12, (c, e)
     assert np.max(np.abs(dts - rdts) / rdts) <= 1e-12
+    # the level's snapshot payload, packed on the device through the Fortran
+    # binding: the restated Chombo writer's data of the final state
+    want = ochombo.level_data(nd, [out], out.shape[0])
+    np.testing.assert_array_equal(payload.view(np.uint32), want.view(np.uint32))
