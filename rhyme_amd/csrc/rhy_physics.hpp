@@ -54,6 +54,41 @@ template <>
 __device__ __forceinline__ float rdiv<float>(float a, float b) {
   return a * __builtin_amdgcn_rcpf(b);
 }
+
+// FP64 quotients that share one divisor b, each equal to the IEEE a / b bit
+// for bit.  gfx950 lowers a / b to: div_scale of b and of a, rcp of the
+// scaled b, two Newton steps on that reciprocal (fma(-b, y, 1), fma(y, e,
+// y)), q = a * y, r = fma(-b, q, a), div_fmas(r, y, q), div_fixup(q, b, a).
+// div_scale returns its operand unchanged, and div_fmas is then a plain fma,
+// unless |b| is denormal or near overflow, a / b is denormal, a is below
+// 2^-969 or the exponents of a and b differ by 768 or more.  With
+// 2^-64 <= |b| <= 2^64 and a zero or 2^-601 <= |a| < 2^399 none of that
+// applies, so the reciprocal is formed once here and each quotient is
+// finished with the same mul, fma, fma and div_fixup (a zero a: div_fixup
+// returns the signed zero).  `ok` collects the range checks; a caller redoes
+// its quotients with `/` on the lanes where one failed.
+struct SharedDiv {
+  double b, y;
+  __device__ __forceinline__ explicit SharedDiv(double d) : b(d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-d, r, 1.0);
+    y = __builtin_fma(r, e, r);
+  }
+  __device__ __forceinline__ bool divisor_ok() const {
+    return fabs(b) >= 0x1p-64 && fabs(b) <= 0x1p64;
+  }
+  // a / b; clears ok unless a is zero or frexp(a)'s exponent is in [-600, 400)
+  __device__ __forceinline__ double operator()(double a, bool& ok) const {
+    ok = ok && static_cast<unsigned>(__builtin_amdgcn_frexp_exp(a) + 600) < 1000u;
+    double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    q = __builtin_fma(r, y, q);
+    return __builtin_amdgcn_div_fixup(q, b, a);
+  }
+};
+
 template <typename R>
 __device__ __forceinline__ R root(R x);
 template <>
@@ -102,7 +137,20 @@ __device__ __forceinline__ R speed_of_sound(const Consts& k, const R* u) {
   return root<R>(R(k.g) * R(k.kb_amu) * t_mu);
 }
 
-// rhyme_ideal_gas_flux.f90:12-23; A is the 0-based axis.
+// rhyme_ideal_gas_flux.f90:12-23 for rho >= tiny; A is the 0-based axis.
+template <int ND, int A, typename R = double>
+__device__ __forceinline__ void flux_of(const Consts& k, const R* u, R* f) {
+  R p = pressure<ND, R>(k, u);
+  f[0] = u[1 + A];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) f[1 + d] = rdiv<R>(u[1 + d] * f[0], u[0]);
+  f[1 + A] = f[1 + A] + p;
+  f[ND + 1] = rdiv<R>(f[0], u[0]) * (u[ND + 1] + p);
+}
+
+// rhyme_ideal_gas_flux.f90:12-23; A is the 0-based axis.  In FP64 the five
+// divisions by rho (e / rho in the pressure, m * F_rho / rho, F_rho / rho)
+// share one reciprocal (SharedDiv), with the same results as flux_of.
 template <int ND, int A, typename R = double>
 __device__ __forceinline__ void flux(const Consts& k, const R* u, R* f) {
   if (fabs(u[0]) < tiny_of<R>()) {
@@ -110,12 +158,19 @@ __device__ __forceinline__ void flux(const Consts& k, const R* u, R* f) {
     for (int c = 0; c < ND + 2; ++c) f[c] = R(0);
     return;
   }
-  R p = pressure<ND, R>(k, u);
-  f[0] = u[1 + A];
+  if constexpr (sizeof(R) == sizeof(double)) {
+    const SharedDiv by_rho(u[0]);
+    bool ok = by_rho.divisor_ok();
+    const double p = u[0] * (by_rho(u[ND + 1], ok) - sp_kin_e<ND, R>(u)) * (k.g - 1.0);
+    f[0] = u[1 + A];
 #pragma unroll
-  for (int d = 0; d < ND; ++d) f[1 + d] = rdiv<R>(u[1 + d] * f[0], u[0]);
-  f[1 + A] = f[1 + A] + p;
-  f[ND + 1] = rdiv<R>(f[0], u[0]) * (u[ND + 1] + p);
+    for (int d = 0; d < ND; ++d) f[1 + d] = by_rho(u[1 + d] * f[0], ok);
+    f[1 + A] = f[1 + A] + p;
+    f[ND + 1] = by_rho(f[0], ok) * (u[ND + 1] + p);
+    if (!ok) flux_of<ND, A, R>(k, u, f);
+  } else {
+    flux_of<ND, A, R>(k, u, f);
+  }
 }
 
 // rhyme_ideal_gas_primitive_vars_to_conserved.f90:9-11
@@ -149,8 +204,12 @@ __device__ __forceinline__ R limit_one(int type, R w, R ul, R u, R ur) {
   R dR = ur - u;
   R d = R(.5) * (R(1) + w) * dL + R(.5) * (R(1) + w) * dR;
   R denom = ur - u;
-  R r = fabs(denom) < tiny_of<R>() ? R(0) : rdiv<R>(u - ul, denom);
-  if (!(r > R(0))) return R(0);  // every limiter gives 0 for r <= 0
+  // r = 0 for |denom| < tiny, and every limiter gives 0 for r <= 0: return
+  // before the division (a wave whose lanes all see a flat variable, such
+  // as the momenta of a medium at rest, skips it)
+  if (fabs(denom) < tiny_of<R>()) return R(0);
+  R r = rdiv<R>(u - ul, denom);
+  if (!(r > R(0))) return R(0);
   R xi = rdiv<R>(R(2), R(1) - w + (R(1) + w) * r);
   switch (t) {
     case 1:  // van Leer
@@ -606,6 +665,17 @@ __device__ __forceinline__ void w_kfan_prim(const Consts& k, double srho, const 
 template <int ND, typename RI>
 __device__ __forceinline__ void side_prims(const Consts& k, const RI* s, double* v, double& p,
                                            double& c) {
+  if constexpr (sizeof(RI) == sizeof(double)) {
+    // the divisions by rho (v and e / rho) share one reciprocal (SharedDiv)
+    const SharedDiv by_rho(s[0]);
+    bool ok = by_rho.divisor_ok();
+#pragma unroll
+    for (int d = 0; d < ND; ++d) v[d] = by_rho(s[1 + d], ok);
+    const double ie = by_rho(s[ND + 1], ok) - sp_kin_e<ND, double>(s);
+    p = dmax(s[0] * ie * (k.g - 1.0), k.w_p);
+    c = dmax(sqrt(k.g * k.kb_amu * ((ie * (k.g - 1.0)) / k.kb_amu)), kTiny);
+    if (ok) return;
+  }
 #pragma unroll
   for (int d = 0; d < ND; ++d) v[d] = (double)rdiv<RI>(s[1 + d], s[0]);
   p = dmax((double)pressure<ND, RI>(k, s), k.w_p);

@@ -123,6 +123,73 @@ def test_half_step_and_flux_bitexact(ndim):
     assert np.array_equal(fg, fo)
 
 
+def same_bits(a, b):
+    """Bitwise equality (signed zeros included); NaNs only by position, since
+    the device's and the host's default NaNs differ in sign."""
+    na, nb = np.isnan(a), np.isnan(b)
+    return np.array_equal(na, nb) and np.array_equal(
+        a[~na].view(np.uint64), b[~nb].view(np.uint64))
+
+
+# Edges of the range in which the FP64 flux and the Riemann side states divide
+# by rho through one shared reciprocal (|rho| in [2^-64, 2^64], numerators
+# zero or in [2^-601, 2^399)), with values on both sides of each edge, zeros
+# of either sign, and momenta whose products are denormal or past the edge.
+SHARED_DIV_RHOS = [2.0**-64, np.nextafter(2.0**-64, 0), 2.0**64, np.nextafter(2.0**64, np.inf),
+                   1e-30, 1e30, 1.0, 0.37, 3e-300, -2.5]
+SHARED_DIV_MOMS = [0.0, -0.0, 1e-300, 1e-160, -1e-160, 2.0**199, -(2.0**200), 1e100, 0.3, -7.0]
+SHARED_DIV_ES = [1e-250, 2.0**-601, np.nextafter(2.0**-601, 0), 2.0**398, 2.0**399, 1.0, 1e250]
+
+
+def test_flux_shared_divisor_ranges():
+    say("flux: shared reciprocal of rho, both sides of its range")
+    g = np.random.default_rng(77)
+    p = params(3)
+    n = 6000
+    u = np.empty((n, 5))
+    u[:, 0] = g.choice(SHARED_DIV_RHOS, n)
+    u[:, 1:4] = g.choice(SHARED_DIV_MOMS, (n, 3))
+    u[:, 4] = g.choice(SHARED_DIV_ES, n)
+    for ax in (1, 2, 3):
+        axis = np.full(n, ax, dtype=np.int32)
+        fg = hydro.flux_batch(p, u, axis)
+        fo = oracle.flux_batch(oracle.params_from(p), u, axis)
+        assert same_bits(fg, fo), ("axis", ax)
+
+
+def test_riemann_shared_divisor_ranges():
+    say("riemann: side states at the edges of the shared-reciprocal range")
+    g = np.random.default_rng(78)
+    p = params(3)
+    n = 4000
+    # Both sides of a record take densities from the same edge (2^-64 or
+    # 2^64, each side of it): pairing 2^64 with ~1 makes the Newton
+    # iteration stop, at tol 1e-6, on an iteration that pow's last bit
+    # decides, and the records then differ by up to ~2e-5 from the oracle
+    # with or without the shared reciprocal.
+    edges = np.array([[2.0**-64, np.nextafter(2.0**-64, 0)],
+                      [2.0**64, np.nextafter(2.0**64, np.inf)]])
+    pick = g.integers(0, 3, size=n)
+    group = g.integers(0, 2, size=n)
+
+    def states():
+        s = random_states(g, n, 3, p.gamma)
+        sel = pick == 1
+        s[sel, 0] = edges[group[sel], g.integers(0, 2, size=int(sel.sum()))]
+        sel = pick == 2
+        s[sel, 1:4] = g.choice([0.0, -0.0, 1e-160, -1e-170], (int(sel.sum()), 3))
+        return s
+    l, r = states(), states()
+    axis = g.integers(1, 4, size=n).astype(np.int32)
+    u_gpu = hydro.riemann_batch(p, l, r, axis)
+    u_ref = oracle.rp_solve_batch(oracle.params_from(p), l, r, axis, 1.0)
+    ok = np.isfinite(u_ref)
+    assert np.array_equal(np.isfinite(u_gpu), ok)
+    rel = np.abs(u_gpu[ok] - u_ref[ok]) / np.maximum(np.abs(u_ref[ok]), 1e-300)
+    say("  max rel", rel.max(), "bit-identical fraction", np.mean(u_gpu[ok] == u_ref[ok]))
+    assert rel.max() < 1e-10
+
+
 def box_case(ndim, dims, seed, gamma):
     return ics.random_smooth(ndim, dims, seed=seed, gamma=gamma)
 
