@@ -42,13 +42,13 @@ HBM_PEAK_GBS = 8000.0
 # (--copy-probe): FETCH_SIZE reads 0.500 KiB per KiB a 1 GiB scaled copy
 # reads, WRITE_SIZE 1.000 per KiB written.
 PMC_TRAFFIC_PROFILE = "profiles/blast256_fp64_mixed_kernels.md"
-PMC_TRAFFIC_B_PER_ZONE = {"fp64": 597.8, "mixed": 273.8}
+PMC_TRAFFIC_B_PER_ZONE = {"fp64": 547.1, "mixed": 273.7}
 PMC_TRAFFIC_N = 256
 # The flux kernels are bound by VALU issue, not HBM: issue cycles (4 per
 # wave64 VALU instruction, 16 per transcendental) over the SIMD cycles of
 # their run at 2.4 GHz, mean of the x, y, z launches, from the SQ counter
 # passes in the same profile.
-PMC_VALU_ISSUE = {"fp64": 0.75, "mixed": 0.89}
+PMC_VALU_ISSUE = {"fp64": 0.80, "mixed": 0.89}
 
 
 def blast_params(hydro, solver):

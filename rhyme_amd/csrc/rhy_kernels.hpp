@@ -27,6 +27,15 @@
 #define RHY_FLUX_OCCUPANCY
 #endif
 
+// 1: the FP64 y/z walks of single boxes stage their cells in a lane-private
+// LDS ring (each cell loaded once); 0: they re-read the three cells of every
+// half step, as the batched kernel's walks do (measured: the ring makes the
+// single-box FP64 walks 7% faster and neither the FP32 walks, whose register
+// window it would replace, nor the batched walks, which then spill).
+#ifndef RHY_LDS_WALK
+#define RHY_LDS_WALK 1
+#endif
+
 namespace rhy {
 
 // Geometry of a ghosted box: Fortran cell (i, j, k), 1-based interior
@@ -49,6 +58,15 @@ __device__ __forceinline__ void load_state(const R* __restrict__ cells, int64_t 
                                            R* u) {
 #pragma unroll
   for (int c = 0; c < ND + 2; ++c) u[c] = cells[o + c * plane];
+}
+
+// The LDS ring of the FP64 y/z walks: 4 cells of up to 5 doubles for each
+// lane of a 256-thread workgroup (40 KB), one allocation per kernel whatever
+// walks the kernel instantiates.
+constexpr int kWalkLanes = 256;
+__device__ __forceinline__ double* walk_ring() {
+  __shared__ double ring[4 * 5 * kWalkLanes];
+  return ring;
 }
 
 // Boundary types per face (left, right, bottom, top, back, front).
@@ -306,9 +324,10 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
 // flux_walk: one thread's walk of nf_max consecutive faces normal to A
 // (y, z), starting at face chunk `t` of the box (thread t of the axis' index
 // space); nothing if t is past the end.  A == 0 goes to flux_xrows.
-// WINDOW: FP32 cells through a sliding window (faster for single boxes);
-// otherwise re-read like FP64 cells, which keeps the batched kernel, whose
-// three walks share one register budget, within 3 waves per SIMD unspilled.
+// WINDOW (single boxes): FP32 cells through a register sliding window, FP64
+// cells through the LDS ring (RHY_LDS_WALK); otherwise (the batched kernel,
+// whose three walks share one register budget) every half step re-reads its
+// three cells, which keeps it within 3 waves per SIMD unspilled.
 template <int ND, int A, int LIM, typename R, class PW, bool WINDOW = true>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
@@ -360,6 +379,47 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
       }
       half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
       if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
+      R Fr[NV];
+      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+      const int64_t of = o + m * s;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        F[of + c * G.plane] = Fr[c];
+        rp[c] = rn[c];
+      }
+      defer_take<PW>(k, dl, box, A, of);
+    }
+  } else if constexpr (sizeof(R) == sizeof(double) && WINDOW && RHY_LDS_WALK) {
+    // FP64 cells through a lane-private ring of 4 cells in LDS (cell j of the
+    // walk, counted from f0, in slot j & 3): each cell is loaded from HBM
+    // once.  At face m the load of cell m + 3 is issued before the half step
+    // of cell m + 1 (cells m .. m + 2, read from LDS) and written to the ring
+    // after it, so it is in flight during the half step and needs no
+    // registers across the Riemann solve.  A lane touches only its own slots
+    // (no barrier).
+    double* ring = walk_ring();
+    const int lane = threadIdx.x;
+    auto put = [&](int j, const R* v) {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) ring[((j & 3) * NV + c) * kWalkLanes + lane] = v[c];
+    };
+    auto get = [&](int j, R* v) {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) v[c] = ring[((j & 3) * NV + c) * kWalkLanes + lane];
+    };
+    put(0, u0);
+    put(1, up);
+    R nx[NV];
+    load_state<ND, R>(cells, G.plane, o + 2 * s, nx);
+    put(2, nx);
+#pragma unroll 1
+    for (int m = 0; m < nf; ++m) {
+      if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
+      get(m, um);
+      get(m + 1, u0);
+      get(m + 2, up);
+      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+      if (m + 1 < nf) put(m + 3, nx);
       R Fr[NV];
       riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
       const int64_t of = o + m * s;
