@@ -27,11 +27,12 @@
 #define RHY_FLUX_OCCUPANCY
 #endif
 
-// 1: the FP64 y/z walks of single boxes stage their cells in a lane-private
-// LDS ring (each cell loaded once); 0: they re-read the three cells of every
-// half step, as the batched kernel's walks do (measured: the ring makes the
-// single-box FP64 walks 7% faster and neither the FP32 walks, whose register
-// window it would replace, nor the batched walks, which then spill).
+// 1: the y/z walks stage their cells in a lane-private LDS ring (each cell
+// loaded once) -- FP64 walks of single boxes and FP32 walks of the batched
+// kernel; 0: they re-read the three cells of every half step.  Measured: the
+// ring makes the single-box FP64 walks 7% faster and the batched mixed form
+// 4%; it is not used for the single-box FP32 walks, whose register window
+// is as fast, nor for the batched FP64 walks, which would spill.
 #ifndef RHY_LDS_WALK
 #define RHY_LDS_WALK 1
 #endif
@@ -60,9 +61,9 @@ __device__ __forceinline__ void load_state(const R* __restrict__ cells, int64_t 
   for (int c = 0; c < ND + 2; ++c) u[c] = cells[o + c * plane];
 }
 
-// The LDS ring of the FP64 y/z walks: 4 cells of up to 5 doubles for each
-// lane of a 256-thread workgroup (40 KB), one allocation per kernel whatever
-// walks the kernel instantiates.
+// The LDS ring of the y/z walks: 4 cells of up to 5 doubles (or floats) for
+// each lane of a 256-thread workgroup (40 KB), one allocation per kernel
+// whatever walks the kernel instantiates.
 constexpr int kWalkLanes = 256;
 __device__ __forceinline__ double* walk_ring() {
   __shared__ double ring[4 * 5 * kWalkLanes];
@@ -325,9 +326,10 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
 // (y, z), starting at face chunk `t` of the box (thread t of the axis' index
 // space); nothing if t is past the end.  A == 0 goes to flux_xrows.
 // WINDOW (single boxes): FP32 cells through a register sliding window, FP64
-// cells through the LDS ring (RHY_LDS_WALK); otherwise (the batched kernel,
-// whose three walks share one register budget) every half step re-reads its
-// three cells, which keeps it within 3 waves per SIMD unspilled.
+// cells through the LDS ring (RHY_LDS_WALK).  Otherwise (the batched kernel,
+// whose three walks share one register budget): FP32 cells through the LDS
+// ring, while FP64 half steps re-read their three cells, which keeps the
+// kernel within 3 waves per SIMD unspilled.
 template <int ND, int A, int LIM, typename R, class PW, bool WINDOW = true>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
                                           const R* __restrict__ cells, R* __restrict__ F,
@@ -389,15 +391,15 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
       }
       defer_take<PW>(k, dl, box, A, of);
     }
-  } else if constexpr (sizeof(R) == sizeof(double) && WINDOW && RHY_LDS_WALK) {
-    // FP64 cells through a lane-private ring of 4 cells in LDS (cell j of the
+  } else if constexpr (RHY_LDS_WALK && (sizeof(R) == sizeof(double) ? WINDOW : !WINDOW)) {
+    // Cells through a lane-private ring of 4 cells in LDS (cell j of the
     // walk, counted from f0, in slot j & 3): each cell is loaded from HBM
     // once.  At face m the load of cell m + 3 is issued before the half step
     // of cell m + 1 (cells m .. m + 2, read from LDS) and written to the ring
     // after it, so it is in flight during the half step and needs no
     // registers across the Riemann solve.  A lane touches only its own slots
     // (no barrier).
-    double* ring = walk_ring();
+    R* ring = reinterpret_cast<R*>(walk_ring());
     const int lane = threadIdx.x;
     auto put = [&](int j, const R* v) {
 #pragma unroll
