@@ -28,11 +28,12 @@
 #endif
 
 // 1: the y/z walks stage their cells in a lane-private LDS ring (each cell
-// loaded once) -- FP64 walks of single boxes and FP32 walks of the batched
-// kernel; 0: they re-read the three cells of every half step.  Measured: the
-// ring makes the single-box FP64 walks 7% faster and the batched mixed form
-// 4%; it is not used for the single-box FP32 walks, whose register window
-// is as fast, nor for the batched FP64 walks, which would spill.
+// loaded once) -- FP64 walks of single boxes and both walks of the batched
+// kernel (its FP64 walks without the early load, which would spill there);
+// 0: they re-read the three cells of every half step.  Measured: 7% faster
+// single-box FP64 walks, 4% for the batched mixed form, 1.4% for the
+// batched FP64 form; the single-box FP32 walks keep their register window,
+// which is as fast.
 #ifndef RHY_LDS_WALK
 #define RHY_LDS_WALK 1
 #endif
@@ -328,7 +329,7 @@ __device__ __forceinline__ void flux_xrows(const Consts& k, const Geom& G,
 // WINDOW (single boxes): FP32 cells through a register sliding window, FP64
 // cells through the LDS ring (RHY_LDS_WALK).  Otherwise (the batched kernel,
 // whose three walks share one register budget): FP32 cells through the LDS
-// ring, while FP64 half steps re-read their three cells, which keeps the
+// ring, FP64 cells through the ring without the early load, which keeps the
 // kernel within 3 waves per SIMD unspilled.
 template <int ND, int A, int LIM, typename R, class PW, bool WINDOW = true>
 __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
@@ -432,10 +433,41 @@ __device__ __forceinline__ void flux_walk(const Consts& k, const Geom& G,
       }
       defer_take<PW>(k, dl, box, A, of);
     }
+  } else if constexpr (RHY_LDS_WALK && sizeof(R) == sizeof(double)) {
+    // Batched FP64 walks: the ring without the early load, so no registers
+    // beyond the re-reading walk's (the early load makes this kernel spill):
+    // each face loads its newest cell and takes the two below from the ring
+    R* ring = reinterpret_cast<R*>(walk_ring());
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < NV; ++c) {
+      ring[(0 * NV + c) * kWalkLanes + lane] = u0[c];
+      ring[(1 * NV + c) * kWalkLanes + lane] = up[c];
+    }
+#pragma unroll 1
+    for (int m = 0; m < nf; ++m) {
+      load_state<ND, R>(cells, G.plane, o + (m + 2) * s, up);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        um[c] = ring[((m & 3) * NV + c) * kWalkLanes + lane];
+        u0[c] = ring[(((m + 1) & 3) * NV + c) * kWalkLanes + lane];
+        ring[(((m + 2) & 3) * NV + c) * kWalkLanes + lane] = up[c];
+      }
+      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+      R Fr[NV];
+      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+      const int64_t of = o + m * s;
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        F[of + c * G.plane] = Fr[c];
+        rp[c] = rn[c];
+      }
+      defer_take<PW>(k, dl, box, A, of);
+    }
   } else {
-    // FP64 cells (and FP32 without the window): only rp (the right state of
-    // the low cell) is carried across the Riemann solve; the three cells of
-    // the next half step are re-read (cache hits), which keeps the register
+    // Without the LDS ring (RHY_LDS_WALK=0): only rp (the right state of the
+    // low cell) is carried across the Riemann solve; the three cells of the
+    // next half step are re-read (cache hits), which keeps the register
     // count near the single-face one.
 #pragma unroll 1
     for (int m = 0; m < nf; ++m) {
