@@ -98,6 +98,16 @@ int rhy_init(int32_t device);
 const char *rhy_last_error(void);
 /* Number of kernels / variants compiled in (for "library loads" checks). */
 int32_t rhy_version(void);
+/* The compile-time options this library was built with, as bits:
+ * RHY_OPT_POW_FAST (own FP64 exponentials; clear: device libm pow),
+ * RHY_OPT_CONTACT_EXACT (exact contact star states), RHY_OPT_LDS_WALK (LDS
+ * ring in the y/z flux walks), RHY_OPT_FAST_RIEMANN (the mixed form's FP64
+ * Riemann arithmetic in the FP64 flux kernels). */
+#define RHY_OPT_POW_FAST 1
+#define RHY_OPT_CONTACT_EXACT 2
+#define RHY_OPT_LDS_WALK 4
+#define RHY_OPT_FAST_RIEMANN 8
+int32_t rhy_build_options(void);
 
 /* ---- box lifetime and host mirror transfers ----
  * Replaces the allocation side of rhyme_samr_init_box (rhyme_samr/src/
@@ -245,6 +255,77 @@ int rhy_box_paint(rhy_box_t *box, const rhy_ic_t *ic);
  * conserved state on each interface; axis[q] in 1..ndim. */
 int rhy_riemann_batch(const rhy_params_t *p, int64_t n, const double *l,
                       const double *r, const int32_t *axis, double *u);
+/* rhyme_riemann_problem_solve(rp, l, r, dx, dt, axis, u) (…_solve.f90:3-104)
+ * at any sample point: record q is sampled at x/t = dx[q] / dt[q] (the
+ * reference's dxdt = dx/dt, :17), through the same device routine the flux
+ * kernels use at dx = 0.  u receives the conserved state. */
+int rhy_riemann_solve_batch(const rhy_params_t *p, int64_t n, const double *l,
+                            const double *r, const int32_t *axis, const double *dx,
+                            const double *dt, double *u);
+
+/* riemann_problem_solution_t (rhyme_riemann_problem/src/rhyme_riemann_problem.f90:33-61)
+ * flattened: the two sides' primitive states (rp_side_t: rho, v(NDIM), p,
+ * cs; v holds ndim components) and the star region (rp_star_t: p, u, and per
+ * side is_shock, shock%rho/speed, fan%rho/cs/speedH/speedT, and the wave
+ * function value f at p* that rhyme_irs_exec returns through l_f / r_f). */
+typedef struct rhy_rp_solution {
+    double l_rho, l_v[3], l_p, l_cs;
+    double r_rho, r_v[3], r_p, r_cs;
+    double star_p, star_u;
+    int32_t l_is_shock, r_is_shock;
+    double l_shock_rho, l_shock_speed;
+    double l_fan_rho, l_fan_cs, l_fan_speedH, l_fan_speedT;
+    double r_shock_rho, r_shock_speed;
+    double r_fan_rho, r_fan_cs, r_fan_speedH, r_fan_speedT;
+    double l_f, r_f;
+} rhy_rp_solution_t;
+
+/* rhyme_riemann_problem_star(rp, sol, axis) (…_star.f90:3-66): sol[q]'s sides
+ * are read, its star members written (p* floored at w_vacuum(p)). */
+int rhy_riemann_star_batch(const rhy_params_t *p, int64_t n, rhy_rp_solution_t *sol,
+                           const int32_t *axis);
+/* rhyme_riemann_problem_sampling(solution, axis, dx, dt, u) (…_sampling.f90:3-48):
+ * the conserved state of sol[q] at x/t = dx[q] / dt[q]. */
+int rhy_riemann_sampling_batch(const rhy_params_t *p, int64_t n, const rhy_rp_solution_t *sol,
+                               const int32_t *axis, const double *dx, const double *dt,
+                               double *u);
+/* The region states the sampling chooses between, one reference function
+ * each: riemann_problem_w_k of the left / right side (…_w_k.f90:3-11),
+ * w_starL_sho / w_starR_sho (…_w_starL_sho.f90:3-15, …_w_starR_sho.f90),
+ * w_starL_fan / w_starR_fan (…_w_starL_fan.f90, …_w_starR_fan.f90) and
+ * w_kfan(side, dxdt, axis, is_right) (…_w_kfan.f90:3-32). */
+#define RHY_W_L 1
+#define RHY_W_R 2
+#define RHY_W_STARL_SHO 3
+#define RHY_W_STARR_SHO 4
+#define RHY_W_STARL_FAN 5
+#define RHY_W_STARR_FAN 6
+#define RHY_W_LFAN 7
+#define RHY_W_RFAN 8
+int rhy_riemann_region_batch(const rhy_params_t *p, int64_t n, const rhy_rp_solution_t *sol,
+                             const int32_t *region, const int32_t *axis, const double *dxdt,
+                             double *u);
+
+/* rhyme_irs_exec(irs, l_rho, l_v, l_p, l_cs, l_f, r_rho, r_v, r_p, r_cs, r_f, axis)
+ * (rhyme_irs/src/rhyme_irs_exec.f90:3-45): l and r are [n][4] records
+ * (rho, v(axis), p, cs); p_star, l_f, r_f receive p* and the wave functions'
+ * values at the last iterate (the reference's intent(inout) l_f, r_f). */
+int rhy_irs_exec_batch(const rhy_params_t *p, int64_t n, const double *l, const double *r,
+                       double *p_star, double *l_f, double *r_f);
+/* rhyme_irs_guess_p_star (rhyme_irs/src/rhyme_irs_guess_p_star.f90:3-72):
+ * the six starting guesses, [n][6], from the same records. */
+int rhy_irs_guess_batch(const rhy_params_t *p, int64_t n, const double *l, const double *r,
+                        double *guesses);
+
+/* x**y on the device, for pinning the exponentiation of the Riemann solve
+ * (Fortran ** on real(8), i.e. libm pow in the reference): policy
+ * RHY_POW_PATH is the FP64 path's own (this build's flux/fixup policy),
+ * RHY_POW_LIBM the device math library's pow, RHY_POW_MIXED the mixed form's. */
+#define RHY_POW_PATH 0
+#define RHY_POW_LIBM 1
+#define RHY_POW_MIXED 2
+int rhy_pow_batch(int32_t policy, int64_t n, const double *x, const double *y, double *out);
+
 /* rhyme_slope_limiter_run (rhyme_slope_limiter/src/rhyme_slope_limiter_run.f90:3-21) */
 int rhy_limiter_batch(const rhy_params_t *p, int64_t n, const double *ul,
                       const double *u, const double *ur, double *delta);

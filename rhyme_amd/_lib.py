@@ -39,7 +39,17 @@ EXPORTS = [
     "rhy_amr_restrict",
     "rhy_box_snapshot_pack", "rhy_box_snapshot_unpack", "rhy_batch_snapshot_length",
     "rhy_batch_snapshot_pack", "rhy_batch_snapshot_unpack",
+    "rhy_build_options", "rhy_riemann_solve_batch", "rhy_riemann_star_batch",
+    "rhy_riemann_sampling_batch", "rhy_riemann_region_batch", "rhy_irs_exec_batch",
+    "rhy_irs_guess_batch", "rhy_pow_batch",
 ]
+
+# rhy_build_options bits
+OPT_POW_FAST, OPT_CONTACT_EXACT, OPT_LDS_WALK, OPT_FAST_RIEMANN = 1, 2, 4, 8
+# rhy_pow_batch policies
+POW_PATH, POW_LIBM, POW_MIXED = 0, 1, 2
+# rhy_riemann_region_batch region ids
+W_L, W_R, W_STARL_SHO, W_STARR_SHO, W_STARL_FAN, W_STARR_FAN, W_LFAN, W_RFAN = range(1, 9)
 
 
 class RhyParams(C.Structure):
@@ -59,6 +69,25 @@ class RhyParams(C.Structure):
         ("ncmp", C.c_int32),
         ("reserved", C.c_int32),
         ("cons_inflows", (C.c_double * MAX_CMP) * 6),
+    ]
+
+
+class RhyRpSolution(C.Structure):
+    """rhy_rp_solution_t: riemann_problem_solution_t flattened."""
+    _fields_ = [
+        ("l_rho", C.c_double), ("l_v", C.c_double * 3), ("l_p", C.c_double),
+        ("l_cs", C.c_double),
+        ("r_rho", C.c_double), ("r_v", C.c_double * 3), ("r_p", C.c_double),
+        ("r_cs", C.c_double),
+        ("star_p", C.c_double), ("star_u", C.c_double),
+        ("l_is_shock", C.c_int32), ("r_is_shock", C.c_int32),
+        ("l_shock_rho", C.c_double), ("l_shock_speed", C.c_double),
+        ("l_fan_rho", C.c_double), ("l_fan_cs", C.c_double), ("l_fan_speedH", C.c_double),
+        ("l_fan_speedT", C.c_double),
+        ("r_shock_rho", C.c_double), ("r_shock_speed", C.c_double),
+        ("r_fan_rho", C.c_double), ("r_fan_cs", C.c_double), ("r_fan_speedH", C.c_double),
+        ("r_fan_speedT", C.c_double),
+        ("l_f", C.c_double), ("r_f", C.c_double),
     ]
 
 
@@ -121,6 +150,7 @@ class RhyError(RuntimeError):
 
 
 _lib = None
+_by_path = {}
 
 
 def load():
@@ -128,9 +158,38 @@ def load():
     global _lib
     if _lib is not None:
         return _lib
-    if not os.path.exists(LIB_PATH):
-        raise RhyError("librhyme_hip.so is not built (run __graft_entry__.build())")
-    L = C.CDLL(LIB_PATH)
+    _lib = open_library(LIB_PATH)
+    return _lib
+
+
+class using:
+    """Route every call made inside the block to another build of the library
+    (e.g. one compiled with other build options), loaded beside the default
+    one; handles made inside the block belong to that build and must be
+    closed inside it."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self._saved = _lib
+        _lib = open_library(self.path)
+        return _lib
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self._saved
+        return False
+
+
+def open_library(path):
+    """A ctypes handle of the library at `path`, signatures declared."""
+    if path in _by_path:
+        return _by_path[path]
+    if not os.path.exists(path):
+        raise RhyError("%s is not built (run __graft_entry__.build())" % path)
+    L = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int32)
     vp = C.c_void_p
@@ -167,11 +226,22 @@ def load():
         "rhy_box_snapshot_pack": (C.c_int, [vp, C.POINTER(C.c_float)]),
         "rhy_box_snapshot_unpack": (C.c_int, [vp, C.POINTER(C.c_float)]),
     }
+    sp = C.POINTER(RhyRpSolution)
+    sig.update({
+        "rhy_build_options": (C.c_int32, []),
+        "rhy_riemann_solve_batch": (C.c_int, [pp, C.c_int64, dp, dp, ip, dp, dp, dp]),
+        "rhy_riemann_star_batch": (C.c_int, [pp, C.c_int64, sp, ip]),
+        "rhy_riemann_sampling_batch": (C.c_int, [pp, C.c_int64, sp, ip, dp, dp, dp]),
+        "rhy_riemann_region_batch": (C.c_int, [pp, C.c_int64, sp, ip, ip, dp, dp]),
+        "rhy_irs_exec_batch": (C.c_int, [pp, C.c_int64, dp, dp, dp, dp, dp]),
+        "rhy_irs_guess_batch": (C.c_int, [pp, C.c_int64, dp, dp, dp]),
+        "rhy_pow_batch": (C.c_int, [C.c_int32, C.c_int64, dp, dp, dp]),
+    })
     for name, (res, args) in sig.items():
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
-    _lib = L
+    _by_path[path] = L
     return L
 
 

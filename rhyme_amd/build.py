@@ -61,5 +61,31 @@ def build(verbose=False, extra=None, out=None):
     return lib
 
 
+# Libraries built from the same sources with the documented build options
+# switched, for the tests that run the parity checks against each of them
+# (selected per call through _lib.using).
+VARIANT_DIR = os.path.join(HERE, "variants")
+VARIANTS = {
+    "powlib": ["-DRHY_FP64_POW_FAST=0"],       # device libm pow in the FP64 path
+    "nocontact": ["-DRHY_CONTACT_EXACT=0"],    # Newton iteration on contacts too
+    "noldswalk": ["-DRHY_LDS_WALK=0"],         # y/z walks re-read their cells
+    "fastriemann": ["-DRHY_FP64_FAST_RIEMANN=1"],  # mixed-form FP64 Riemann policy
+}
+
+
+def variant_path(name):
+    return os.path.join(VARIANT_DIR, "librhyme_hip_%s.so" % name)
+
+
+def build_all(verbose=False):
+    """The default library and every variant, built concurrently."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(VARIANT_DIR, exist_ok=True)
+    jobs = [(None, None)] + [(variant_path(n), f) for n, f in VARIANTS.items()]
+    with ThreadPoolExecutor(len(jobs)) as ex:
+        futs = [ex.submit(build, verbose, f, o) for o, f in jobs]
+        return [f.result() for f in futs]
+
+
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build_all(verbose=True))

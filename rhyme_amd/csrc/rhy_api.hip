@@ -57,20 +57,33 @@ struct rhy_box {
   int32_t precision = RHY_FP64;
   Geom geom{};
   hipStream_t stream = nullptr;
-  void* cells = nullptr;          // ncmp planes (hydro first), double or float
+  // State: two buffers of the ndim+2 hydro planes (the step reads one and
+  // writes the other, then they swap) and one set of the other ncmp-ndim-2
+  // component planes, in one block [hydro 0 | hydro 1 | others]; `cells`
+  // is the current hydro buffer.  Double or float.
+  void* block = nullptr;
+  void* state[2] = {nullptr, nullptr};
+  void* extra = nullptr;
+  int32_t cur = 0;
+  void* cells = nullptr;
   void* flux = nullptr;           // ndim * (ndim+2) planes, same type
   size_t esz = sizeof(double);    // bytes per stored value
   double* partial = nullptr;      // per-block CFL maxima of the last reduction
-  int32_t partial_n = 0;
+  int32_t partial_n = 0;          // partials of the stand-alone CFL pass
+  int32_t acc_blocks = 0;         // partials of the last accumulating walk
+  int32_t partial_cap = 0;
   double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
   double inflow_copied[6 * RHY_MAX_CMP] = {};  // what the device inflow slots hold
   bool inflow_valid = false;
   double* dt = nullptr;           // current dt
   double* dt_log = nullptr;       // per-step dt of the last run
   void* diag = nullptr;           // workgroup partials of rhy_box_diagnostics
-  uint32_t* defer_count = nullptr;  // faces the flux kernels left to the fixup
+  uint32_t* defer_count = nullptr;  // faces the flux kernels left to the fixup:
+                                    // [0] or per axis [0..ndim)
   uint64_t* defer_items = nullptr;
-  uint32_t defer_cap = 0;
+  uint32_t defer_cap = 0;           // the whole list (one list for every axis)
+  uint32_t defer_cap_axis[3] = {0, 0, 0};  // per-axis lists carved from it
+  int64_t defer_off_axis[3] = {0, 0, 0};
   float* snap = nullptr;          // staging of the FP32 snapshot payload (lazy)
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
@@ -189,6 +202,21 @@ void collect_timings(rhy_box* b) {
 constexpr int kNfX = RHY_NF_X;
 constexpr int kNfYZ = RHY_NF_YZ;
 
+// Cells walked per thread by the accumulating y/z walks of rhy_run.
+#ifndef RHY_ACC_L
+#define RHY_ACC_L 16
+#endif
+constexpr int kAccL = RHY_ACC_L;
+
+// Faces of axis a a step solves (faces 0..n_a along a, interior across).
+int64_t axis_faces(const Geom& G, int a) {
+  int64_t n = (int64_t)G.n[a] + 1;
+  for (int d = 0; d < 3; ++d)
+    if (d != a) n *= G.n[d];
+  return n;
+}
+
+
 DeferList defer_list(rhy_box* b) {
   DeferList dl;
   dl.count = b->defer_count;
@@ -227,7 +255,8 @@ void launch_flux_r(rhy_box* b, const Consts& k, const double* dx) {
   const DeferList dl = defer_list(b);
   TimedLaunch tr;
   timed_begin(b, 5, tr);
-  hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 0u);
+  hipLaunchKernelGGL(step_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 4,
+                     (double*)nullptr);
   timed_end(b, tr);
   switch (k.limiter) {
     case 1: launch_flux_lim<ND, 1, R, PW>(b, k, dx, dl); break;
@@ -240,7 +269,7 @@ void launch_flux_r(rhy_box* b, const Consts& k, const double* dx) {
     timed_begin(b, 5, tf);
     hipLaunchKernelGGL((flux_fixup_kernel<ND, R, P>), dim3(kFixupBlocks), dim3(kThreads), 0,
                        b->stream, k, b->geom, static_cast<const R*>(b->cells),
-                       static_cast<R*>(b->flux), b->dt, dx[0], dx[1], dx[2], dl);
+                       static_cast<R*>(b->flux), b->dt, dx[0], dx[1], dx[2], dl, -1);
     timed_end(b, tf);
   }
 }
@@ -291,8 +320,9 @@ int launch_update(rhy_box* b, const Consts& k, const double* dx, int order, bool
   return launch_update_r<ND, double>(b, k, dx, order, fused, bc);
 }
 
+// Ghost fill of the current state (hydro: of another hydro buffer instead).
 template <int ND>
-int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
+int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only, void* hydro = nullptr) {
   const Geom& G = b->geom;
   int64_t maxface = 1;
   for (int a = 0; a < ND; ++a) {
@@ -317,10 +347,12 @@ int launch_bc(rhy_box* b, const rhy_params_t* p, int face_only) {
   dim3 grid(grid_for(maxface), face_only >= 0 ? 1 : 2 * ND);
   if (b->precision == RHY_MIXED)
     hipLaunchKernelGGL((bc_kernel<ND, float>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p),
-                       dinf, b->ncmp, static_cast<float*>(b->cells), face_only);
+                       dinf, b->ncmp, static_cast<float*>(hydro ? hydro : b->cells),
+                       static_cast<float*>(b->extra), face_only);
   else
     hipLaunchKernelGGL((bc_kernel<ND, double>), grid, dim3(kThreads), 0, b->stream, G, make_bc(p),
-                       dinf, b->ncmp, static_cast<double*>(b->cells), face_only);
+                       dinf, b->ncmp, static_cast<double*>(hydro ? hydro : b->cells),
+                       static_cast<double*>(b->extra), face_only);
   timed_end(b, tl);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -354,12 +386,116 @@ int ensure_dt_log(rhy_box* b, int32_t n) {
   if (b->dt_log) (void)hipFree(b->dt_log);
   b->dt_log = nullptr;
   b->dt_log_cap = 0;
-  if (hipMalloc(&b->dt_log, sizeof(double) * (size_t)n) != hipSuccess)
+  if (dev_alloc(&b->dt_log, sizeof(double) * (size_t)n) != hipSuccess)
     return fail(RHY_ERR_OOM, "dt log allocation failed");
   b->dt_log_cap = n;
   return RHY_OK;
 }
 
+// dt and dt/dx, (.5 dt)/dx per axis of the current step (dt_kernel writes
+// them with dt): scalars[8..14]
+double* step_consts(rhy_box* b) { return b->scalars + 8; }
+
+// The deferred-face list of axis a in the accumulating step.
+DeferList defer_list_axis(rhy_box* b, int a) {
+  DeferList dl;
+  dl.count = b->defer_count + a;
+  dl.items = b->defer_items + b->defer_off_axis[a];
+  const int64_t cap = defer_cap_setting();
+  dl.cap = cap >= 0 ? (uint32_t)std::min<int64_t>(cap, b->defer_cap_axis[a])
+                    : b->defer_cap_axis[a];
+  return dl;
+}
+
+// One accumulating walk (and its fixup) along A of an ND box: reads the
+// current state, writes Q (kAccMid) or the other state buffer (kAccLast).
+template <int ND, int A, int LIM, typename R, class PW, class P, int MODE>
+void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+                     int order) {
+  const Geom& G = b->geom;
+  const int nv = ND + 2;
+  const R* cur = static_cast<const R*>(b->cells);
+  const R* fx = static_cast<const R*>(b->flux);
+  R* q = static_cast<R*>(b->flux) + (int64_t)nv * G.plane;
+  R* other = static_cast<R*>(b->state[1 - b->cur]);
+  const R* acc = MODE == kAccLast && ND == 3 ? q : nullptr;
+  R* out = MODE == kAccMid ? q : other;
+  double* slot = b->partial + b->acc_blocks;
+  const DeferList dl = defer_list_axis(b, A);
+  TimedLaunch tl;
+  timed_begin(b, 0, tl);
+  hipLaunchKernelGGL((acc_walk_kernel<ND, A, LIM, R, PW, MODE>),
+                     dim3(grid_for(acc_threads(G, A, kAccL))), dim3(kThreads), 0, b->stream, k, G,
+                     cur, fx, acc, out, bc, step_consts(b), kAccL, order, dl, b->partial);
+  timed_end(b, tl);
+  if constexpr (PW::kDefer) {
+    TimedLaunch tf;
+    timed_begin(b, 5, tf);
+    hipLaunchKernelGGL((acc_fixup_kernel<ND, A, R, P, MODE>), dim3(kFixupBlocks), dim3(kThreads),
+                       0, b->stream, k, G, cur, fx, acc, out, bc, b->dt, dx[0], dx[A], order, dl,
+                       slot);
+    timed_end(b, tf);
+  }
+}
+
+// One step of a 2D/3D box without the flux-plane update pass: the x walk's
+// fluxes into the first flux planes (and their fixup), then the
+// accumulating walks; the new state lands in the other state buffer.
+template <int ND, int LIM, typename R, class PW, class P>
+void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+                         int order) {
+  static_assert(ND > 1, "1D boxes step through the flux planes");
+  hipLaunchKernelGGL(step_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 4,
+                     b->partial + b->acc_blocks);
+  const DeferList dlx = defer_list_axis(b, 0);
+  launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx, dlx);
+  if constexpr (PW::kDefer) {
+    TimedLaunch tf;
+    timed_begin(b, 5, tf);
+    hipLaunchKernelGGL((flux_fixup_kernel<ND, R, P>), dim3(kFixupBlocks), dim3(kThreads), 0,
+                       b->stream, k, b->geom, static_cast<const R*>(b->cells),
+                       static_cast<R*>(b->flux), b->dt, dx[0], dx[1], dx[2], dlx, 0);
+    timed_end(b, tf);
+  }
+  if constexpr (ND == 3) {
+    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccMid>(b, k, dx, bc, order);
+    launch_acc_axis<ND, 2, LIM, R, PW, P, kAccLast>(b, k, dx, bc, order);
+  } else {
+    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccLast>(b, k, dx, bc, order);
+  }
+  b->cur = 1 - b->cur;
+  b->cells = b->state[b->cur];
+}
+
+template <int ND, typename R, class PW, class P>
+void launch_acc_step_r(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+                       int order) {
+  switch (k.limiter) {
+    case 1: launch_acc_step_lim<ND, 1, R, PW, P>(b, k, dx, bc, order); break;
+    case 2: launch_acc_step_lim<ND, 2, R, PW, P>(b, k, dx, bc, order); break;
+    case 3: launch_acc_step_lim<ND, 3, R, PW, P>(b, k, dx, bc, order); break;
+    default: launch_acc_step_lim<ND, 4, R, PW, P>(b, k, dx, bc, order); break;
+  }
+}
+
+template <int ND>
+int launch_acc_step(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+                    int order) {
+  if (b->precision == RHY_MIXED)
+    launch_acc_step_r<ND, float, MixedFlux, MixedMath>(b, k, dx, bc, order);
+  else
+    launch_acc_step_r<ND, double, PowF64Flux, PowF64Fix>(b, k, dx, bc, order);
+  HIP_TRY(hipGetLastError());
+  return RHY_OK;
+}
+
+// The main loop's steps (src/main.f90:109-136) on the device.  1D boxes:
+// fluxes into planes, then the update with the fused epilogue (ghost
+// images and CFL partials of the new state).  2D/3D boxes: the
+// accumulating walks, whose last one writes the new state with the same
+// epilogue; the state buffers swap every step, so the prologue fills the
+// ghosts of both (the inflow faces, which the epilogue leaves alone, then
+// hold their values in either buffer).
 template <int ND>
 int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nsteps) {
   const Consts k = make_consts(p);
@@ -369,15 +505,24 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
   // prologue: CFL of the current state and the ghost fill it is stepped with
   if ((rc = launch_cfl<ND>(b, k))) return rc;
   if ((rc = launch_bc<ND>(b, p, -1))) return rc;
+  if constexpr (ND > 1)
+    if ((rc = launch_bc<ND>(b, p, -1, b->state[1 - b->cur]))) return rc;
   const double cmdx = p->courant * min_dx(ND, dx);
   for (int32_t s = 0; s < nsteps; ++s) {
+    // partials: the CFL pass's before the first step, the last walk's and
+    // the fixup's slot after
+    const int32_t np = ND > 1 && s > 0 ? b->acc_blocks + 1 : b->partial_n;
     TimedLaunch tl;
     timed_begin(b, 4, tl);
-    hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1024), 0, b->stream, b->partial, b->partial_n,
-                       cmdx, b->dt, b->dt_log + s, (double*)nullptr);
+    hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1024), 0, b->stream, b->partial, np, cmdx, b->dt,
+                       b->dt_log + s, (double*)nullptr, dx[0], dx[1], dx[2], step_consts(b));
     timed_end(b, tl);
-    if ((rc = launch_flux<ND>(b, k, dx))) return rc;
-    if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc))) return rc;
+    if constexpr (ND > 1) {
+      if ((rc = launch_acc_step<ND>(b, k, dx, bc, p->solver_type))) return rc;
+    } else {
+      if ((rc = launch_flux<ND>(b, k, dx))) return rc;
+      if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc))) return rc;
+    }
   }
   b->last_nsteps = nsteps;
   HIP_TRY(hipGetLastError());
@@ -386,18 +531,20 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
 
 int check_box(rhy_box* b) {
   if (!b) return fail(RHY_ERR_ARG, "box is NULL");
+  if (!b->cells) return fail(RHY_ERR_ARG, "box has no device storage");
   return RHY_OK;
 }
 
 void free_box_memory(rhy_box* b) {
-  if (b->cells) (void)hipFree(b->cells);
+  if (b->block) (void)hipFree(b->block);
   if (b->flux) (void)hipFree(b->flux);
   if (b->partial) (void)hipFree(b->partial);
   if (b->defer_count) (void)hipFree(b->defer_count);
   if (b->defer_items) (void)hipFree(b->defer_items);
   if (b->snap) (void)hipFree(b->snap);
   b->snap = nullptr;
-  b->cells = nullptr;
+  b->block = b->state[0] = b->state[1] = b->extra = b->cells = nullptr;
+  b->cur = 0;
   b->flux = nullptr;
   b->partial = nullptr;
   b->defer_count = nullptr;
@@ -410,33 +557,54 @@ int alloc_box_memory(rhy_box* b) {
   const int64_t P = b->geom.plane;
   const int nv = b->ndim + 2;
   b->esz = b->precision == RHY_MIXED ? sizeof(float) : sizeof(double);
-  if (hipMalloc(&b->cells, b->esz * (size_t)P * b->ncmp) != hipSuccess) {
+  const size_t planes = (size_t)(2 * nv + (b->ncmp - nv));
+  if (dev_alloc(&b->block, b->esz * (size_t)P * planes) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "state allocation failed");
   }
-  if (hipMalloc(&b->flux, b->esz * (size_t)P * nv * b->ndim) != hipSuccess) {
+  unsigned char* base = static_cast<unsigned char*>(b->block);
+  b->state[0] = base;
+  b->state[1] = base + b->esz * (size_t)P * nv;
+  b->extra = base + b->esz * (size_t)P * 2 * nv;
+  b->cur = 0;
+  b->cells = b->state[0];
+  if (dev_alloc(&b->flux, b->esz * (size_t)P * nv * b->ndim) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "flux scratch allocation failed");
   }
   const int64_t ncell = (int64_t)b->geom.n[0] * b->geom.n[1] * b->geom.n[2];
   b->partial_n = (int32_t)grid_for(ncell);
-  if (hipMalloc(&b->partial, sizeof(double) * (size_t)b->partial_n) != hipSuccess) {
+  b->acc_blocks =
+      b->ndim > 1 ? (int32_t)grid_for(acc_threads(b->geom, b->ndim - 1, kAccL)) : 0;
+  // one more slot: the CFL speeds of cells the fixup recomputed
+  b->partial_cap = std::max(b->partial_n, b->acc_blocks) + 1;
+  if (dev_alloc(&b->partial, sizeof(double) * (size_t)b->partial_cap) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "reduction scratch allocation failed");
   }
-  // deferred-face list: an eighth of the faces of a step, within
-  // [4096, 2^22] entries (the fixup recomputes every face past that)
-  int64_t faces = 0;
-  for (int a = 0; a < b->ndim; ++a) faces += ncell / b->geom.n[a] * (b->geom.n[a] + 1);
-  b->defer_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(faces / 8, 4096), 1 << 22);
-  if (hipMalloc(&b->defer_count, sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&b->defer_items, sizeof(uint64_t) * (size_t)b->defer_cap) != hipSuccess) {
+  // deferred-face lists: per axis an eighth of the axis' faces, within
+  // [4096, 2^22] entries (the fixup recomputes the whole axis past that);
+  // the flux-plane path uses them as one list for every axis
+  int64_t total = 0;
+  for (int a = 0; a < b->ndim; ++a) {
+    const int64_t cap = std::min<int64_t>(std::max<int64_t>(axis_faces(b->geom, a) / 8, 4096),
+                                          1 << 22);
+    b->defer_cap_axis[a] = (uint32_t)cap;
+    b->defer_off_axis[a] = total;
+    total += cap;
+  }
+  b->defer_cap = (uint32_t)std::min<int64_t>(total, 0xffffffffll);
+  if (dev_alloc(&b->defer_count, sizeof(uint32_t) * 4) != hipSuccess ||
+      dev_alloc(&b->defer_items, sizeof(uint64_t) * (size_t)total) != hipSuccess) {
     free_box_memory(b);
     return fail(RHY_ERR_OOM, "deferred-face list allocation failed");
   }
-  HIP_TRY(hipMemsetAsync(b->defer_count, 0, sizeof(uint32_t), b->stream));
   // ghosts start at 0 (rhyme_samr_bc_init.f90:61-87)
-  HIP_TRY(hipMemsetAsync(b->cells, 0, b->esz * (size_t)P * b->ncmp, b->stream));
+  if (hipMemsetAsync(b->defer_count, 0, sizeof(uint32_t) * 4, b->stream) != hipSuccess ||
+      hipMemsetAsync(b->block, 0, b->esz * (size_t)P * planes, b->stream) != hipSuccess) {
+    free_box_memory(b);
+    return fail(RHY_ERR_HIP, "clearing the new box storage failed");
+  }
   return RHY_OK;
 }
 
@@ -491,7 +659,14 @@ int rhy_init(int32_t device) {
 
 const char* rhy_last_error(void) { return g_err.c_str(); }
 
-int32_t rhy_version(void) { return 1; }
+int32_t rhy_version(void) { return 2; }
+
+int32_t rhy_build_options(void) {
+  return (RHY_FP64_POW_FAST ? RHY_OPT_POW_FAST : 0) |
+         (RHY_CONTACT_EXACT ? RHY_OPT_CONTACT_EXACT : 0) |
+         (RHY_LDS_WALK ? RHY_OPT_LDS_WALK : 0) |
+         (RHY_FP64_FAST_RIEMANN ? RHY_OPT_FAST_RIEMANN : 0);
+}
 
 int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t precision,
                    rhy_box_t** out) {
@@ -512,8 +687,8 @@ int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t pr
     delete b;
     return fail(RHY_ERR_HIP, "stream creation failed");
   }
-  if (hipMalloc(&b->scalars, sizeof(double) * (16 + 6 * RHY_MAX_CMP)) != hipSuccess ||
-      hipMalloc(&b->dt, sizeof(double)) != hipSuccess) {
+  if (dev_alloc(&b->scalars, sizeof(double) * (16 + 6 * RHY_MAX_CMP)) != hipSuccess ||
+      dev_alloc(&b->dt, sizeof(double)) != hipSuccess) {
     rhy_box_destroy(b);
     return fail(RHY_ERR_OOM, "scalar allocation failed");
   }
@@ -533,9 +708,39 @@ int rhy_box_resize(rhy_box_t* b, const int32_t dims[3]) {
   for (int d = 0; d < 3; ++d) same = same && dims[d] == b->dims[d];
   if (same) return RHY_OK;
   HIP_TRY(hipStreamSynchronize(b->stream));
+  // storage of the new shape first: if it cannot be had, the box keeps its
+  // old shape and storage and stays usable
+  rhy_box t;
+  t.ndim = b->ndim;
+  for (int d = 0; d < 3; ++d) t.dims[d] = dims[d];
+  t.ncmp = b->ncmp;
+  t.precision = b->precision;
+  t.stream = b->stream;
+  if ((rc = alloc_box_memory(&t))) return rc;
   free_box_memory(b);
   for (int d = 0; d < 3; ++d) b->dims[d] = dims[d];
-  return alloc_box_memory(b);
+  b->geom = t.geom;
+  b->esz = t.esz;
+  b->block = t.block;
+  b->state[0] = t.state[0];
+  b->state[1] = t.state[1];
+  b->extra = t.extra;
+  b->cur = t.cur;
+  b->cells = t.cells;
+  b->flux = t.flux;
+  b->partial = t.partial;
+  b->partial_n = t.partial_n;
+  b->acc_blocks = t.acc_blocks;
+  b->partial_cap = t.partial_cap;
+  b->defer_count = t.defer_count;
+  b->defer_items = t.defer_items;
+  b->defer_cap = t.defer_cap;
+  for (int a = 0; a < 3; ++a) {
+    b->defer_cap_axis[a] = t.defer_cap_axis[a];
+    b->defer_off_axis[a] = t.defer_off_axis[a];
+  }
+  b->last_nsteps = 0;
+  return RHY_OK;
 }
 
 int rhy_box_destroy(rhy_box_t* b) {
@@ -552,40 +757,62 @@ int rhy_box_destroy(rhy_box_t* b) {
   return RHY_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Host array (ncmp planes, hydro first) to or from the box's current hydro
+// buffer and its other-component planes.
+int box_copy(rhy_box* b, const double* in, double* out) {
+  const size_t P = (size_t)b->geom.plane;
+  const size_t nv = (size_t)(b->ndim + 2);
+  const size_t n = P * b->ncmp;
+  const size_t nh = P * nv;
+  const hipMemcpyKind kind = in ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  auto copy = [&](void* host, size_t esz) -> int {
+    unsigned char* h = static_cast<unsigned char*>(host);
+    void* dh = b->cells;
+    void* dx = b->extra;
+    if (in) {
+      HIP_TRY(hipMemcpyAsync(dh, h, esz * nh, kind, b->stream));
+      if (n > nh) HIP_TRY(hipMemcpyAsync(dx, h + esz * nh, esz * (n - nh), kind, b->stream));
+    } else {
+      HIP_TRY(hipMemcpyAsync(h, dh, esz * nh, kind, b->stream));
+      if (n > nh) HIP_TRY(hipMemcpyAsync(h + esz * nh, dx, esz * (n - nh), kind, b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RHY_OK;
+  };
+  if (b->precision == RHY_MIXED) {
+    std::vector<float> tmp(n);
+    if (in) {
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)in[i];
+      return copy(tmp.data(), sizeof(float));
+    }
+    int rc = copy(tmp.data(), sizeof(float));
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) out[i] = (double)tmp[i];
+    return RHY_OK;
+  }
+  return copy(in ? const_cast<double*>(in) : out, sizeof(double));
+}
+
+}  // namespace
+
+extern "C" {
+
 int rhy_box_upload(rhy_box_t* b, const double* cells) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
-  const size_t n = (size_t)b->geom.plane * b->ncmp;
-  if (b->precision == RHY_MIXED) {
-    std::vector<float> tmp(n);
-    for (size_t i = 0; i < n; ++i) tmp[i] = (float)cells[i];
-    HIP_TRY(hipMemcpyAsync(b->cells, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice,
-                           b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    return RHY_OK;
-  }
-  HIP_TRY(hipMemcpyAsync(b->cells, cells, sizeof(double) * n, hipMemcpyHostToDevice, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  return RHY_OK;
+  return box_copy(b, cells, nullptr);
 }
 
 int rhy_box_download(rhy_box_t* b, double* cells) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
-  const size_t n = (size_t)b->geom.plane * b->ncmp;
-  if (b->precision == RHY_MIXED) {
-    std::vector<float> tmp(n);
-    HIP_TRY(hipMemcpyAsync(tmp.data(), b->cells, sizeof(float) * n, hipMemcpyDeviceToHost,
-                           b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
-    for (size_t i = 0; i < n; ++i) cells[i] = (double)tmp[i];
-    return RHY_OK;
-  }
-  HIP_TRY(hipMemcpyAsync(cells, b->cells, sizeof(double) * n, hipMemcpyDeviceToHost, b->stream));
-  HIP_TRY(hipStreamSynchronize(b->stream));
-  return RHY_OK;
+  return box_copy(b, nullptr, cells);
 }
 
 int rhy_box_sync(rhy_box_t* b) {
@@ -619,10 +846,10 @@ int rhy_box_last_deferred(rhy_box_t* b, int64_t* n) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!n) return fail(RHY_ERR_ARG, "n is NULL");
-  uint32_t c = 0;
+  uint32_t c[4] = {0, 0, 0, 0};
   HIP_TRY(hipStreamSynchronize(b->stream));
-  HIP_TRY(hipMemcpy(&c, b->defer_count, sizeof(c), hipMemcpyDeviceToHost));
-  *n = c;
+  HIP_TRY(hipMemcpy(c, b->defer_count, sizeof(c), hipMemcpyDeviceToHost));
+  *n = (int64_t)c[0] + c[1] + c[2] + c[3];
   return RHY_OK;
 }
 
@@ -634,7 +861,7 @@ static int box_snapshot(rhy_box_t* b, float* out, const float* in) {
   if (!out && !in) return fail(RHY_ERR_ARG, "data is NULL");
   const int64_t blen = (int64_t)b->geom.n[0] * b->geom.n[1] * b->geom.n[2];
   const size_t bytes = sizeof(float) * (size_t)blen * b->ncmp;
-  if (!b->snap && hipMalloc(&b->snap, bytes) != hipSuccess) {
+  if (!b->snap && dev_alloc(&b->snap, bytes) != hipSuccess) {
     b->snap = nullptr;
     return fail(RHY_ERR_OOM, "snapshot staging allocation failed");
   }
@@ -643,18 +870,22 @@ static int box_snapshot(rhy_box_t* b, float* out, const float* in) {
     HIP_TRY(hipMemcpyAsync(b->snap, in, bytes, hipMemcpyHostToDevice, b->stream));
     if (b->precision == RHY_MIXED)
       hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, b->ncmp, static_cast<float*>(b->cells), b->snap);
+                         b->geom, b->ncmp, b->ndim + 2, static_cast<float*>(b->cells),
+                         static_cast<float*>(b->extra), b->snap);
     else
       hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, b->ncmp, static_cast<double*>(b->cells), b->snap);
+                         b->geom, b->ncmp, b->ndim + 2, static_cast<double*>(b->cells),
+                         static_cast<double*>(b->extra), b->snap);
     HIP_TRY(hipGetLastError());
   } else {
     if (b->precision == RHY_MIXED)
       hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, b->ncmp, static_cast<const float*>(b->cells), b->snap);
+                         b->geom, b->ncmp, b->ndim + 2, static_cast<const float*>(b->cells),
+                         static_cast<const float*>(b->extra), b->snap);
     else
       hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(kThreads), 0, b->stream,
-                         b->geom, b->ncmp, static_cast<const double*>(b->cells), b->snap);
+                         b->geom, b->ncmp, b->ndim + 2, static_cast<const double*>(b->cells),
+                         static_cast<const double*>(b->extra), b->snap);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, b->snap, bytes, hipMemcpyDeviceToHost, b->stream));
   }
@@ -678,7 +909,7 @@ int rhy_box_diagnostics(rhy_box_t* b, rhy_diag_t* out) {
   int rc;
   if ((rc = check_box(b))) return rc;
   if (!out) return fail(RHY_ERR_ARG, "out is NULL");
-  if (!b->diag && hipMalloc(&b->diag, sizeof(DiagPartial) * kDiagBlocks) != hipSuccess)
+  if (!b->diag && dev_alloc(&b->diag, sizeof(DiagPartial) * kDiagBlocks) != hipSuccess)
     return fail(RHY_ERR_OOM, "diagnostics buffer allocation failed");
   const Geom& G = b->geom;
   const unsigned nb = std::min<unsigned>(grid_for(G.plane), kDiagBlocks);
@@ -688,10 +919,12 @@ int rhy_box_diagnostics(rhy_box_t* b, rhy_diag_t* out) {
   do {                                                                                        \
     if (mixed)                                                                                \
       hipLaunchKernelGGL((diag_kernel<ND, float>), dim3(nb), dim3(kThreads), 0, b->stream, G, \
-                         b->ncmp, static_cast<const float*>(b->cells), dp);                   \
+                         b->ncmp, static_cast<const float*>(b->cells),                       \
+                         static_cast<const float*>(b->extra), dp);                            \
     else                                                                                      \
       hipLaunchKernelGGL((diag_kernel<ND, double>), dim3(nb), dim3(kThreads), 0, b->stream,   \
-                         G, b->ncmp, static_cast<const double*>(b->cells), dp);               \
+                         G, b->ncmp, static_cast<const double*>(b->cells),                    \
+                         static_cast<const double*>(b->extra), dp);                           \
   } while (0)
   if (b->ndim == 1) RHY_DIAG(1);
   else if (b->ndim == 2) RHY_DIAG(2);
@@ -724,10 +957,12 @@ int rhy_box_paint(rhy_box_t* b, const rhy_ic_t* ic) {
   do {                                                                                        \
     if (mixed)                                                                                \
       hipLaunchKernelGGL((paint_kernel<ND, float>), grid, dim3(kThreads), 0, b->stream, G, pc, \
-                         b->ncmp, static_cast<float*>(b->cells));                             \
+                         b->ncmp, static_cast<float*>(b->cells),                              \
+                         static_cast<float*>(b->extra));                                      \
     else                                                                                      \
       hipLaunchKernelGGL((paint_kernel<ND, double>), grid, dim3(kThreads), 0, b->stream, G,   \
-                         pc, b->ncmp, static_cast<double*>(b->cells));                        \
+                         pc, b->ncmp, static_cast<double*>(b->cells),                         \
+                         static_cast<double*>(b->extra));                                     \
   } while (0)
   if (b->ndim == 1) RHY_PAINT(1);
   else if (b->ndim == 2) RHY_PAINT(2);
@@ -772,9 +1007,13 @@ int rhy_cfl_time_step(rhy_box_t* b, const rhy_params_t* p, const double dx[3], d
   return RHY_OK;
 }
 
-int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double dt) {
+}  // extern "C"
+
+namespace {
+
+// The launches of one MUSCL-Hancock solve of a box, enqueued on its stream.
+int mh_solve_enqueue(rhy_box* b, const rhy_params_t* p, const double* dx, double dt) {
   int rc;
-  if ((rc = check_box(b))) return rc;
   if ((rc = check_params(p, b->ndim))) return rc;
   if (!dx) return fail(RHY_ERR_ARG, "dx is NULL");
   if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
@@ -792,7 +1031,17 @@ int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double
     if ((rc = launch_flux<3>(b, k, dx))) return rc;
     rc = launch_update<3>(b, k, dx, p->solver_type, false, bc);
   }
-  if (rc) return rc;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double dt) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if ((rc = mh_solve_enqueue(b, p, dx, dt))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
 }
@@ -867,8 +1116,16 @@ struct DevBuf {
 };
 
 int to_dev(DevBuf& d, const void* h, size_t bytes) {
-  if (hipMalloc(&d.p, bytes ? bytes : 8) != hipSuccess) return fail(RHY_ERR_OOM, "batch alloc");
+  if (dev_alloc(&d.p, bytes ? bytes : 8) != hipSuccess) return fail(RHY_ERR_OOM, "batch alloc");
   if (h && bytes) HIP_TRY(hipMemcpy(d.p, h, bytes, hipMemcpyHostToDevice));
+  return RHY_OK;
+}
+
+// axis ids of a unit batch must name an axis of the problem (1..ndim)
+int check_axes(const int32_t* axis, int64_t n, int nd) {
+  if (!axis) return fail(RHY_ERR_ARG, "axis is NULL");
+  for (int64_t q = 0; q < n; ++q)
+    if (axis[q] < 1 || axis[q] > nd) return fail(RHY_ERR_ARG, "axis out of range 1..ndim");
   return RHY_OK;
 }
 
@@ -883,31 +1140,203 @@ int batch_prelude(const rhy_params_t* p, int64_t n) {
 
 }  // namespace
 
-int rhy_riemann_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
-                      const int32_t* axis, double* u) {
+// dx/dt arrays (or nullptr: x/t = 0) for riemann_batch_kernel
+static int riemann_solve_common(const rhy_params_t* p, int64_t n, const double* l,
+                                const double* r, const int32_t* axis, const double* dx,
+                                const double* dt, double* u) {
   int rc;
   if ((rc = batch_prelude(p, n))) return rc;
   if (n == 0) return RHY_OK;
+  if (!l || !r || !u || (dx && !dt)) return fail(RHY_ERR_ARG, "array is NULL");
+  if ((rc = check_axes(axis, n, p->ndim))) return rc;
   const int nv = p->ndim + 2;
   const size_t vb = sizeof(double) * (size_t)n * nv;
-  DevBuf dl, dr, da, du;
+  DevBuf dl, dr, da, du, ddx, ddt;
   if ((rc = to_dev(dl, l, vb)) || (rc = to_dev(dr, r, vb)) ||
       (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)) || (rc = to_dev(du, nullptr, vb)))
+    return rc;
+  if (dx && ((rc = to_dev(ddx, dx, sizeof(double) * (size_t)n)) ||
+             (rc = to_dev(ddt, dt, sizeof(double) * (size_t)n))))
     return rc;
   const Consts k = make_consts(p);
   const unsigned g = grid_for(n);
   const double* L = (const double*)dl.p;
   const double* R = (const double*)dr.p;
   const int32_t* A = (const int32_t*)da.p;
+  const double* DX = (const double*)ddx.p;
+  const double* DT = (const double*)ddt.p;
   double* U = (double*)du.p;
   if (p->ndim == 1)
-    hipLaunchKernelGGL(riemann_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+    hipLaunchKernelGGL(riemann_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, DX,
+                       DT, U);
   else if (p->ndim == 2)
-    hipLaunchKernelGGL(riemann_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+    hipLaunchKernelGGL(riemann_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, DX,
+                       DT, U);
   else
-    hipLaunchKernelGGL(riemann_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, U);
+    hipLaunchKernelGGL(riemann_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n, L, R, A, DX,
+                       DT, U);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpy(u, du.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_riemann_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                      const int32_t* axis, double* u) {
+  return riemann_solve_common(p, n, l, r, axis, nullptr, nullptr, u);
+}
+
+int rhy_riemann_solve_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                            const int32_t* axis, const double* dx, const double* dt, double* u) {
+  if (n > 0 && (!dx || !dt)) return fail(RHY_ERR_ARG, "dx/dt is NULL");
+  return riemann_solve_common(p, n, l, r, axis, dx, dt, u);
+}
+
+int rhy_riemann_star_batch(const rhy_params_t* p, int64_t n, rhy_rp_solution_t* sol,
+                           const int32_t* axis) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  if (!sol) return fail(RHY_ERR_ARG, "array is NULL");
+  if ((rc = check_axes(axis, n, p->ndim))) return rc;
+  const size_t sb = sizeof(rhy_rp_solution_t) * (size_t)n;
+  DevBuf ds, da;
+  if ((rc = to_dev(ds, sol, sb)) || (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)))
+    return rc;
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  rhy_rp_solution_t* S = (rhy_rp_solution_t*)ds.p;
+  const int32_t* A = (const int32_t*)da.p;
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(rp_star_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(rp_star_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A);
+  else
+    hipLaunchKernelGGL(rp_star_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(sol, ds.p, sb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+// sampling at dx/dt (dx != nullptr) or one region per record at dxdt
+static int rp_sample_common(const rhy_params_t* p, int64_t n, const rhy_rp_solution_t* sol,
+                            const int32_t* axis, const double* dx, const double* dt,
+                            const int32_t* region, const double* dxdt, double* u) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  if (!sol || !u) return fail(RHY_ERR_ARG, "array is NULL");
+  if ((rc = check_axes(axis, n, p->ndim))) return rc;
+  const int nv = p->ndim + 2;
+  const size_t vb = sizeof(double) * (size_t)n * nv;
+  const size_t nb = sizeof(double) * (size_t)n;
+  DevBuf ds, da, du, d1, d2, dreg;
+  if ((rc = to_dev(ds, sol, sizeof(rhy_rp_solution_t) * (size_t)n)) ||
+      (rc = to_dev(da, axis, sizeof(int32_t) * (size_t)n)) || (rc = to_dev(du, nullptr, vb)))
+    return rc;
+  if (region) {
+    if ((rc = to_dev(dreg, region, sizeof(int32_t) * (size_t)n)) || (rc = to_dev(d1, dxdt, nb)))
+      return rc;
+  } else if ((rc = to_dev(d1, dx, nb)) || (rc = to_dev(d2, dt, nb))) {
+    return rc;
+  }
+  const Consts k = make_consts(p);
+  const unsigned g = grid_for(n);
+  const rhy_rp_solution_t* S = (const rhy_rp_solution_t*)ds.p;
+  const int32_t* A = (const int32_t*)da.p;
+  const double* DX = region ? nullptr : (const double*)d1.p;
+  const double* DT = region ? nullptr : (const double*)d2.p;
+  const int32_t* REG = region ? (const int32_t*)dreg.p : nullptr;
+  const double* DXDT = region ? (const double*)d1.p : nullptr;
+  double* U = (double*)du.p;
+  if (p->ndim == 1)
+    hipLaunchKernelGGL(rp_sample_batch_kernel<1>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A, DX,
+                       DT, REG, DXDT, U);
+  else if (p->ndim == 2)
+    hipLaunchKernelGGL(rp_sample_batch_kernel<2>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A, DX,
+                       DT, REG, DXDT, U);
+  else
+    hipLaunchKernelGGL(rp_sample_batch_kernel<3>, dim3(g), dim3(kThreads), 0, 0, k, n, S, A, DX,
+                       DT, REG, DXDT, U);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(u, du.p, vb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_riemann_sampling_batch(const rhy_params_t* p, int64_t n, const rhy_rp_solution_t* sol,
+                               const int32_t* axis, const double* dx, const double* dt,
+                               double* u) {
+  if (n > 0 && (!dx || !dt)) return fail(RHY_ERR_ARG, "dx/dt is NULL");
+  return rp_sample_common(p, n, sol, axis, dx, dt, nullptr, nullptr, u);
+}
+
+int rhy_riemann_region_batch(const rhy_params_t* p, int64_t n, const rhy_rp_solution_t* sol,
+                             const int32_t* region, const int32_t* axis, const double* dxdt,
+                             double* u) {
+  if (n > 0 && (!region || !dxdt)) return fail(RHY_ERR_ARG, "region/dxdt is NULL");
+  for (int64_t q = 0; q < n; ++q)
+    if (region[q] < RHY_W_L || region[q] > RHY_W_RFAN)
+      return fail(RHY_ERR_ARG, "unknown region id");
+  return rp_sample_common(p, n, sol, axis, nullptr, nullptr, region, dxdt, u);
+}
+
+static int irs_common(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                      double* p_star, double* l_f, double* r_f, double* guesses) {
+  int rc;
+  if ((rc = batch_prelude(p, n))) return rc;
+  if (n == 0) return RHY_OK;
+  if (!l || !r) return fail(RHY_ERR_ARG, "array is NULL");
+  const size_t rb = sizeof(double) * 4 * (size_t)n;
+  const size_t nb = sizeof(double) * (size_t)n;
+  DevBuf dl, dr, dp, dlf, drf, dg;
+  if ((rc = to_dev(dl, l, rb)) || (rc = to_dev(dr, r, rb))) return rc;
+  if (guesses) {
+    if ((rc = to_dev(dg, nullptr, 6 * nb))) return rc;
+  } else if ((rc = to_dev(dp, nullptr, nb)) || (rc = to_dev(dlf, nullptr, nb)) ||
+             (rc = to_dev(drf, nullptr, nb))) {
+    return rc;
+  }
+  const Consts k = make_consts(p);
+  hipLaunchKernelGGL(irs_batch_kernel, dim3(grid_for(n)), dim3(kThreads), 0, 0, k, n,
+                     (const double*)dl.p, (const double*)dr.p, (double*)dp.p, (double*)dlf.p,
+                     (double*)drf.p, (double*)dg.p);
+  HIP_TRY(hipGetLastError());
+  if (guesses) {
+    HIP_TRY(hipMemcpy(guesses, dg.p, 6 * nb, hipMemcpyDeviceToHost));
+    return RHY_OK;
+  }
+  HIP_TRY(hipMemcpy(p_star, dp.p, nb, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(l_f, dlf.p, nb, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(r_f, drf.p, nb, hipMemcpyDeviceToHost));
+  return RHY_OK;
+}
+
+int rhy_irs_exec_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                       double* p_star, double* l_f, double* r_f) {
+  if (n > 0 && (!p_star || !l_f || !r_f)) return fail(RHY_ERR_ARG, "output is NULL");
+  return irs_common(p, n, l, r, p_star, l_f, r_f, nullptr);
+}
+
+int rhy_irs_guess_batch(const rhy_params_t* p, int64_t n, const double* l, const double* r,
+                        double* guesses) {
+  if (n > 0 && !guesses) return fail(RHY_ERR_ARG, "guesses is NULL");
+  return irs_common(p, n, l, r, nullptr, nullptr, nullptr, guesses);
+}
+
+int rhy_pow_batch(int32_t policy, int64_t n, const double* x, const double* y, double* out) {
+  int rc;
+  if ((rc = rhy_init(0))) return rc;
+  if (policy < RHY_POW_PATH || policy > RHY_POW_MIXED) return fail(RHY_ERR_ARG, "unknown policy");
+  if (n < 0) return fail(RHY_ERR_ARG, "n < 0");
+  if (n == 0) return RHY_OK;
+  if (!x || !y || !out) return fail(RHY_ERR_ARG, "array is NULL");
+  const size_t nb = sizeof(double) * (size_t)n;
+  DevBuf dx, dy, dout;
+  if ((rc = to_dev(dx, x, nb)) || (rc = to_dev(dy, y, nb)) || (rc = to_dev(dout, nullptr, nb)))
+    return rc;
+  hipLaunchKernelGGL(pow_batch_kernel, dim3(grid_for(n)), dim3(kThreads), 0, 0, policy, n,
+                     (const double*)dx.p, (const double*)dy.p, (double*)dout.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpy(out, dout.p, nb, hipMemcpyDeviceToHost));
   return RHY_OK;
 }
 
@@ -915,6 +1344,7 @@ int rhy_limiter_batch(const rhy_params_t* p, int64_t n, const double* ul, const 
                       const double* ur, double* delta) {
   int rc;
   if ((rc = batch_prelude(p, n))) return rc;
+  if (p->limiter < 1 || p->limiter > 4) return fail(RHY_ERR_ARG, "unknown slope limiter");
   if (n == 0) return RHY_OK;
   const int nv = p->ndim + 2;
   const size_t vb = sizeof(double) * (size_t)n * nv;
@@ -943,6 +1373,7 @@ int rhy_half_step_batch(const rhy_params_t* p, int64_t n, const double* u, const
   int rc;
   if ((rc = batch_prelude(p, n))) return rc;
   if (n == 0) return RHY_OK;
+  if ((rc = check_axes(axis, n, p->ndim))) return rc;
   const int nv = p->ndim + 2;
   const size_t vb = sizeof(double) * (size_t)n * nv;
   DevBuf du, dd, da, dl, dr;
@@ -975,6 +1406,7 @@ int rhy_flux_batch(const rhy_params_t* p, int64_t n, const double* u, const int3
   int rc;
   if ((rc = batch_prelude(p, n))) return rc;
   if (n == 0) return RHY_OK;
+  if ((rc = check_axes(axis, n, p->ndim))) return rc;
   const int nv = p->ndim + 2;
   const size_t vb = sizeof(double) * (size_t)n * nv;
   DevBuf du, da, df;

@@ -585,7 +585,7 @@ int grow(void** ptr, size_t* cap, size_t need, size_t elem) {
   *ptr = nullptr;
   *cap = 0;
   size_t n = std::max(need, (size_t)64);
-  if (hipMalloc(ptr, n * elem) != hipSuccess) return bfail(RHY_ERR_OOM, "batch table allocation");
+  if (dev_alloc(ptr, n * elem) != hipSuccess) return bfail(RHY_ERR_OOM, "batch table allocation");
   *cap = n;
   return RHY_OK;
 }
@@ -734,7 +734,8 @@ int prologue(rhy_batch* b, const Consts& k, bool need_partials) {
       maxface = std::max(maxface, m);
     }
     hipLaunchKernelGGL((bc_kernel<ND, R>), dim3(cdiv(maxface, 256), 2 * ND), dim3(256), 0,
-                       b->stream, B.G, B.bc, b->inflow, b->ncmp, a + B.cell_off, -1);
+                       b->stream, B.G, B.bc, b->inflow, b->ncmp, a + B.cell_off,
+                       a + B.cell_off + (int64_t)(ND + 2) * B.G.plane, -1);
   }
   if (need_partials && (!b->pool || !b->partials_valid)) {
     const unsigned g = persistent_grid(b, b->nupd_tiles);
@@ -797,7 +798,7 @@ int run_t(rhy_batch* b, const rhy_params_t* p, int32_t nsteps, int32_t use_graph
     BHIP_TRY(hipStreamSynchronize(b->stream));
     if (b->dtlog) (void)hipFree(b->dtlog);
     b->dtlog = nullptr;
-    if (hipMalloc(&b->dtlog, sizeof(double) * (size_t)need) != hipSuccess)
+    if (dev_alloc(&b->dtlog, sizeof(double) * (size_t)need) != hipSuccess)
       return bfail(RHY_ERR_OOM, "dt log allocation failed");
     b->dtlog_steps = nsteps;
   }
@@ -880,12 +881,12 @@ rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t aren
   b->arena_elems = arena_elems;
   if (arena_elems > 0) b->free_blocks[0] = arena_elems;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
-      (arena_elems > 0 && hipMalloc(&b->arena, b->esz * (size_t)arena_elems) != hipSuccess) ||
-      hipMalloc(&b->dboxes, sizeof(BBox) * max_slots) != hipSuccess ||
-      hipMalloc(&b->dactive, sizeof(int32_t) * max_slots) != hipSuccess ||
-      hipMalloc(&b->dt, sizeof(double) * max_slots) != hipSuccess ||
-      hipMalloc(&b->step, sizeof(int32_t)) != hipSuccess ||
-      hipMalloc(&b->inflow, sizeof(double) * 6 * RHY_MAX_CMP) != hipSuccess) {
+      (arena_elems > 0 && dev_alloc(&b->arena, b->esz * (size_t)arena_elems) != hipSuccess) ||
+      dev_alloc(&b->dboxes, sizeof(BBox) * max_slots) != hipSuccess ||
+      dev_alloc(&b->dactive, sizeof(int32_t) * max_slots) != hipSuccess ||
+      dev_alloc(&b->dt, sizeof(double) * max_slots) != hipSuccess ||
+      dev_alloc(&b->step, sizeof(int32_t)) != hipSuccess ||
+      dev_alloc(&b->inflow, sizeof(double) * 6 * RHY_MAX_CMP) != hipSuccess) {
     rhy_batch_destroy(b);
     *rc = bfail(RHY_ERR_OOM, "batch allocation failed");
     return nullptr;
@@ -893,8 +894,8 @@ rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t aren
   // deferred-face list: about one entry per 64 arena elements, within
   // [4096, 2^20] (the fixup recomputes every face past that)
   b->defer_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(arena_elems / 64, 4096), 1 << 20);
-  if (hipMalloc(&b->defer_count, sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc(&b->defer_items, sizeof(uint64_t) * (size_t)b->defer_cap) != hipSuccess ||
+  if (dev_alloc(&b->defer_count, sizeof(uint32_t)) != hipSuccess ||
+      dev_alloc(&b->defer_items, sizeof(uint64_t) * (size_t)b->defer_cap) != hipSuccess ||
       hipMemsetAsync(b->defer_count, 0, sizeof(uint32_t), b->stream) != hipSuccess) {
     rhy_batch_destroy(b);
     *rc = bfail(RHY_ERR_OOM, "deferred-face list allocation failed");
@@ -907,12 +908,15 @@ rhy_batch* new_batch(int32_t ndim, int32_t ncmp, int32_t precision, int64_t aren
 template <int ND>
 void launch_paint(rhy_batch* b, const Geom& G, const PaintIC& ic, int64_t off) {
   const unsigned grid = cdiv(G.plane, 256);
+  const int64_t xo = off + (int64_t)(ND + 2) * G.plane;  // the non-hydro planes
   if (b->precision == RHY_MIXED)
     hipLaunchKernelGGL((paint_kernel<ND, float>), dim3(grid), dim3(256), 0, b->stream, G, ic,
-                       b->ncmp, static_cast<float*>(b->arena) + off);
+                       b->ncmp, static_cast<float*>(b->arena) + off,
+                       static_cast<float*>(b->arena) + xo);
   else
     hipLaunchKernelGGL((paint_kernel<ND, double>), dim3(grid), dim3(256), 0, b->stream, G, ic,
-                       b->ncmp, static_cast<double*>(b->arena) + off);
+                       b->ncmp, static_cast<double*>(b->arena) + off,
+                       static_cast<double*>(b->arena) + xo);
 }
 
 // Place a box: its state from host `cells`, painted from `ic`, or zero.
@@ -1153,7 +1157,7 @@ static int batch_snapshot(rhy_batch_t* b, float* out, const float* in) {
     if (b->snap) (void)hipFree(b->snap);
     b->snap = nullptr;
     b->snap_elems = 0;
-    if (hipMalloc(&b->snap, sizeof(float) * (size_t)total) != hipSuccess) {
+    if (dev_alloc(&b->snap, sizeof(float) * (size_t)total) != hipSuccess) {
       b->snap = nullptr;
       return bfail(RHY_ERR_OOM, "snapshot staging allocation failed");
     }
@@ -1168,21 +1172,27 @@ static int batch_snapshot(rhy_batch_t* b, float* out, const float* in) {
     const Geom& G = S.B.G;
     const int64_t blen = (int64_t)G.n[0] * G.n[1] * G.n[2];
     const dim3 grid = snapshot_grid(G);
+    const int nv = b->ndim + 2;
+    const int64_t co = S.B.cell_off, xo = S.B.cell_off + (int64_t)nv * G.plane;
     if (in) {
       S.needs_bc = true;
       if (b->precision == RHY_MIXED)
         hipLaunchKernelGGL(snapshot_unpack_kernel<float>, grid, dim3(256), 0, b->stream, G, b->ncmp,
-                           static_cast<float*>(b->arena) + S.B.cell_off, b->snap + ofs);
+                           nv, static_cast<float*>(b->arena) + co,
+                           static_cast<float*>(b->arena) + xo, b->snap + ofs);
       else
         hipLaunchKernelGGL(snapshot_unpack_kernel<double>, grid, dim3(256), 0, b->stream, G, b->ncmp,
-                           static_cast<double*>(b->arena) + S.B.cell_off, b->snap + ofs);
+                           nv, static_cast<double*>(b->arena) + co,
+                           static_cast<double*>(b->arena) + xo, b->snap + ofs);
     } else {
       if (b->precision == RHY_MIXED)
         hipLaunchKernelGGL(snapshot_pack_kernel<float>, grid, dim3(256), 0, b->stream, G, b->ncmp,
-                           static_cast<const float*>(b->arena) + S.B.cell_off, b->snap + ofs);
+                           nv, static_cast<const float*>(b->arena) + co,
+                           static_cast<const float*>(b->arena) + xo, b->snap + ofs);
       else
         hipLaunchKernelGGL(snapshot_pack_kernel<double>, grid, dim3(256), 0, b->stream, G, b->ncmp,
-                           static_cast<const double*>(b->arena) + S.B.cell_off, b->snap + ofs);
+                           nv, static_cast<const double*>(b->arena) + co,
+                           static_cast<const double*>(b->arena) + xo, b->snap + ofs);
     }
     ofs += blen * b->ncmp;
   }
@@ -1332,7 +1342,7 @@ static int amr_upload(rhy_batch_t* f) {
     if (f->amr_items) (void)hipFree(f->amr_items);
     f->amr_items = nullptr;
     f->amr_cap = 0;
-    if (hipMalloc(&f->amr_items, sizeof(AmrItem) * (size_t)n) != hipSuccess) {
+    if (dev_alloc(&f->amr_items, sizeof(AmrItem) * (size_t)n) != hipSuccess) {
       f->amr_items = nullptr;
       return bfail(RHY_ERR_OOM, "hierarchy work items allocation failed");
     }
@@ -1510,7 +1520,7 @@ int rhy_batch_last_deferred(rhy_batch_t* b, int64_t* n) {
 int rhy_batch_diagnostics(rhy_batch_t* b, int32_t box, rhy_diag_t* out) {
   if (!slot_ok(b, box) || !out) return bfail(RHY_ERR_ARG, "bad diagnostics request");
   constexpr int kBlocks = 256;
-  if (!b->diag && hipMalloc(&b->diag, sizeof(DiagPartial) * kBlocks) != hipSuccess)
+  if (!b->diag && dev_alloc(&b->diag, sizeof(DiagPartial) * kBlocks) != hipSuccess)
     return bfail(RHY_ERR_OOM, "diagnostics buffer allocation failed");
   const BBox& B = b->slots[box].B;
   const Geom& G = B.G;
@@ -1520,10 +1530,13 @@ int rhy_batch_diagnostics(rhy_batch_t* b, int32_t box, rhy_diag_t* out) {
   do {                                                                                       \
     if (mixed)                                                                               \
       hipLaunchKernelGGL((diag_kernel<ND, float>), dim3(nb), dim3(256), 0, b->stream, G,      \
-                         b->ncmp, static_cast<const float*>(b->arena) + B.cell_off, b->diag); \
+                         b->ncmp, static_cast<const float*>(b->arena) + B.cell_off,           \
+                         static_cast<const float*>(b->arena) + B.cell_off + (ND + 2) * G.plane, \
+                         b->diag);                                                           \
     else                                                                                     \
       hipLaunchKernelGGL((diag_kernel<ND, double>), dim3(nb), dim3(256), 0, b->stream, G,     \
                          b->ncmp, static_cast<const double*>(b->arena) + B.cell_off,          \
+                         static_cast<const double*>(b->arena) + B.cell_off + (ND + 2) * G.plane, \
                          b->diag);                                                           \
   } while (0)
   if (b->ndim == 1) RHY_BDIAG(1);

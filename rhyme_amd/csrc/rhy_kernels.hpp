@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/rhyme_hip.h"
@@ -40,6 +41,19 @@
 
 namespace rhy {
 
+// hipMalloc that leaves no error behind when it fails: a refused allocation
+// is reported through the return value (and *p = nullptr) and is not also
+// picked up by the next hipGetLastError of an unrelated launch.
+template <typename T>
+inline hipError_t dev_alloc(T** p, size_t bytes) {
+  const hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+  }
+  return e;
+}
+
 // Geometry of a ghosted box: Fortran cell (i, j, k), 1-based interior
 // indices, lives at (i-1+g0)*s0 + (j-1+g1)*s1 + (k-1+g2)*s2 inside each
 // component plane; inactive axes have n = 1 and g = 0.
@@ -62,6 +76,15 @@ __device__ __forceinline__ void load_state(const R* __restrict__ cells, int64_t 
   for (int c = 0; c < ND + 2; ++c) u[c] = cells[o + c * plane];
 }
 
+// Component c of a state whose hydro planes (c < nv) start at `hydro` and
+// whose other components (temperature, tracers) start at `extra`: the
+// single-box path keeps two hydro buffers and one set of the others, a
+// batch keeps them contiguous (extra = hydro + nv planes).
+template <typename R>
+__device__ __forceinline__ R* comp_plane(R* hydro, R* extra, int nv, int c, int64_t plane) {
+  return c < nv ? hydro + (int64_t)c * plane : extra + (int64_t)(c - nv) * plane;
+}
+
 // The LDS ring of the y/z walks: 4 cells of up to 5 doubles (or floats) for
 // each lane of a 256-thread workgroup (40 KB), one allocation per kernel
 // whatever walks the kernel instantiates.
@@ -69,6 +92,15 @@ constexpr int kWalkLanes = 256;
 __device__ __forceinline__ double* walk_ring() {
   __shared__ double ring[4 * 5 * kWalkLanes];
   return ring;
+}
+
+// The flux of the last face of each lane of a 256-thread workgroup (5
+// values), for the accumulating walks: the cell between two faces of a
+// walk takes both, and the lower one waits here across the Riemann solve of
+// the upper one instead of in registers.
+__device__ __forceinline__ double* walk_fprev() {
+  __shared__ double fp[5 * kWalkLanes];
+  return fp;
 }
 
 // Boundary types per face (left, right, bottom, top, back, front).
@@ -105,9 +137,10 @@ __device__ __forceinline__ uint64_t defer_item(uint32_t box, int axis, int64_t o
 }
 
 // After a face's Riemann solve: list the face if the lane's flag is up
-// (or, under the test hook k.defer_every, if its offset is a multiple).
+// (or, under the test hook k.defer_every, if its offset is a multiple);
+// true if it was listed.
 template <class PW>
-__device__ __forceinline__ void defer_take(const Consts& k, const DeferList& dl, uint32_t box,
+__device__ __forceinline__ bool defer_take(const Consts& k, const DeferList& dl, uint32_t box,
                                            int axis, int64_t of) {
   if constexpr (PW::kDefer) {
     uint32_t& f = defer_flag();
@@ -116,8 +149,10 @@ __device__ __forceinline__ void defer_take(const Consts& k, const DeferList& dl,
       f = 0u;
       const uint32_t i = atomicAdd(dl.count, 1u);
       if (i < dl.cap) dl.items[i] = defer_item(box, axis, of);
+      return true;
     }
   }
+  return false;
 }
 
 template <class PW>
@@ -510,17 +545,198 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(
   flux_walk<ND, A, LIM, R, PW>(k, G, cells, F, *dtp, dx, nf, t, dl, 0u);
 }
 
+// ------------------------------------------------ accumulating walks ----
+//
+// The single-box step without a separate update pass.  The x walk stores
+// its fluxes F_x (flux_kernel).  Walking y and z, a lane takes a chunk of L
+// consecutive cells c0 .. c0+L-1 and solves the L+1 faces c0-1 .. c0+L-1
+// around them (one face more than the cells, shared with the neighbouring
+// chunk), so that every cell of the chunk has both its faces in the walk:
+//   kAccMid  (y of a 3D box): Q = A + B (memory_intensive) or (u + A) + B
+//            (cpu_intensive), A = dt/dx_x (F_x[c-1] - F_x[c]) from the F_x
+//            planes, B = dt/dx_y (F_y[c-1] - F_y[c]) from the walk;
+//   kAccLast (z of a 3D box, y of a 2D box): the new state u + (Q + C) or
+//            Q + C (3D), u + (A + B) or (u + A) + B (2D), written to the
+//            other state buffer with its ghost images for the next step's
+//            fill, and its CFL speed folded into the lane's maximum.
+// These are the update's expressions operation for operation
+// (rhyme_muscl_hancock_solve_memory_intensive.f90:142-176,
+// …_solve_cpu_intensive.f90:117-136), so the state equals the flux-plane
+// path's bit for bit.  A cell beside a face the walk deferred gets a
+// placeholder here and no CFL speed; acc_fixup_kernel recomputes it.
+constexpr int kAccMid = 1, kAccLast = 2;
+
+// The accumulation of component value uc of one cell (q: its offset in the
+// component's plane) from its lower and upper face fluxes fl, fu along A;
+// stored to out[q] and returned.
+template <int ND, int A, typename R, int MODE>
+__device__ __forceinline__ R acc_one(const Geom& G, const R* __restrict__ fx,
+                                     const R* __restrict__ acc, R* __restrict__ out, int64_t q,
+                                     R uc, R fl, R fu, R dtdx0, R dtdxA, int order) {
+  const R b = dtdxA * (fl - fu);
+  R v;
+  if constexpr (MODE == kAccMid || ND == 2) {
+    const R a0 = dtdx0 * (fx[q - G.s[0]] - fx[q]);
+    if constexpr (MODE == kAccMid)
+      v = order == RHY_MEMORY_INTENSIVE ? a0 + b : (uc + a0) + b;
+    else
+      v = order == RHY_MEMORY_INTENSIVE ? uc + (a0 + b) : (uc + a0) + b;
+  } else {
+    v = order == RHY_MEMORY_INTENSIVE ? uc + (acc[q] + b) : acc[q] + b;
+  }
+  out[q] = v;
+  return v;
+}
+
+// All components of one cell (offset oc, state uc, face fluxes fl, fu);
+// the new values also into un.
+template <int ND, int A, typename R, int MODE>
+__device__ __forceinline__ void acc_cell(const Geom& G, const R* __restrict__ fx,
+                                         const R* __restrict__ acc, R* __restrict__ out,
+                                         int64_t oc, const R* uc, const R* fl, const R* fu,
+                                         R dtdx0, R dtdxA, int order, R* un) {
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c)
+    un[c] = acc_one<ND, A, R, MODE>(G, fx, acc, out, oc + c * G.plane, uc[c], fl[c], fu[c],
+                                    dtdx0, dtdxA, order);
+}
+
+template <int ND, int A, int LIM, typename R, class PW, int MODE>
+__device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
+                                                const R* __restrict__ cells,
+                                                const R* __restrict__ fx,
+                                                const R* __restrict__ acc, R* __restrict__ out,
+                                                const BcTypes& bc,
+                                                const double* __restrict__ sc, int L,
+                                                int order, int64_t t, const DeferList& dl) {
+  static_assert(A > 0 && A < ND, "the accumulating walks run along y or z");
+  constexpr int NV = ND + 2;
+  defer_clear<PW>();
+  double smax = 0.0;
+  const int nA = G.n[A];
+  const int nchunk = (nA + L - 1) / L;
+  const int e0 = G.n[0];
+  const int e1 = A == 1 ? nchunk : G.n[1];
+  const int e2 = A == 2 ? nchunk : G.n[2];
+  if (t >= (int64_t)e0 * e1 * e2) return smax;
+  int idx[3];
+  idx[0] = (int)(t % e0);
+  const int64_t rr = t / e0;
+  idx[1] = (int)(rr % e1);
+  idx[2] = (int)(rr / e1);
+  const int f0 = idx[A] * L;  // faces f0 .. f0+nf-1; cells f0+1 .. f0+nf-1
+  const int nf = nA + 1 - f0 < L + 1 ? nA + 1 - f0 : L + 1;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) idx[d] = d == A ? f0 : idx[d] + 1;
+  const int64_t o = goff(G, idx[0], idx[1], idx[2]);
+  const int64_t s = G.s[A];
+  // dt/dx and (.5 dt)/dx from the step's constants (uniform loads, held in
+  // scalar registers)
+  const R hdtdx = (R)sc[4 + A];
+  const R dtdxA = (R)sc[1 + A];
+  const R dtdx0 = (R)sc[1];
+  // cells of the walk through the lane's LDS ring (cell j from f0 in slot
+  // j & 3, the next one loaded during the half step, as in flux_walk); the
+  // lower face's flux of the next cell in the lane's walk_fprev slot
+  R* ring = reinterpret_cast<R*>(walk_ring());
+  R* fprev = reinterpret_cast<R*>(walk_fprev());
+  const int lane = threadIdx.x;
+  auto put = [&](int j, const R* v) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) ring[((j & 3) * NV + c) * kWalkLanes + lane] = v[c];
+  };
+  auto get = [&](int j, R* v) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) v[c] = ring[((j & 3) * NV + c) * kWalkLanes + lane];
+  };
+  R um[NV], u0[NV], up[NV], lw[NV], rp[NV], rn[NV];
+  load_state<ND, R>(cells, G.plane, o - s, um);
+  load_state<ND, R>(cells, G.plane, o, u0);
+  load_state<ND, R>(cells, G.plane, o + s, up);
+  half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rp);
+  put(0, u0);
+  put(1, up);
+  R nx[NV];
+  load_state<ND, R>(cells, G.plane, o + 2 * s, nx);
+  put(2, nx);
+  bool dprev = false;
+#pragma unroll 1
+  for (int m = 0; m < nf; ++m) {
+    if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
+    get(m, um);
+    get(m + 1, u0);
+    get(m + 2, up);
+    half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+    if (m + 1 < nf) put(m + 3, nx);
+    R Fr[NV];
+    riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+    const int64_t of = o + m * s;
+    const bool dm = defer_take<PW>(k, dl, 0u, A, of);
+#pragma unroll
+    for (int c = 0; c < NV; ++c) rp[c] = rn[c];
+    if (m >= 1) {  // cell f0 + m: faces m - 1 (held) and m
+      // one component at a time: its lower flux out of the lane's slot, the
+      // upper one in, so Fr dies as the new values come
+      R un[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        const R fl = fprev[c * kWalkLanes + lane];
+        fprev[c * kWalkLanes + lane] = Fr[c];
+        const R uc = ring[((m & 3) * NV + c) * kWalkLanes + lane];
+        un[c] = acc_one<ND, A, R, MODE>(G, fx, acc, out, of + c * G.plane, uc, fl, Fr[c], dtdx0,
+                                        dtdxA, order);
+      }
+      if constexpr (MODE == kAccLast) {
+        int ci[3] = {idx[0], idx[1], idx[2]};
+        ci[A] = f0 + m;
+        write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
+        if (!(dm || dprev)) {
+          double ud[NV];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
+          const double sp = cfl_speed<ND>(k, ud);
+          smax = sp > smax ? sp : smax;
+        }
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NV; ++c) fprev[c * kWalkLanes + lane] = Fr[c];
+    }
+    dprev = dm;
+  }
+  return smax;
+}
+
+// Threads of an accumulating walk along A with chunks of L cells.
+__host__ __device__ inline int64_t acc_threads(const Geom& G, int A, int L) {
+  int64_t n = (G.n[A] + L - 1) / L;
+  for (int d = 0; d < 3; ++d)
+    if (d != A) n *= G.n[d];
+  return n;
+}
+
+template <int ND, int A, int LIM, typename R, class PW, int MODE>
+__global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void acc_walk_kernel(
+    Consts k, Geom G, const R* __restrict__ cells, const R* __restrict__ fx,
+    const R* __restrict__ acc, R* __restrict__ out, BcTypes bc, const double* __restrict__ sc,
+    int L, int order, DeferList dl, double* __restrict__ partial) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double sp = flux_walk_acc<ND, A, LIM, R, PW, MODE>(k, G, cells, fx, acc, out, bc, sc, L,
+                                                          order, t, dl);
+  if constexpr (MODE == kAccLast) block_max_to_partial(sp, partial);
+}
+
 // One face's flux from scratch: the half steps of its two cells (the
 // stencil of four cells along A) and the Riemann solve with policy PW, the
 // computation the flux walks make for it.  of: offset of the low cell.
 template <int ND, int A, typename R, class PW>
-__device__ __forceinline__ void flux_one_face(const Consts& k, const Geom& G,
-                                              const R* __restrict__ cells, R* __restrict__ FA,
-                                              double dt, double dx, int64_t of) {
+__device__ __forceinline__ void face_flux(const Consts& k, const Geom& G,
+                                          const R* __restrict__ cells, double dt, double dx,
+                                          int64_t of, R* Fr) {
   constexpr int NV = ND + 2;
   const int64_t s = G.s[A];
   const R hdtdx = (R)(.5 * dt / dx);
-  R um[NV], u0[NV], up[NV], l0[NV], r0[NV], l1[NV], r1[NV], Fr[NV];
+  R um[NV], u0[NV], up[NV], l0[NV], r0[NV], l1[NV], r1[NV];
   load_state<ND, R>(cells, G.plane, of - s, um);
   load_state<ND, R>(cells, G.plane, of, u0);
   load_state<ND, R>(cells, G.plane, of + s, up);
@@ -528,6 +744,15 @@ __device__ __forceinline__ void flux_one_face(const Consts& k, const Geom& G,
   load_state<ND, R>(cells, G.plane, of + 2 * s, um);
   half_step<ND, A, 0, R>(k, u0, up, um, hdtdx, l1, r1);
   riemann_flux<ND, A, PW, R>(k, r0, l1, Fr);
+}
+
+template <int ND, int A, typename R, class PW>
+__device__ __forceinline__ void flux_one_face(const Consts& k, const Geom& G,
+                                              const R* __restrict__ cells, R* __restrict__ FA,
+                                              double dt, double dx, int64_t of) {
+  constexpr int NV = ND + 2;
+  R Fr[NV];
+  face_flux<ND, A, R, PW>(k, G, cells, dt, dx, of, Fr);
 #pragma unroll
   for (int c = 0; c < NV; ++c) FA[of + c * G.plane] = Fr[c];
 }
@@ -573,7 +798,7 @@ __global__ __launch_bounds__(256) void flux_fixup_kernel(Consts k, Geom G,
                                                          R* __restrict__ F,
                                                          const double* __restrict__ dtp,
                                                          double dx0, double dx1, double dx2,
-                                                         DeferList dl) {
+                                                         DeferList dl, int axis_only) {
   const uint32_t n = *dl.count;
   if (n == 0) return;
   const double dt = *dtp;
@@ -589,9 +814,90 @@ __global__ __launch_bounds__(256) void flux_fixup_kernel(Consts k, Geom G,
     return;
   }
   for (int a = 0; a < ND; ++a) {
+    if (axis_only >= 0 && a != axis_only) continue;
     const int64_t na = faces_of_axis(G, a);
     for (int64_t q = t0; q < na; q += stride)
       flux_face_fix<ND, R, PW>(k, G, cells, F, dt, dx, a, face_low_cell(G, a, q));
+  }
+}
+
+// The cells an accumulating walk along A left beside deferred faces,
+// recomputed with the full policy PW: both faces of each such cell from
+// scratch, then the walk's own accumulation (acc_cell); kAccLast also
+// writes the ghost images and folds the CFL speed into *slot (an extra
+// partial slot the dt reduction reads; speeds are >= 0, so their bit
+// patterns order as integers).  If the list overflowed, every cell of the
+// box.  Cells listed twice (two deferred faces) are written twice with the
+// same values.
+template <int ND, int A, typename R, class PW, int MODE>
+__device__ __forceinline__ void acc_fix_cell(const Consts& k, const Geom& G,
+                                             const R* __restrict__ cells,
+                                             const R* __restrict__ fx, const R* __restrict__ acc,
+                                             R* __restrict__ out, const BcTypes& bc, double dt,
+                                             double dx0, double dxA, int order, int64_t oc,
+                                             double* slot) {
+  constexpr int NV = ND + 2;
+  int ci[3];
+  int64_t rem = oc;
+#pragma unroll
+  for (int d = 2; d >= 0; --d) {
+    ci[d] = (int)(rem / G.s[d]) - G.g[d] + 1;
+    rem -= (int64_t)(ci[d] - 1 + G.g[d]) * G.s[d];
+  }
+  if (ci[A] < 1 || ci[A] > G.n[A]) return;
+  R fl[NV], fu[NV], uc[NV], un[NV];
+  face_flux<ND, A, R, PW>(k, G, cells, dt, dxA, oc - G.s[A], fl);
+  face_flux<ND, A, R, PW>(k, G, cells, dt, dxA, oc, fu);
+  load_state<ND, R>(cells, G.plane, oc, uc);
+  acc_cell<ND, A, R, MODE>(G, fx, acc, out, oc, uc, fl, fu, (R)(dt / dx0), (R)(dt / dxA),
+                           order, un);
+  if constexpr (MODE == kAccLast) {
+    write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
+    double ud[NV];
+#pragma unroll
+    for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
+    const double sp = cfl_speed<ND>(k, ud);
+    if (sp > 0.0)
+      atomicMax(reinterpret_cast<unsigned long long*>(slot),
+                (unsigned long long)__double_as_longlong(sp));
+  }
+}
+
+template <int ND, int A, typename R, class PW, int MODE>
+__global__ __launch_bounds__(256) void acc_fixup_kernel(
+    Consts k, Geom G, const R* __restrict__ cells, const R* __restrict__ fx,
+    const R* __restrict__ acc, R* __restrict__ out, BcTypes bc, const double* __restrict__ dtp,
+    double dx0, double dxA, int order, DeferList dl, double* slot) {
+  const uint32_t n = *dl.count;
+  if (n == 0) return;
+  const double dt = *dtp;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n <= dl.cap) {
+    for (int64_t i = t0; i < (int64_t)n; i += stride) {
+      const int64_t of = (int64_t)(dl.items[i] & ((1ull << 40) - 1));
+      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order, of,
+                                       slot);
+      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order,
+                                       of + G.s[A], slot);
+    }
+    return;
+  }
+  const int64_t total = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  for (int64_t q = t0; q < total; q += stride) {
+    const int i = (int)(q % G.n[0]) + 1;
+    const int64_t r = q / G.n[0];
+    acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order,
+                                     goff(G, i, (int)(r % G.n[1]) + 1, (int)(r / G.n[1]) + 1),
+                                     slot);
+  }
+}
+
+// Empties the deferred-face counters of a step (and the extra CFL slot).
+[[maybe_unused]] static __global__ void step_reset_kernel(uint32_t* counts, int n, double* slot) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    for (int i = 0; i < n; ++i) counts[i] = 0u;
+    if (slot) *slot = 0.0;
   }
 }
 
@@ -660,7 +966,7 @@ __global__ __launch_bounds__(256) void update_kernel(Consts k, Geom G, R* __rest
 template <int ND, typename R = double>
 __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const double* __restrict__ inflow,
                                                  int32_t ncmp, R* __restrict__ cells,
-                                                 int32_t face_only) {
+                                                 R* __restrict__ extra, int32_t face_only) {
   const int face = face_only >= 0 ? face_only : (int)blockIdx.y;
   const int a = face / 2, side = face % 2;
   if (a >= ND) return;
@@ -682,8 +988,9 @@ __global__ __launch_bounds__(256) void bc_kernel(Geom G, BcTypes bc, const doubl
   const int64_t og1 = goff(G, gi[0], gi[1], gi[2]);
   if (type == 4) {
     for (int c = 0; c < ncmp; ++c) {
-      cells[og0 + c * G.plane] = (R)inflow[face * 16 + c];
-      cells[og1 + c * G.plane] = (R)inflow[face * 16 + c];
+      R* pc = comp_plane<R>(cells, extra, ND + 2, c, G.plane);
+      pc[og0] = (R)inflow[face * 16 + c];
+      pc[og1] = (R)inflow[face * 16 + c];
     }
     return;
   }
@@ -761,7 +1068,8 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
 // raw max_u bits to maxu_out when given.
 [[maybe_unused]] static __global__ __launch_bounds__(1024) void dt_kernel(
     const double* __restrict__ partial, int32_t n, double cmdx, double* dt_out, double* dt_log,
-    double* maxu_out) {
+    double* maxu_out, double dx0 = 1.0, double dx1 = 1.0, double dx2 = 1.0,
+    double* stepc = nullptr) {
   __shared__ double wave_max[16];
   double v = strided_max(partial, 0, n, (int)threadIdx.x, (int)blockDim.x);
 #pragma unroll
@@ -779,6 +1087,15 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
       double dt = cmdx / m;
       *dt_out = dt;
       if (dt_log) *dt_log = dt;
+      if (stepc) {
+        // the walks' dt/dx and (.5 dt)/dx, as they would form them
+        const double dx[3] = {dx0, dx1, dx2};
+        stepc[0] = dt;
+        for (int a = 0; a < 3; ++a) {
+          stepc[1 + a] = dt / dx[a];
+          stepc[4 + a] = .5 * dt / dx[a];
+        }
+      }
     }
   }
 }
@@ -824,6 +1141,7 @@ __device__ __forceinline__ void min_take(double& v, int64_t& at, double ov, int6
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void diag_kernel(Geom G, int32_t ncmp,
                                                    const R* __restrict__ cells,
+                                                   const R* __restrict__ extra,
                                                    DiagPartial* __restrict__ out) {
   constexpr int NV = ND + 2;
   DiagPartial p;
@@ -861,7 +1179,7 @@ __global__ __launch_bounds__(256) void diag_kernel(Geom G, int32_t ncmp,
 #pragma unroll
     for (int c = 0; c < RHY_MAX_CMP; ++c) {
       if (c < ncmp) {
-        const double v = (double)cells[t + (int64_t)c * G.plane];
+        const double v = (double)comp_plane<const R>(cells, extra, NV, c, G.plane)[t];
         if (v != v) p.nan[c] += 1;
       }
     }
@@ -1024,7 +1342,7 @@ inline PaintIC make_paint(int32_t kind, int32_t axis, double gamma, double rho_l
 // conserved state (zero velocity; non-hydro components 0), ghosts 0.
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void paint_kernel(Geom G, PaintIC ic, int32_t ncmp,
-                                                    R* __restrict__ cells) {
+                                                    R* __restrict__ cells, R* __restrict__ extra) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.plane) return;
   int q[3];
@@ -1070,17 +1388,149 @@ __global__ __launch_bounds__(256) void paint_kernel(Geom G, PaintIC ic, int32_t 
       else if (c == ND + 1)
         v = .5 * rho * 0.0 + p / ic.gm1;
     }
-    cells[t + c * G.plane] = (R)v;
+    comp_plane<R>(cells, extra, ND + 2, c, G.plane)[t] = (R)v;
   }
 }
 
 // ------------------------------------------------------ unit batches ----
 
+// Run body(integral_constant A) for the 0-based axis of a 1-based axis id;
+// false if the id is outside 1..ND.
+template <int ND, class F>
+__device__ __forceinline__ bool with_axis(int a, F&& body) {
+  if (a == 1) {
+    body(std::integral_constant<int, 0>{});
+    return true;
+  }
+  if constexpr (ND > 1)
+    if (a == 2) {
+      body(std::integral_constant<int, 1>{});
+      return true;
+    }
+  if constexpr (ND > 2)
+    if (a == 3) {
+      body(std::integral_constant<int, 2>{});
+      return true;
+    }
+  return false;
+}
+
+// rhyme_riemann_problem_star.f90:3-66 on a flattened solution record: p*
+// from rhyme_irs_exec (floored at w_vacuum(p), :137), u* (:139-143), and for
+// each side the shock density and speed, or the fan density, sound speed,
+// head and tail speeds (:145-170).  The flux kernels form the same values,
+// for the sampled side only, inside riemann_sample.
+template <int ND, int A, class PW>
+__device__ __forceinline__ void rp_star(const Consts& k, rhy_rp_solution_t& s) {
+  double lf = 0.0, rf = 0.0;
+  double ps =
+      irs_exec<PW>(k, s.l_rho, s.l_v[A], s.l_p, s.l_cs, s.r_rho, s.r_v[A], s.r_p, s.r_cs, lf, rf);
+  s.l_f = lf;
+  s.r_f = rf;
+  if (ps < k.w_p) ps = k.w_p;
+  s.star_p = ps;
+  s.star_u = 0.5 * ((s.r_v[A] + s.l_v[A]) + (rf - lf));
+  const double ps_pl = PW::dv(ps, s.l_p);
+  const double ps_pr = PW::dv(ps, s.r_p);
+  if (ps > s.l_p) {
+    s.l_is_shock = 1;
+    s.l_shock_rho = PW::dv(s.l_rho * (k.gm1_gp1 + ps_pl), k.gm1_gp1 * ps_pl + 1.0);
+    s.l_shock_speed = s.l_v[A] - s.l_cs * PW::sq(k.gp1_2g * ps_pl + k.gm1_2g);
+  } else {
+    s.l_is_shock = 0;
+    double xa, xb;
+    PW::pw2(ps_pl, k.gm1_2g, k.g_inv, xa, xb);
+    s.l_fan_rho = s.l_rho * xb;
+    s.l_fan_cs = s.l_cs * xa;
+    s.l_fan_speedH = s.l_v[A] - s.l_cs;
+    s.l_fan_speedT = s.star_u - s.l_fan_cs;
+  }
+  if (ps > s.r_p) {
+    s.r_is_shock = 1;
+    s.r_shock_rho = PW::dv(s.r_rho * (k.gm1_gp1 + ps_pr), k.gm1_gp1 * ps_pr + 1.0);
+    s.r_shock_speed = s.r_v[A] + s.r_cs * PW::sq(k.gp1_2g * ps_pr + k.gm1_2g);
+  } else {
+    s.r_is_shock = 0;
+    double xa, xb;
+    PW::pw2(ps_pr, k.gm1_2g, k.g_inv, xa, xb);
+    s.r_fan_rho = s.r_rho * xb;
+    s.r_fan_cs = s.r_cs * xa;
+    s.r_fan_speedH = s.r_v[A] + s.r_cs;
+    s.r_fan_speedT = s.star_u + s.r_fan_cs;
+  }
+}
+
+// The conserved state of one region of a solved Riemann problem (region
+// ids RHY_W_*): w_k of a side, the star states behind a shock or a fan
+// (tangential velocity of that side, normal velocity u*), or the state
+// inside a fan at x/t = dxdt (w_kfan_prim, the flux kernels' routine).
+template <int ND, int A, class PW>
+__device__ __forceinline__ void rp_region(const Consts& k, const rhy_rp_solution_t& s,
+                                          int region, double dxdt, double* u) {
+  const bool right = region == RHY_W_R || region == RHY_W_STARR_SHO ||
+                     region == RHY_W_STARR_FAN || region == RHY_W_RFAN;
+  double v[ND];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) v[d] = right ? s.r_v[d] : s.l_v[d];
+  double rho = right ? s.r_rho : s.l_rho;
+  double p = right ? s.r_p : s.l_p;
+  switch (region) {
+    case RHY_W_STARL_SHO:
+    case RHY_W_STARR_SHO:
+      rho = right ? s.r_shock_rho : s.l_shock_rho;
+      v[A] = s.star_u;
+      p = s.star_p;
+      break;
+    case RHY_W_STARL_FAN:
+    case RHY_W_STARR_FAN:
+      rho = right ? s.r_fan_rho : s.l_fan_rho;
+      v[A] = s.star_u;
+      p = s.star_p;
+      break;
+    case RHY_W_LFAN:
+    case RHY_W_RFAN: {
+      double sv[ND];
+#pragma unroll
+      for (int d = 0; d < ND; ++d) sv[d] = v[d];
+      w_kfan_prim<ND, A, PW>(k, rho, sv, p, right ? s.r_cs : s.l_cs, dxdt, right, rho, v, p);
+      break;
+    }
+    default:  // RHY_W_L, RHY_W_R
+      break;
+  }
+  prim_to_cons<ND, double>(k, rho, v, p, u);
+}
+
+// rhyme_riemann_problem_sampling.f90:13-47: the region of x/t = dxdt.
+template <int ND, int A, class PW>
+__device__ __forceinline__ void rp_sampling(const Consts& k, const rhy_rp_solution_t& s,
+                                            double dxdt, double* u) {
+  int region;
+  if (dxdt > s.star_u) {
+    if (s.r_is_shock)
+      region = dxdt < s.r_shock_speed ? RHY_W_STARR_SHO : RHY_W_R;
+    else
+      region = dxdt < s.r_fan_speedT ? RHY_W_STARR_FAN
+                                     : (dxdt < s.r_fan_speedH ? RHY_W_RFAN : RHY_W_R);
+  } else {
+    if (s.l_is_shock)
+      region = dxdt > s.l_shock_speed ? RHY_W_STARL_SHO : RHY_W_L;
+    else
+      region = dxdt > s.l_fan_speedT ? RHY_W_STARL_FAN
+                                     : (dxdt > s.l_fan_speedH ? RHY_W_LFAN : RHY_W_L);
+  }
+  rp_region<ND, A, PW>(k, s, region, dxdt, u);
+}
+
+// rhyme_riemann_problem_solve at x/t = dx[q] / dt[q] (0: the kernels' case),
+// with the flux kernels' full solver policy.
 template <int ND>
 __global__ __launch_bounds__(256) void riemann_batch_kernel(Consts k, int64_t n,
                                                             const double* __restrict__ l,
                                                             const double* __restrict__ r,
                                                             const int32_t* __restrict__ axis,
+                                                            const double* __restrict__ dx,
+                                                            const double* __restrict__ dt,
                                                             double* __restrict__ u) {
   constexpr int NV = ND + 2;
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1091,16 +1541,95 @@ __global__ __launch_bounds__(256) void riemann_batch_kernel(Consts k, int64_t n,
     L[c] = l[q * NV + c];
     R[c] = r[q * NV + c];
   }
-  const int a = axis[q];
-  if (a == 1) riemann_state<ND, 0, PowF64>(k, L, R, U);
-  else if (ND > 1 && a == 2) riemann_state<ND, (ND > 1 ? 1 : 0), PowF64>(k, L, R, U);
-  else if (ND > 2 && a == 3) riemann_state<ND, (ND > 2 ? 2 : 0), PowF64>(k, L, R, U);
-  else {
+  const double dxdt = dx ? dx[q] / dt[q] : 0.0;
+  if (!with_axis<ND>(axis[q], [&](auto a) {
+        riemann_state<ND, decltype(a)::value, PowF64Fix>(k, L, R, dxdt, U);
+      })) {
 #pragma unroll
     for (int c = 0; c < NV; ++c) U[c] = __builtin_nan("");
   }
 #pragma unroll
   for (int c = 0; c < NV; ++c) u[q * NV + c] = U[c];
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void rp_star_batch_kernel(Consts k, int64_t n,
+                                                            rhy_rp_solution_t* __restrict__ sol,
+                                                            const int32_t* __restrict__ axis) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  rhy_rp_solution_t s = sol[q];
+  if (!with_axis<ND>(axis[q], [&](auto a) { rp_star<ND, decltype(a)::value, PowF64Fix>(k, s); }))
+    s.star_p = s.star_u = __builtin_nan("");
+  sol[q] = s;
+}
+
+// Sampling (dx != nullptr: x/t = dx[q] / dt[q]) or one given region
+// (region != nullptr, x/t = dxdt[q] for the fans).
+template <int ND>
+__global__ __launch_bounds__(256) void rp_sample_batch_kernel(
+    Consts k, int64_t n, const rhy_rp_solution_t* __restrict__ sol,
+    const int32_t* __restrict__ axis, const double* __restrict__ dx,
+    const double* __restrict__ dt, const int32_t* __restrict__ region,
+    const double* __restrict__ dxdt, double* __restrict__ u) {
+  constexpr int NV = ND + 2;
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const rhy_rp_solution_t s = sol[q];
+  double U[NV];
+  const bool ok = with_axis<ND>(axis[q], [&](auto a) {
+    constexpr int A = decltype(a)::value;
+    if (region)
+      rp_region<ND, A, PowF64Fix>(k, s, region[q], dxdt[q], U);
+    else
+      rp_sampling<ND, A, PowF64Fix>(k, s, dx[q] / dt[q], U);
+  });
+  if (!ok) {
+#pragma unroll
+    for (int c = 0; c < NV; ++c) U[c] = __builtin_nan("");
+  }
+#pragma unroll
+  for (int c = 0; c < NV; ++c) u[q * NV + c] = U[c];
+}
+
+// rhyme_irs_exec on records (rho, v(axis), p, cs); guesses != nullptr: the
+// six starting guesses instead (rhyme_irs_guess_p_star: guess 1 as the
+// flux kernels form it, guesses 2-6 as their fallback forms them).
+__global__ __launch_bounds__(256) static void irs_batch_kernel(
+    Consts k, int64_t n, const double* __restrict__ l, const double* __restrict__ r,
+    double* __restrict__ p_star, double* __restrict__ l_f, double* __restrict__ r_f,
+    double* __restrict__ guesses) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const double lr = l[4 * q], lv = l[4 * q + 1], lp = l[4 * q + 2], lc = l[4 * q + 3];
+  const double rr = r[4 * q], rv = r[4 * q + 1], rp = r[4 * q + 2], rc = r[4 * q + 3];
+  if (guesses) {
+    guesses[6 * q] = guess1<PowF64Fix>(k, lr, lv, lp, lc, rr, rv, rp, rc);
+    for (int i = 1; i < 6; ++i)
+      guesses[6 * q + i] = guess_n<PowF64Fix>(k, i, lr, lv, lp, lc, rr, rv, rp, rc);
+    return;
+  }
+  double lf = 0.0, rf = 0.0;
+  p_star[q] = irs_exec<PowF64Fix>(k, lr, lv, lp, lc, rr, rv, rp, rc, lf, rf);
+  l_f[q] = lf;
+  r_f[q] = rf;
+}
+
+// x**y under one of the library's policies (RHY_POW_*)
+__global__ __launch_bounds__(256) static void pow_batch_kernel(int32_t policy, int64_t n,
+                                                               const double* __restrict__ x,
+                                                               const double* __restrict__ y,
+                                                               double* __restrict__ out) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double v;
+  if (policy == RHY_POW_LIBM)
+    v = pow(x[q], y[q]);
+  else if (policy == RHY_POW_MIXED)
+    v = MixedMath::pw(x[q], y[q]);
+  else
+    v = PowF64Fix::pw(x[q], y[q]);
+  out[q] = v;
 }
 
 template <int ND>
@@ -1199,8 +1728,9 @@ inline dim3 snapshot_grid(const Geom& G) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, int ncmp,
+__global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, int ncmp, int nv,
                                                             const R* __restrict__ cells,
+                                                            const R* __restrict__ extra,
                                                             float* __restrict__ out) {
   const int chunks = (G.n[0] + 255) >> 8;
   const int row = (int)blockIdx.x / chunks;
@@ -1212,7 +1742,7 @@ __global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, int ncmp,
   R v[RHY_MAX_CMP];
 #pragma unroll
   for (int c = 0; c < RHY_MAX_CMP; ++c)
-    if (c < ncmp) v[c] = cells[src + c * G.plane];
+    if (c < ncmp) v[c] = comp_plane<const R>(cells, extra, nv, c, G.plane)[src];
 #pragma unroll
   for (int c = 0; c < RHY_MAX_CMP; ++c)
     if (c < ncmp) out[c * blen + dst] = (float)v[c];
@@ -1221,8 +1751,9 @@ __global__ __launch_bounds__(256) void snapshot_pack_kernel(Geom G, int ncmp,
 // The reverse (the restart reader, rhyme_initial_condition_load_rhyme.f90:57-76):
 // interior cells from an FP32 payload, widened exactly; ghosts untouched.
 template <typename R>
-__global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, int ncmp,
+__global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, int ncmp, int nv,
                                                               R* __restrict__ cells,
+                                                              R* __restrict__ extra,
                                                               const float* __restrict__ in) {
   const int chunks = (G.n[0] + 255) >> 8;
   const int row = (int)blockIdx.x / chunks;
@@ -1237,7 +1768,7 @@ __global__ __launch_bounds__(256) void snapshot_unpack_kernel(Geom G, int ncmp,
     if (c < ncmp) v[c] = in[c * blen + src];
 #pragma unroll
   for (int c = 0; c < RHY_MAX_CMP; ++c)
-    if (c < ncmp) cells[dst + c * G.plane] = (R)v[c];
+    if (c < ncmp) comp_plane<R>(cells, extra, nv, c, G.plane)[dst] = (R)v[c];
 }
 
 }  // namespace rhy

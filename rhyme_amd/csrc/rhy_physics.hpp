@@ -682,15 +682,19 @@ __device__ __forceinline__ void side_prims(const Consts& k, const RI* s, double*
   c = dmax((double)speed_of_sound<ND, RI>(k, s), kTiny);
 }
 
-// rhyme_riemann_problem_solve.f90:3-104 at dx = 0 (Godunov state, x/t = 0):
-// the primitive state (rho, v, p) sampled on the interface.  l is the right
-// half-step state of the cell on the low side of the interface, r the left
-// half-step state of the cell on the high side.  Every branch of the
-// reference ends in conv_prim_vars_to_cons of the state chosen here.
+// rhyme_riemann_problem_solve.f90:3-104: the primitive state (rho, v, p)
+// sampled at x/t = dxdt (dx / dt of the reference's arguments, :17).  The
+// MUSCL-Hancock loops call it with dx = 0, i.e. dxdt = 0, the Godunov state
+// on the interface (…_solve_memory_intensive.f90:110-114); the flux kernels
+// pass the constant 0, which folds away every comparison's other operand.
+// l is the right half-step state of the cell on the low side of the
+// interface, r the left half-step state of the cell on the high side.  Every
+// branch of the reference ends in conv_prim_vars_to_cons of the state chosen
+// here.
 template <int ND, int A, class PW, typename RI>
 __device__ __forceinline__ void riemann_sample(const Consts& k, const RI* l, const RI* r,
-                                               double& srho, double* sv, double& sp) {
-  const double dxdt = 0.0;  // dx/dt with dx = 0 and dt > 0
+                                               double dxdt, double& srho, double* sv,
+                                               double& sp) {
   const double l_rho = dmax((double)l[0], k.w_rho);
   const double r_rho = dmax((double)r[0], k.w_rho);
   const double e_rho = k.e_rho;
@@ -796,12 +800,12 @@ __device__ __forceinline__ void riemann_sample(const Consts& k, const RI* l, con
   }
 }
 
-// The conserved state sampled on the interface (FP64 unit-level solve).
+// The conserved state sampled at x/t = dxdt (FP64 unit-level solve).
 template <int ND, int A, class PW = PowLib>
 __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, const double* r,
-                                              double* u) {
+                                              double dxdt, double* u) {
   double srho, sv[ND], sp;
-  riemann_sample<ND, A, PW, double>(k, l, r, srho, sv, sp);
+  riemann_sample<ND, A, PW, double>(k, l, r, dxdt, srho, sv, sp);
   prim_to_cons<ND, double>(k, srho, sv, sp, u);
 }
 
@@ -813,7 +817,7 @@ __device__ __forceinline__ void riemann_state(const Consts& k, const double* l, 
 template <int ND, int A, class PW, typename R>
 __device__ __forceinline__ void riemann_flux(const Consts& k, const R* l, const R* r, R* F) {
   double srho, sv[ND], sp;
-  riemann_sample<ND, A, PW, R>(k, l, r, srho, sv, sp);
+  riemann_sample<ND, A, PW, R>(k, l, r, 0.0, srho, sv, sp);
   R v[ND], u[ND + 2];
 #pragma unroll
   for (int d = 0; d < ND; ++d) v[d] = (R)sv[d];

@@ -1,9 +1,10 @@
 """Build the Fortran host modules (rhy_capi.f90, rhyme_host.f90) with flang
-for NDIM = 1, 2, 3, and the host driver used by the tests
-(tests/fhost/rhy_fhost_driver.f90), linked against librhyme_hip.so.
+for NDIM = 1, 2, 3, and the host drivers used by the tests
+(tests/fhost/rhy_fhost_driver.f90, the main loop; tests/fhost/rhy_fhost_units.f90,
+the unit procedures and forced errors), linked against librhyme_hip.so.
 
 Outputs under rhyme_amd/fhost/_build/<N>d/: the .mod files, librhyme_fhost.a
-and the executable rhy_fhost_driver.exe.
+and the executables rhy_fhost_driver.exe and rhy_fhost_units.exe.
 """
 import os
 import subprocess
@@ -15,6 +16,7 @@ PKG = os.path.dirname(HERE)
 ROOT = os.path.dirname(PKG)
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 DRIVER = os.path.join(ROOT, "tests", "fhost", "rhy_fhost_driver.f90")
+UNITS = os.path.join(ROOT, "tests", "fhost", "rhy_fhost_units.f90")
 
 
 def run(cmd):
@@ -38,10 +40,11 @@ def build_one(nd):
         os.remove(lib)
     run(["ar", "rcs", lib] + objs)
     exe = os.path.join(out, "rhy_fhost_driver.exe")
-    if os.path.exists(DRIVER):
-        run([FLANG] + fl + [DRIVER, lib, "-L", PKG, "-lrhyme_hip",
-                            "-Wl,-rpath,$ORIGIN/../../..", "-Wl,-rpath,/opt/rocm/lib",
-                            "-o", exe])
+    for src, name in ((DRIVER, "rhy_fhost_driver.exe"), (UNITS, "rhy_fhost_units.exe")):
+        if os.path.exists(src):
+            run([FLANG] + fl + [src, lib, "-L", PKG, "-lrhyme_hip",
+                                "-Wl,-rpath,$ORIGIN/../../..", "-Wl,-rpath,/opt/rocm/lib",
+                                "-o", os.path.join(out, name)])
     return exe
 
 

@@ -32,6 +32,24 @@ module rhy_capi
       real(c_double) :: cons_inflows(RHY_MAX_CMP, 6) = 0
    end type rhy_params_t
 
+   ! riemann_problem_solution_t flattened (include/rhyme_hip.h rhy_rp_solution_t)
+   type, bind(C) :: rhy_rp_solution_t
+      real(c_double) :: l_rho = 0, l_v(3) = 0, l_p = 0, l_cs = 0
+      real(c_double) :: r_rho = 0, r_v(3) = 0, r_p = 0, r_cs = 0
+      real(c_double) :: star_p = 0, star_u = 0
+      integer(c_int32_t) :: l_is_shock = 0, r_is_shock = 0
+      real(c_double) :: l_shock_rho = 0, l_shock_speed = 0
+      real(c_double) :: l_fan_rho = 0, l_fan_cs = 0, l_fan_speedH = 0, l_fan_speedT = 0
+      real(c_double) :: r_shock_rho = 0, r_shock_speed = 0
+      real(c_double) :: r_fan_rho = 0, r_fan_cs = 0, r_fan_speedH = 0, r_fan_speedT = 0
+      real(c_double) :: l_f = 0, r_f = 0
+   end type rhy_rp_solution_t
+
+   ! region ids of rhy_riemann_region_batch
+   integer(c_int32_t), parameter :: RHY_W_L = 1, RHY_W_R = 2, RHY_W_STARL_SHO = 3, &
+                                    RHY_W_STARR_SHO = 4, RHY_W_STARL_FAN = 5, &
+                                    RHY_W_STARR_FAN = 6, RHY_W_LFAN = 7, RHY_W_RFAN = 8
+
    type, bind(C) :: rhy_ic_t
       integer(c_int32_t) :: kind = 0
       integer(c_int32_t) :: axis = 0
@@ -122,6 +140,66 @@ module rhy_capi
          integer(c_int32_t), intent(in) :: axis(*)
          real(c_double), intent(out) :: u(*)
       end function rhy_riemann_batch
+
+      integer(c_int) function rhy_riemann_solve_batch(p, n, l, r, axis, dx, dt, u) &
+         bind(C, name="rhy_riemann_solve_batch")
+         import :: c_int, c_int64_t, c_int32_t, c_double, rhy_params_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         real(c_double), intent(in) :: l(*), r(*)
+         integer(c_int32_t), intent(in) :: axis(*)
+         real(c_double), intent(in) :: dx(*), dt(*)
+         real(c_double), intent(out) :: u(*)
+      end function rhy_riemann_solve_batch
+
+      integer(c_int) function rhy_riemann_star_batch(p, n, sol, axis) &
+         bind(C, name="rhy_riemann_star_batch")
+         import :: c_int, c_int64_t, c_int32_t, rhy_params_t, rhy_rp_solution_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         type(rhy_rp_solution_t), intent(inout) :: sol(*)
+         integer(c_int32_t), intent(in) :: axis(*)
+      end function rhy_riemann_star_batch
+
+      integer(c_int) function rhy_riemann_sampling_batch(p, n, sol, axis, dx, dt, u) &
+         bind(C, name="rhy_riemann_sampling_batch")
+         import :: c_int, c_int64_t, c_int32_t, c_double, rhy_params_t, rhy_rp_solution_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         type(rhy_rp_solution_t), intent(in) :: sol(*)
+         integer(c_int32_t), intent(in) :: axis(*)
+         real(c_double), intent(in) :: dx(*), dt(*)
+         real(c_double), intent(out) :: u(*)
+      end function rhy_riemann_sampling_batch
+
+      integer(c_int) function rhy_riemann_region_batch(p, n, sol, region, axis, dxdt, u) &
+         bind(C, name="rhy_riemann_region_batch")
+         import :: c_int, c_int64_t, c_int32_t, c_double, rhy_params_t, rhy_rp_solution_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         type(rhy_rp_solution_t), intent(in) :: sol(*)
+         integer(c_int32_t), intent(in) :: region(*), axis(*)
+         real(c_double), intent(in) :: dxdt(*)
+         real(c_double), intent(out) :: u(*)
+      end function rhy_riemann_region_batch
+
+      integer(c_int) function rhy_irs_exec_batch(p, n, l, r, p_star, l_f, r_f) &
+         bind(C, name="rhy_irs_exec_batch")
+         import :: c_int, c_int64_t, c_double, rhy_params_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         real(c_double), intent(in) :: l(*), r(*)
+         real(c_double), intent(out) :: p_star(*), l_f(*), r_f(*)
+      end function rhy_irs_exec_batch
+
+      integer(c_int) function rhy_irs_guess_batch(p, n, l, r, guesses) &
+         bind(C, name="rhy_irs_guess_batch")
+         import :: c_int, c_int64_t, c_double, rhy_params_t
+         type(rhy_params_t), intent(in) :: p
+         integer(c_int64_t), value :: n
+         real(c_double), intent(in) :: l(*), r(*)
+         real(c_double), intent(out) :: guesses(*)
+      end function rhy_irs_guess_batch
 
       integer(c_int) function rhy_limiter_batch(p, n, ul, u, ur, delta) &
          bind(C, name="rhy_limiter_batch")

@@ -57,6 +57,9 @@ module rhyme_logger
    ! rhyme_logger/src/rhyme_logger.f90:24-68): begin/end_section, log, warn, err.
    implicit none
 
+   integer, save :: rhy_host_nerr = 0
+   character(len=1024), save :: rhy_host_last_err = ''
+
    type logger_t
       integer :: secid = 0
       logical :: quiet = .true.
@@ -103,6 +106,19 @@ contains
       logger%last_err = message
       write (0, '(A,A)') 'error: ', trim(message)
    end subroutine logger_err
+
+   ! Errors of the procedures whose reference interfaces take no logger
+   ! (rhyme_cfl_time_step, rhyme_samr_bc_set_*, the two solver variants, the
+   ! unit procedures of the Riemann solver, limiter and half step): printed
+   ! like logger%err and counted here, readable through rhy_host_nerr and
+   ! rhy_host_last_err.  The host arrays such a procedure would have written
+   ! are left as they were.
+   subroutine rhy_host_error(message)
+      character(len=*), intent(in) :: message
+      rhy_host_nerr = rhy_host_nerr + 1
+      rhy_host_last_err = message
+      write (0, '(A,A)') 'error: ', trim(message)
+   end subroutine rhy_host_error
 end module rhyme_logger
 
 module rhyme_thermo_base
@@ -250,6 +266,7 @@ module rhy_device
    use, intrinsic :: iso_c_binding
    use rhy_capi
    use rhyme_samr
+   use rhyme_logger
    implicit none
 
    type rhy_slot_t
@@ -316,7 +333,12 @@ contains
       character(len=:), allocatable :: msg
       integer(c_int) :: rc
       h = rhy_box_handle(box, msg)
-      if (c_associated(h)) rc = rhy_box_upload(h, c_loc(box%cells))
+      if (.not. c_associated(h)) then
+         call rhy_host_error('rhyme_samr_upload: '//msg)
+         return
+      end if
+      rc = rhy_box_upload(h, c_loc(box%cells))
+      if (rc /= RHY_OK) call rhy_host_error('rhyme_samr_upload: '//rhy_error_text())
    end subroutine rhyme_samr_box_upload
 
    subroutine rhyme_samr_box_download(box)
@@ -325,7 +347,12 @@ contains
       character(len=:), allocatable :: msg
       integer(c_int) :: rc
       h = rhy_box_handle(box, msg)
-      if (c_associated(h)) rc = rhy_box_download(h, c_loc(box%cells))
+      if (.not. c_associated(h)) then
+         call rhy_host_error('rhyme_samr_download: '//msg)
+         return
+      end if
+      rc = rhy_box_download(h, c_loc(box%cells))
+      if (rc /= RHY_OK) call rhy_host_error('rhyme_samr_download: '//rhy_error_text())
    end subroutine rhyme_samr_box_download
 
    subroutine rhyme_samr_upload(samr)
@@ -402,6 +429,7 @@ module rhyme_slope_limiter
    use rhy_capi
    use rhyme_units
    use rhyme_thermo_base
+   use rhyme_logger
    implicit none
 
    type, private :: slope_limiter_indices_t
@@ -430,17 +458,122 @@ contains
       p%kb_amu = get_kb_amu()
       p%ncmp = NCMP
       rc = rhy_limiter_batch(p, 1_c_int64_t, ul, u, ur, delta)
-      if (rc /= RHY_OK) delta = 0d0
+      if (rc /= RHY_OK) then
+         delta = 0d0
+         call rhy_host_error('rhyme_slope_limiter_run: '//rhy_error_text())
+      end if
    end subroutine rhyme_slope_limiter_run
 end module rhyme_slope_limiter
 
-module rhyme_riemann_problem
-   ! rhyme_riemann_problem/src/rhyme_riemann_problem.f90:18-31 and
-   ! rhyme_irs/src/rhyme_irs.f90:16-19
+module rhyme_irs
+   ! rhyme_riemann_problem/src/rhyme_irs/src/rhyme_irs.f90:16-52: the exact
+   ! iterative Riemann solver's p* and its starting guesses, on the device.
+   ! The gamma constants the reference keeps module-private (:9-14) come from
+   ! rhyme_thermo_base at each call.  rhyme_irs_guess_p_star is not pure here
+   ! (it calls the device); its only reference caller is rhyme_irs_exec.
    use, intrinsic :: iso_c_binding
    use rhy_capi
    use rhyme_units
    use rhyme_thermo_base
+   use rhyme_logger
+   implicit none
+
+   type irs_t
+      integer :: n_iteration = 10000
+      real(kind=8) :: tolerance = 1d-8
+   end type irs_t
+
+contains
+
+   subroutine rhyme_irs_init(irs, thermo, logger)
+      type(irs_t), intent(inout) :: irs
+      type(thermo_base_t), intent(in) :: thermo
+      type(logger_t), intent(inout) :: logger
+      integer(c_int) :: rc
+      rc = rhy_init(0_c_int32_t)
+      if (rc /= RHY_OK) call logger%err('rhy_init: '//rhy_error_text())
+   end subroutine rhyme_irs_init
+
+   subroutine rhy_irs_params(irs, p)
+      type(irs_t), intent(in) :: irs
+      type(rhy_params_t), intent(inout) :: p
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%n_iteration = irs%n_iteration
+      p%tolerance = irs%tolerance
+      p%ncmp = NCMP
+   end subroutine rhy_irs_params
+
+   ! rhyme_irs_exec.f90:3-45.  On a device error p_star is 0, l_f and r_f
+   ! keep their values, and the error is reported (rhy_host_error).
+   function rhyme_irs_exec(irs, l_rho, l_v, l_p, l_cs, l_f, r_rho, r_v, r_p, r_cs, r_f, &
+                           axis) result(p_star)
+      type(irs_t), intent(in) :: irs
+      real(kind=8), intent(in) :: l_rho, l_v(NDIM), l_p, l_cs, r_rho, r_v(NDIM), r_p, r_cs
+      real(kind=8), intent(inout) :: l_f, r_f
+      integer, intent(in) :: axis
+      real(kind=8) :: p_star
+      type(rhy_params_t) :: p
+      real(c_double) :: l(4), r(4), ps(1), lf(1), rf(1)
+      integer(c_int) :: rc
+      p_star = 0d0
+      if (axis < 1 .or. axis > NDIM) then
+         call rhy_host_error('rhyme_irs_exec: axis out of range')
+         return
+      end if
+      call rhy_irs_params(irs, p)
+      l = [l_rho, l_v(axis), l_p, l_cs]
+      r = [r_rho, r_v(axis), r_p, r_cs]
+      rc = rhy_irs_exec_batch(p, 1_c_int64_t, l, r, ps, lf, rf)
+      if (rc /= RHY_OK) then
+         call rhy_host_error('rhyme_irs_exec: '//rhy_error_text())
+         return
+      end if
+      p_star = ps(1)
+      l_f = lf(1)
+      r_f = rf(1)
+   end function rhyme_irs_exec
+
+   ! rhyme_irs_guess_p_star.f90:3-72 (irs supplies nothing to the guesses in
+   ! the reference; it is not an argument there either).
+   function rhyme_irs_guess_p_star(l_rho, l_v, l_p, l_cs, r_rho, r_v, r_p, r_cs, axis) &
+      result(p_star)
+      real(kind=8), intent(in) :: l_rho, l_v(NDIM), l_p, l_cs, r_rho, r_v(NDIM), r_p, r_cs
+      integer, intent(in) :: axis
+      real(kind=8) :: p_star(6)
+      type(rhy_params_t) :: p
+      real(c_double) :: l(4), r(4), g(6)
+      integer(c_int) :: rc
+      p_star = 0d0
+      if (axis < 1 .or. axis > NDIM) then
+         call rhy_host_error('rhyme_irs_guess_p_star: axis out of range')
+         return
+      end if
+      call rhy_irs_params(irs_t(), p)
+      l = [l_rho, l_v(axis), l_p, l_cs]
+      r = [r_rho, r_v(axis), r_p, r_cs]
+      rc = rhy_irs_guess_batch(p, 1_c_int64_t, l, r, g)
+      if (rc /= RHY_OK) then
+         call rhy_host_error('rhyme_irs_guess_p_star: '//rhy_error_text())
+         return
+      end if
+      p_star = g
+   end function rhyme_irs_guess_p_star
+end module rhyme_irs
+
+module rhyme_riemann_problem
+   ! rhyme_riemann_problem/src/rhyme_riemann_problem.f90:18-133: the types of
+   ! the Riemann problem and its solution, and the module procedures, each
+   ! one call into the device library.  The sampling, star-state and region
+   ! functions the reference declares pure are not pure here (they call the
+   ! device); their reference callers (rhyme_riemann_problem_solve and the
+   ! tests) do not rely on purity.
+   use, intrinsic :: iso_c_binding
+   use rhy_capi
+   use rhyme_units
+   use rhyme_thermo_base
+   use rhyme_irs
    use rhyme_logger
    implicit none
 
@@ -450,11 +583,6 @@ module rhyme_riemann_problem
 
    type(riemann_problem_indices_t), parameter :: rpid = riemann_problem_indices_t()
 
-   type irs_t
-      integer :: n_iteration = 10000
-      real(kind=8) :: tolerance = 1d-8
-   end type irs_t
-
    type riemann_problem_t
       real(kind=8), dimension(cid%rho:cid%p) :: w_vacuum = 0d0
       real(kind=8) :: tolerance = 1d-8
@@ -462,6 +590,36 @@ module rhyme_riemann_problem
       character(len=1024) :: path = ""
       type(irs_t) :: irs = irs_t()
    end type riemann_problem_t
+
+   type rp_shock_t
+      real(kind=8) :: rho = 0.d0, speed = 0.d0
+   end type rp_shock_t
+
+   type rp_fan_t
+      real(kind=8) :: rho = 0.d0, cs = 0.d0, speedH = 0.d0, speedT = 0.d0
+   end type rp_fan_t
+
+   type rp_star_side_t
+      logical :: is_shock = .false.
+      type(rp_shock_t) :: shock
+      type(rp_fan_t) :: fan
+      real(kind=8) :: f, fprime
+   end type rp_star_side_t
+
+   type rp_star_t
+      real(kind=8) :: u = 0.d0, p = 0.d0
+      type(rp_star_side_t) :: left, right
+   end type rp_star_t
+
+   type rp_side_t
+      real(kind=8) :: rho = 0.d0, v(NDIM) = 0.d0, p = 0.d0
+      real(kind=8) :: cs = 0.d0
+   end type rp_side_t
+
+   type riemann_problem_solution_t
+      type(rp_star_t) :: star
+      type(rp_side_t) :: left, right
+   end type riemann_problem_solution_t
 
 contains
 
@@ -476,29 +634,6 @@ contains
       if (rc /= RHY_OK) call logger%err('rhy_init: '//rhy_error_text())
    end subroutine rhyme_riemann_problem_init
 
-   ! Godunov state of the interface problem.  The device samples x/t = 0,
-   ! which is what the MUSCL-Hancock loops request (dx = 0,
-   ! rhyme_muscl_hancock_solve_memory_intensive.f90:110-114); other dx are
-   ! reported through u = NaN.
-   subroutine rhyme_riemann_problem_solve(rp, l, r, dx, dt, axis, u)
-      type(riemann_problem_t), intent(in) :: rp
-      real(kind=8), dimension(cid%rho:cid%e_tot), intent(in) :: l, r
-      real(kind=8), intent(in) :: dx, dt
-      integer, intent(in) :: axis
-      real(kind=8), intent(inout) :: u(cid%rho:cid%e_tot)
-      type(rhy_params_t) :: p
-      integer(c_int32_t) :: ax(1)
-      integer(c_int) :: rc
-      if (dx /= 0d0 .or. .not. (dt > 0d0)) then
-         u = ieee_nan()
-         return
-      end if
-      call rhy_rp_params(rp, p)
-      ax(1) = axis
-      rc = rhy_riemann_batch(p, 1_c_int64_t, l, r, ax, u)
-      if (rc /= RHY_OK) u = ieee_nan()
-   end subroutine rhyme_riemann_problem_solve
-
    subroutine rhy_rp_params(rp, p)
       type(riemann_problem_t), intent(in) :: rp
       type(rhy_params_t), intent(inout) :: p
@@ -512,11 +647,231 @@ contains
       p%ncmp = NCMP
    end subroutine rhy_rp_params
 
-   function ieee_nan() result(x)
-      use, intrinsic :: ieee_arithmetic
-      real(kind=8) :: x
-      x = ieee_value(x, ieee_quiet_nan)
-   end function ieee_nan
+   ! rhyme_riemann_problem_solve.f90:3-104: the conserved state at
+   ! x/t = dx/dt (dx = 0: the Godunov state of the MUSCL-Hancock loops).  On
+   ! a device error u is left as it was and the error reported.
+   subroutine rhyme_riemann_problem_solve(rp, l, r, dx, dt, axis, u)
+      type(riemann_problem_t), intent(in) :: rp
+      real(kind=8), dimension(cid%rho:cid%e_tot), intent(in) :: l, r
+      real(kind=8), intent(in) :: dx, dt
+      integer, intent(in) :: axis
+      real(kind=8), intent(inout) :: u(cid%rho:cid%e_tot)
+      type(rhy_params_t) :: p
+      integer(c_int32_t) :: ax(1)
+      real(c_double) :: dx1(1), dt1(1), uo(cid%rho:cid%e_tot)
+      integer(c_int) :: rc
+      call rhy_rp_params(rp, p)
+      ax(1) = axis
+      dx1(1) = dx
+      dt1(1) = dt
+      rc = rhy_riemann_solve_batch(p, 1_c_int64_t, l, r, ax, dx1, dt1, uo)
+      if (rc /= RHY_OK) then
+         call rhy_host_error('rhyme_riemann_problem_solve: '//rhy_error_text())
+         return
+      end if
+      u = uo
+   end subroutine rhyme_riemann_problem_solve
+
+   subroutine rhy_solution_to_c(s, c)
+      type(riemann_problem_solution_t), intent(in) :: s
+      type(rhy_rp_solution_t), intent(out) :: c
+      c%l_rho = s%left%rho
+      c%l_v(1:NDIM) = s%left%v
+      c%l_p = s%left%p
+      c%l_cs = s%left%cs
+      c%r_rho = s%right%rho
+      c%r_v(1:NDIM) = s%right%v
+      c%r_p = s%right%p
+      c%r_cs = s%right%cs
+      c%star_p = s%star%p
+      c%star_u = s%star%u
+      c%l_is_shock = merge(1, 0, s%star%left%is_shock)
+      c%r_is_shock = merge(1, 0, s%star%right%is_shock)
+      c%l_shock_rho = s%star%left%shock%rho
+      c%l_shock_speed = s%star%left%shock%speed
+      c%l_fan_rho = s%star%left%fan%rho
+      c%l_fan_cs = s%star%left%fan%cs
+      c%l_fan_speedH = s%star%left%fan%speedH
+      c%l_fan_speedT = s%star%left%fan%speedT
+      c%r_shock_rho = s%star%right%shock%rho
+      c%r_shock_speed = s%star%right%shock%speed
+      c%r_fan_rho = s%star%right%fan%rho
+      c%r_fan_cs = s%star%right%fan%cs
+      c%r_fan_speedH = s%star%right%fan%speedH
+      c%r_fan_speedT = s%star%right%fan%speedT
+      c%l_f = s%star%left%f
+      c%r_f = s%star%right%f
+   end subroutine rhy_solution_to_c
+
+   ! the star members of a device record into sol (sides untouched)
+   subroutine rhy_star_from_c(c, s)
+      type(rhy_rp_solution_t), intent(in) :: c
+      type(riemann_problem_solution_t), intent(inout) :: s
+      s%star%p = c%star_p
+      s%star%u = c%star_u
+      s%star%left%is_shock = c%l_is_shock /= 0
+      s%star%right%is_shock = c%r_is_shock /= 0
+      s%star%left%shock%rho = c%l_shock_rho
+      s%star%left%shock%speed = c%l_shock_speed
+      s%star%left%fan%rho = c%l_fan_rho
+      s%star%left%fan%cs = c%l_fan_cs
+      s%star%left%fan%speedH = c%l_fan_speedH
+      s%star%left%fan%speedT = c%l_fan_speedT
+      s%star%right%shock%rho = c%r_shock_rho
+      s%star%right%shock%speed = c%r_shock_speed
+      s%star%right%fan%rho = c%r_fan_rho
+      s%star%right%fan%cs = c%r_fan_cs
+      s%star%right%fan%speedH = c%r_fan_speedH
+      s%star%right%fan%speedT = c%r_fan_speedT
+      s%star%left%f = c%l_f
+      s%star%right%f = c%r_f
+   end subroutine rhy_star_from_c
+
+   ! rhyme_riemann_problem_star.f90:3-66: sol's sides in, its star out.  On
+   ! a device error sol is left as it was and the error reported.
+   subroutine rhyme_riemann_problem_star(rp, sol, axis)
+      type(riemann_problem_t), intent(in) :: rp
+      type(riemann_problem_solution_t), intent(inout) :: sol
+      integer, intent(in) :: axis
+      type(rhy_params_t) :: p
+      type(rhy_rp_solution_t) :: c(1)
+      integer(c_int32_t) :: ax(1)
+      integer(c_int) :: rc
+      if (rp%solver /= rpid%exact_rs) return
+      call rhy_rp_params(rp, p)
+      call rhy_solution_to_c(sol, c(1))
+      ax(1) = axis
+      rc = rhy_riemann_star_batch(p, 1_c_int64_t, c, ax)
+      if (rc /= RHY_OK) then
+         call rhy_host_error('rhyme_riemann_problem_star: '//rhy_error_text())
+         return
+      end if
+      call rhy_star_from_c(c(1), sol)
+   end subroutine rhyme_riemann_problem_star
+
+   ! gamma and kB/amu for the procedures without a riemann_problem_t
+   subroutine rhy_gas_params(p)
+      type(rhy_params_t), intent(inout) :: p
+      p%ndim = NDIM
+      p%gamma = get_gamma()
+      p%kb_amu = get_kb_amu()
+      p%ncmp = NCMP
+   end subroutine rhy_gas_params
+
+   ! rhyme_riemann_problem_sampling.f90:3-48.  On a device error u is 0 and
+   ! the error reported.
+   subroutine rhyme_riemann_problem_sampling(solution, axis, dx, dt, u)
+      type(riemann_problem_solution_t), intent(in) :: solution
+      integer, intent(in) :: axis
+      real(kind=8), intent(in) :: dx, dt
+      real(kind=8), intent(out) :: u(cid%rho:cid%e_tot)
+      type(rhy_params_t) :: p
+      type(rhy_rp_solution_t) :: c(1)
+      integer(c_int32_t) :: ax(1)
+      real(c_double) :: dx1(1), dt1(1)
+      integer(c_int) :: rc
+      call rhy_gas_params(p)
+      call rhy_solution_to_c(solution, c(1))
+      ax(1) = axis
+      dx1(1) = dx
+      dt1(1) = dt
+      rc = rhy_riemann_sampling_batch(p, 1_c_int64_t, c, ax, dx1, dt1, u)
+      if (rc /= RHY_OK) then
+         u = 0d0
+         call rhy_host_error('rhyme_riemann_problem_sampling: '//rhy_error_text())
+      end if
+   end subroutine rhyme_riemann_problem_sampling
+
+   ! one region state of a solution (RHY_W_*), x/t = dxdt for the fans
+   function rhy_region(s, region, axis, dxdt, who) result(u)
+      type(riemann_problem_solution_t), intent(in) :: s
+      integer(c_int32_t), intent(in) :: region
+      integer, intent(in) :: axis
+      real(kind=8), intent(in) :: dxdt
+      character(len=*), intent(in) :: who
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      type(rhy_params_t) :: p
+      type(rhy_rp_solution_t) :: c(1)
+      integer(c_int32_t) :: ax(1), reg(1)
+      real(c_double) :: d(1)
+      integer(c_int) :: rc
+      call rhy_gas_params(p)
+      call rhy_solution_to_c(s, c(1))
+      ax(1) = axis
+      reg(1) = region
+      d(1) = dxdt
+      rc = rhy_riemann_region_batch(p, 1_c_int64_t, c, reg, ax, d, u)
+      if (rc /= RHY_OK) then
+         u = 0d0
+         call rhy_host_error(who//': '//rhy_error_text())
+      end if
+   end function rhy_region
+
+   function rhy_side_solution(side) result(s)
+      type(rp_side_t), intent(in) :: side
+      type(riemann_problem_solution_t) :: s
+      s%left = side
+   end function rhy_side_solution
+
+   ! rhyme_riemann_problem_w_k.f90:3-11
+   function riemann_problem_w_k(s) result(u)
+      type(rp_side_t), intent(in) :: s
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(rhy_side_solution(s), RHY_W_L, 1, 0d0, 'riemann_problem_w_k')
+   end function riemann_problem_w_k
+
+   ! rhyme_riemann_problem_rp_side_to_cons.f90:3-11
+   function riemann_problem_rp_side_to_cons(s) result(u)
+      type(rp_side_t), intent(in) :: s
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(rhy_side_solution(s), RHY_W_L, 1, 0d0, 'riemann_problem_rp_side_to_cons')
+   end function riemann_problem_rp_side_to_cons
+
+   ! rhyme_riemann_problem_w_kfan.f90:3-32
+   function riemann_problem_w_kfan(s, dxdt, axis, is_right) result(u)
+      type(rp_side_t), intent(in) :: s
+      real(kind=8), intent(in) :: dxdt
+      integer, intent(in) :: axis
+      logical, intent(in) :: is_right
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      type(riemann_problem_solution_t) :: sol
+      if (is_right) then
+         sol%right = s
+         u = rhy_region(sol, RHY_W_RFAN, axis, dxdt, 'riemann_problem_w_kfan')
+      else
+         sol%left = s
+         u = rhy_region(sol, RHY_W_LFAN, axis, dxdt, 'riemann_problem_w_kfan')
+      end if
+   end function riemann_problem_w_kfan
+
+   ! rhyme_riemann_problem_w_starL_sho.f90:3-15 and siblings
+   function riemann_problem_w_starl_sho(s, axis) result(u)
+      type(riemann_problem_solution_t), intent(in) :: s
+      integer, intent(in) :: axis
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(s, RHY_W_STARL_SHO, axis, 0d0, 'riemann_problem_w_starl_sho')
+   end function riemann_problem_w_starl_sho
+
+   function riemann_problem_w_starr_sho(s, axis) result(u)
+      type(riemann_problem_solution_t), intent(in) :: s
+      integer, intent(in) :: axis
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(s, RHY_W_STARR_SHO, axis, 0d0, 'riemann_problem_w_starr_sho')
+   end function riemann_problem_w_starr_sho
+
+   function riemann_problem_w_starl_fan(s, axis) result(u)
+      type(riemann_problem_solution_t), intent(in) :: s
+      integer, intent(in) :: axis
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(s, RHY_W_STARL_FAN, axis, 0d0, 'riemann_problem_w_starl_fan')
+   end function riemann_problem_w_starl_fan
+
+   function riemann_problem_w_starr_fan(s, axis) result(u)
+      type(riemann_problem_solution_t), intent(in) :: s
+      integer, intent(in) :: axis
+      real(kind=8) :: u(cid%rho:cid%e_tot)
+      u = rhy_region(s, RHY_W_STARR_FAN, axis, 0d0, 'riemann_problem_w_starr_fan')
+   end function riemann_problem_w_starr_fan
 end module rhyme_riemann_problem
 
 module rhyme_cfl
@@ -544,6 +899,8 @@ contains
       call logger%log('cfl initialised')
    end subroutine rhyme_cfl_init
 
+   ! rhyme_cfl_time_step.f90:3-60.  On an error dt is -1 and the error is
+   ! reported (rhy_host_error): a caller's step with it would be rejected.
    function rhyme_cfl_time_step(c, samr) result(dt)
       real(kind=8), intent(in) :: c
       type(samr_t), intent(inout) :: samr
@@ -555,7 +912,10 @@ contains
       integer(c_int) :: rc
       dt = -1d0
       h = rhy_box_handle(samr%levels(0)%boxes(1), msg)
-      if (.not. c_associated(h)) return
+      if (.not. c_associated(h)) then
+         call rhy_host_error('rhyme_cfl_time_step: '//msg)
+         return
+      end if
       p%ndim = NDIM
       p%gamma = get_gamma()
       p%kb_amu = get_kb_amu()
@@ -564,7 +924,11 @@ contains
       dx3 = 1d0
       dx3(1:NDIM) = samr%levels(0)%dx
       rc = rhy_cfl_time_step(h, p, dx3, dtc)
-      if (rc == RHY_OK) dt = dtc
+      if (rc /= RHY_OK) then
+         call rhy_host_error('rhyme_cfl_time_step: '//rhy_error_text())
+         return
+      end if
+      dt = dtc
    end function rhyme_cfl_time_step
 end module rhyme_cfl
 
@@ -688,13 +1052,17 @@ contains
       character(len=:), allocatable :: msg
       integer(c_int) :: rc
       h = rhy_box_handle(box, msg)
-      if (.not. c_associated(h)) return
+      if (.not. c_associated(h)) then
+         call rhy_host_error('rhyme_samr_bc_set_boundaries: '//msg)
+         return
+      end if
       call rhy_bc_params(bc, p)
       if (face < 0) then
          rc = rhy_bc_set_boundaries(h, p)
       else
          rc = rhy_bc_set_face(h, p, int(face, c_int32_t))
       end if
+      if (rc /= RHY_OK) call rhy_host_error('rhyme_samr_bc_set_boundaries: '//rhy_error_text())
    end subroutine set_faces
 
    subroutine rhyme_samr_bc_set_left_boundary(bc, box)
@@ -811,20 +1179,24 @@ contains
       call rhyme_mh_workspace_init(ws, samr, logger)
    end subroutine rhyme_muscl_hancock_init
 
-   subroutine solve_with(solver_type, box, dx, dt, rp, sl, ws, rc)
+   ! One device step of the box; ok is false and msg set on any failure (no
+   ! device copy, or the library rejected the call: the device state and
+   ! box%cells are then as they were).
+   subroutine solve_with(solver_type, box, dx, dt, rp, sl, ws, ok, msg)
       integer, intent(in) :: solver_type
       type(samr_box_t), intent(inout) :: box
       real(kind=8), intent(in) :: dx(NDIM), dt
       type(riemann_problem_t), intent(in) :: rp
       type(slope_limiter_t), intent(in) :: sl
       type(mh_workspace_t), intent(inout) :: ws
-      integer(c_int), intent(out) :: rc
+      logical, intent(out) :: ok
+      character(len=:), allocatable, intent(out) :: msg
       type(rhy_params_t) :: p
       type(c_ptr) :: h
-      character(len=:), allocatable :: msg
       real(c_double) :: dx3(3)
+      integer(c_int) :: rc
 
-      rc = -1
+      ok = .false.
       h = rhy_box_handle(box, msg)
       if (.not. c_associated(h)) return
       call rhy_rp_params(rp, p)
@@ -834,6 +1206,11 @@ contains
       dx3 = 1d0
       dx3(1:NDIM) = dx
       rc = rhy_mh_solve(h, p, dx3, dt)
+      if (rc /= RHY_OK) then
+         msg = rhy_error_text()
+         return
+      end if
+      ok = .true.
    end subroutine solve_with
 
    subroutine rhyme_muscl_hancock_solve_memory_intensive(box, dx, dt, rp, sl, ws)
@@ -842,8 +1219,10 @@ contains
       type(riemann_problem_t), intent(inout) :: rp
       type(slope_limiter_t), intent(in) :: sl
       type(mh_workspace_t), intent(inout) :: ws
-      integer(c_int) :: rc
-      call solve_with(mhid%memory_intensive, box, dx, dt, rp, sl, ws, rc)
+      logical :: ok
+      character(len=:), allocatable :: msg
+      call solve_with(mhid%memory_intensive, box, dx, dt, rp, sl, ws, ok, msg)
+      if (.not. ok) call rhy_host_error('rhyme_muscl_hancock_solve_memory_intensive: '//msg)
    end subroutine rhyme_muscl_hancock_solve_memory_intensive
 
    subroutine rhyme_muscl_hancock_solve_cpu_intensive(box, dx, dt, rp, sl, ws)
@@ -852,8 +1231,10 @@ contains
       type(riemann_problem_t), intent(inout) :: rp
       type(slope_limiter_t), intent(in) :: sl
       type(mh_workspace_t), intent(inout) :: ws
-      integer(c_int) :: rc
-      call solve_with(mhid%cpu_intensive, box, dx, dt, rp, sl, ws, rc)
+      logical :: ok
+      character(len=:), allocatable :: msg
+      call solve_with(mhid%cpu_intensive, box, dx, dt, rp, sl, ws, ok, msg)
+      if (.not. ok) call rhy_host_error('rhyme_muscl_hancock_solve_cpu_intensive: '//msg)
    end subroutine rhyme_muscl_hancock_solve_cpu_intensive
 
    ! rhyme_muscl_hancock_solve.f90:17-26
@@ -865,13 +1246,14 @@ contains
       type(slope_limiter_t), intent(in) :: sl
       type(mh_workspace_t), intent(inout) :: ws
       type(logger_t), intent(inout) :: logger
-      integer(c_int) :: rc
+      logical :: ok
+      character(len=:), allocatable :: msg
 
       call logger%begin_section('solver')
       select case (mh%solver_type)
       case (mhid%cpu_intensive, mhid%memory_intensive)
-         call solve_with(mh%solver_type, box, dx, dt, rp, sl, ws, rc)
-         if (rc /= RHY_OK) call logger%err('rhy_mh_solve: '//rhy_error_text())
+         call solve_with(mh%solver_type, box, dx, dt, rp, sl, ws, ok, msg)
+         if (.not. ok) call logger%err('rhyme_muscl_hancock_solve: '//msg)
       case DEFAULT
          call logger%err('Unknown solver type')
          return
@@ -894,5 +1276,10 @@ contains
       p%ncmp = NCMP
       ax(1) = axis
       rc = rhy_half_step_batch(p, 1_c_int64_t, u, delta, ax, dx, dt, l, r)
+      if (rc /= RHY_OK) then
+         l = u
+         r = u
+         call rhy_host_error('rhyme_muscl_hancock_half_step_extrapolation: '//rhy_error_text())
+      end if
    end subroutine rhyme_muscl_hancock_half_step_extrapolation
 end module rhyme_muscl_hancock

@@ -313,10 +313,119 @@ def riemann_batch(params, l, r, axis):
     return u
 
 
-def rhyme_riemann_problem_solve(params, l, r, axis):
-    """Godunov state at x/t = 0 (dx = 0, as the MUSCL-Hancock loops call it)."""
-    return riemann_batch(params, np.asarray(l)[None], np.asarray(r)[None],
-                         np.array([axis], dtype=np.int32))[0]
+def riemann_solve_batch(params, l, r, axis, dx, dt):
+    """rhyme_riemann_problem_solve on records: record q sampled at
+    x/t = dx[q] / dt[q] (scalars broadcast)."""
+    l = np.ascontiguousarray(l, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    axis = np.ascontiguousarray(axis, dtype=np.int32)
+    n = len(axis)
+    dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)))
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
+    u = np.zeros_like(l)
+    check(_lib.load().rhy_riemann_solve_batch(C.byref(params), n, dptr(l), dptr(r), iptr(axis),
+                                              dptr(dx), dptr(dt), dptr(u)),
+          "rhy_riemann_solve_batch")
+    return u
+
+
+def rhyme_riemann_problem_solve(params, l, r, dx, dt, axis):
+    """rhyme_riemann_problem_solve(rp, l, r, dx, dt, axis, u): the conserved
+    state at x/t = dx/dt (dx = 0: the Godunov state the MUSCL-Hancock loops
+    use)."""
+    return riemann_solve_batch(params, np.asarray(l)[None], np.asarray(r)[None],
+                               np.array([axis], dtype=np.int32), dx, dt)[0]
+
+
+def rp_solutions(sides):
+    """An array of rhy_rp_solution_t from a list of dicts with the sides'
+    primitive states: l_rho, l_v (sequence), l_p, l_cs and the same for r."""
+    arr = (_lib.RhyRpSolution * len(sides))()
+    for q, sd in enumerate(sides):
+        s = arr[q]
+        for side in ("l", "r"):
+            setattr(s, side + "_rho", float(sd[side + "_rho"]))
+            v = list(sd[side + "_v"])
+            for d in range(len(v)):
+                getattr(s, side + "_v")[d] = float(v[d])
+            setattr(s, side + "_p", float(sd[side + "_p"]))
+            setattr(s, side + "_cs", float(sd[side + "_cs"]))
+    return arr
+
+
+def riemann_star_batch(params, sols, axis):
+    """rhyme_riemann_problem_star on every record of `sols` (an array of
+    RhyRpSolution, updated in place)."""
+    axis = np.ascontiguousarray(np.broadcast_to(np.asarray(axis, dtype=np.int32), (len(sols),)))
+    check(_lib.load().rhy_riemann_star_batch(C.byref(params), len(sols), sols, iptr(axis)),
+          "rhy_riemann_star_batch")
+    return sols
+
+
+def riemann_sampling_batch(params, sols, axis, dx, dt):
+    """rhyme_riemann_problem_sampling of record q at x/t = dx[q] / dt[q]."""
+    n = len(sols)
+    nv = params.ndim + 2
+    axis = np.ascontiguousarray(np.broadcast_to(np.asarray(axis, dtype=np.int32), (n,)))
+    dx = np.ascontiguousarray(np.broadcast_to(np.asarray(dx, dtype=np.float64), (n,)))
+    dt = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, dtype=np.float64), (n,)))
+    u = np.zeros((n, nv))
+    check(_lib.load().rhy_riemann_sampling_batch(C.byref(params), n, sols, iptr(axis), dptr(dx),
+                                                 dptr(dt), dptr(u)),
+          "rhy_riemann_sampling_batch")
+    return u
+
+
+def riemann_region_batch(params, sols, region, axis, dxdt):
+    """The state of region id region[q] (_lib.W_*) of record q; the fans at
+    x/t = dxdt[q]."""
+    n = len(sols)
+    nv = params.ndim + 2
+    region = np.ascontiguousarray(np.broadcast_to(np.asarray(region, dtype=np.int32), (n,)))
+    axis = np.ascontiguousarray(np.broadcast_to(np.asarray(axis, dtype=np.int32), (n,)))
+    dxdt = np.ascontiguousarray(np.broadcast_to(np.asarray(dxdt, dtype=np.float64), (n,)))
+    u = np.zeros((n, nv))
+    check(_lib.load().rhy_riemann_region_batch(C.byref(params), n, sols, iptr(region),
+                                               iptr(axis), dptr(dxdt), dptr(u)),
+          "rhy_riemann_region_batch")
+    return u
+
+
+def irs_exec_batch(params, l, r):
+    """rhyme_irs_exec on records (rho, v(axis), p, cs): returns p*, l_f, r_f."""
+    l = np.ascontiguousarray(l, dtype=np.float64).reshape(-1, 4)
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 4)
+    n = l.shape[0]
+    ps, lf, rf = np.zeros(n), np.zeros(n), np.zeros(n)
+    check(_lib.load().rhy_irs_exec_batch(C.byref(params), n, dptr(l), dptr(r), dptr(ps),
+                                         dptr(lf), dptr(rf)), "rhy_irs_exec_batch")
+    return ps, lf, rf
+
+
+def irs_guess_batch(params, l, r):
+    """rhyme_irs_guess_p_star on records (rho, v(axis), p, cs): [n, 6]."""
+    l = np.ascontiguousarray(l, dtype=np.float64).reshape(-1, 4)
+    r = np.ascontiguousarray(r, dtype=np.float64).reshape(-1, 4)
+    g = np.zeros((l.shape[0], 6))
+    check(_lib.load().rhy_irs_guess_batch(C.byref(params), l.shape[0], dptr(l), dptr(r),
+                                          dptr(g)), "rhy_irs_guess_batch")
+    return g
+
+
+def pow_batch(policy, x, y):
+    """x**y on the device under a library policy (_lib.POW_PATH, POW_LIBM,
+    POW_MIXED)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(np.broadcast_to(np.asarray(y, dtype=np.float64), x.shape))
+    out = np.zeros_like(x)
+    check(_lib.load().rhy_pow_batch(policy, x.size, dptr(x), dptr(y), dptr(out)),
+          "rhy_pow_batch")
+    return out
+
+
+def build_options():
+    """The compile-time options of the loaded library (_lib.OPT_* bits)."""
+    return int(_lib.load().rhy_build_options())
 
 
 def limiter_batch(params, ul, u, ur):
