@@ -40,7 +40,7 @@ int g_device = -1;
 constexpr int kThreads = 256;
 static_assert(kThreads <= kWalkLanes, "the walks' LDS ring has one slot column per thread");
 // workgroups of the fixup kernel (grid-stride over the deferred faces)
-constexpr int kFixupBlocks = 64;
+constexpr int kFixupBlocks = 128;
 constexpr int kNumKernelIds = 6;  // flux/step, update, bc, cfl, dt, deferred-face fixup
 
 struct TimedLaunch {

@@ -451,7 +451,7 @@ __global__ __launch_bounds__(256) void amr_restrict_kernel(const AmrItem* __rest
 unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) boundaries
-constexpr int kFixupBlocks = 64;  // workgroups of the fixup kernel
+constexpr int kFixupBlocks = 256;  // workgroups of the fixup kernel (one per CU)
 
 struct Slot {
   bool active = false;
