@@ -161,6 +161,27 @@ int rhy_cfl_time_step(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
 int rhy_mh_solve(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
                  double dt);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rhy_mh_solve on a state the caller owns, with every
+ * scratch buffer (flux planes, deferred-face list, dt) carved from one static
+ * device array of RHY_NOALLOC_SCRATCH_BYTES that the library holds on each
+ * GPU for the life of the process.  cells is a device pointer to the box's
+ * state laid out as the library's device copy: ncmp planes of the ghosted
+ * box, doubles (RHY_FP64) or floats (RHY_MIXED); its ghost cells are read
+ * as they are and its interior is advanced by dt in place.  stream is a
+ * hipStream_t (NULL: the null stream); the call only enqueues work on it and
+ * returns, so the result is there when the stream gets past it.  A box whose
+ * scratch would exceed the array is refused with RHY_ERR_SHAPE (the array
+ * holds an FP64 3D box of up to 77^3 cells, a mixed one of up to 98^3, an
+ * FP64 2D box of up to 1299^2).  Every call on
+ * one GPU shares that GPU's array: calls that overlap on one GPU (other
+ * streams, other threads) race and must be serialised by the caller. */
+#define RHY_NOALLOC_SCRATCH_BYTES (64 << 20)
+int rhy_mh_solve_noalloc(const rhy_params_t *p, int32_t ndim, const int32_t dims[3],
+                         int32_t ncmp, int32_t precision, const double dx[3], double dt,
+                         void *cells, void *stream);
+
 /* The main-loop step of src/main.f90:109-136 repeated nsteps times, all on
  * the device: dt from the CFL reduction, ghost fill, MUSCL-Hancock.  dts
  * (host, may be NULL) receives the dt of every step.  Blocks until done. */

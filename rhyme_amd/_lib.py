@@ -41,8 +41,10 @@ EXPORTS = [
     "rhy_batch_snapshot_pack", "rhy_batch_snapshot_unpack",
     "rhy_build_options", "rhy_riemann_solve_batch", "rhy_riemann_star_batch",
     "rhy_riemann_sampling_batch", "rhy_riemann_region_batch", "rhy_irs_exec_batch",
-    "rhy_irs_guess_batch", "rhy_pow_batch",
+    "rhy_irs_guess_batch", "rhy_pow_batch", "rhy_mh_solve_noalloc",
 ]
+# size of the static device scratch of rhy_mh_solve_noalloc
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 # rhy_build_options bits
 OPT_POW_FAST, OPT_CONTACT_EXACT, OPT_LDS_WALK, OPT_FAST_RIEMANN = 1, 2, 4, 8
@@ -236,6 +238,8 @@ def open_library(path):
         "rhy_irs_exec_batch": (C.c_int, [pp, C.c_int64, dp, dp, dp, dp, dp]),
         "rhy_irs_guess_batch": (C.c_int, [pp, C.c_int64, dp, dp, dp]),
         "rhy_pow_batch": (C.c_int, [C.c_int32, C.c_int64, dp, dp, dp]),
+        "rhy_mh_solve_noalloc": (C.c_int, [pp, C.c_int32, ip, C.c_int32, C.c_int32, dp,
+                                           C.c_double, vp, vp]),
     })
     for name, (res, args) in sig.items():
         f = getattr(L, name)

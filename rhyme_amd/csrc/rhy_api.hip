@@ -37,6 +37,15 @@ std::once_flag g_init_once;
 int g_init_rc = RHY_ERR_NODEV;
 int g_device = -1;
 
+// Scratch of rhy_mh_solve_noalloc (flux planes, deferred-face list, dt):
+// one copy per device, held for the life of the process.  Its device
+// address on each device is taken once, at rhy_init, with that device
+// current; host code uses that address, never the symbol.
+constexpr size_t kNoAllocScratch = (size_t)RHY_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+void* g_scratch[kMaxDevices] = {};
+
 constexpr int kThreads = 256;
 static_assert(kThreads <= kWalkLanes, "the walks' LDS ring has one slot column per thread");
 // workgroups of the fixup kernel (grid-stride over the deferred faces)
@@ -652,6 +661,9 @@ int rhy_init(int32_t device) {
     }
     g_device = d;
     g_init_rc = RHY_OK;
+    void* ptr = nullptr;
+    if (d < kMaxDevices && hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess)
+      g_scratch[d] = ptr;
   });
   if (g_init_rc != RHY_OK && g_err.empty()) g_err = "device initialisation failed";
   return g_init_rc;
@@ -1034,6 +1046,9 @@ int mh_solve_enqueue(rhy_box* b, const rhy_params_t* p, const double* dx, double
   return rc;
 }
 
+// Round up to the 256-byte alignment of the scratch's parts.
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
 }  // namespace
 
 extern "C" {
@@ -1044,6 +1059,80 @@ int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double
   if ((rc = mh_solve_enqueue(b, p, dx, dt))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Scratch bytes rhy_mh_solve_noalloc needs for a box: its flux planes, its
+// deferred-face list, the list's counter and dt, each part 256-byte aligned.
+int noalloc_scratch_bytes(int32_t ndim, const int32_t dims[3], int32_t precision,
+                          int64_t* bytes) {
+  int rc;
+  if (!bytes) return fail(RHY_ERR_ARG, "bytes is NULL");
+  if ((rc = check_dims(ndim, dims))) return rc;
+  if (precision != RHY_FP64 && precision != RHY_MIXED)
+    return fail(RHY_ERR_ARG, "unsupported precision");
+  const Geom G = make_geom(ndim, dims);
+  const size_t esz = precision == RHY_MIXED ? sizeof(float) : sizeof(double);
+  const int64_t ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
+  int64_t faces = 0;
+  for (int a = 0; a < ndim; ++a) faces += ncell / G.n[a] * (G.n[a] + 1);
+  const int64_t cap = std::min<int64_t>(std::max<int64_t>(faces / 8, 4096), 1 << 22);
+  *bytes = (int64_t)(align256(esz * (size_t)G.plane * (ndim + 2) * ndim) +
+                     align256(sizeof(uint64_t) * (size_t)cap) + 2 * 256);
+  return RHY_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int rhy_mh_solve_noalloc(const rhy_params_t* p, int32_t ndim, const int32_t dims[3],
+                         int32_t ncmp, int32_t precision, const double dx[3], double dt,
+                         void* cells, void* stream) {
+  int rc;
+  if ((rc = rhy_init(0))) return rc;
+  if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
+  if (ncmp < ndim + 2 || ncmp > RHY_MAX_CMP) return fail(RHY_ERR_ARG, "ncmp out of range");
+  int64_t need = 0;
+  if ((rc = noalloc_scratch_bytes(ndim, dims, precision, &need))) return rc;
+  if (need > (int64_t)kNoAllocScratch)
+    return fail(RHY_ERR_SHAPE, "box needs " + std::to_string(need) +
+                                   " bytes of scratch; the no-allocation entry holds " +
+                                   std::to_string(kNoAllocScratch));
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices || !g_scratch[dev])
+    return fail(RHY_ERR_HIP, "no scratch address for the current device (rhy_init selects it)");
+  // a box view of the caller's state, its scratch carved from `scratch`:
+  // flux planes, then the deferred-face list, its counter and dt
+  unsigned char* base = static_cast<unsigned char*>(g_scratch[dev]);
+  rhy_box b;
+  b.ndim = ndim;
+  for (int d = 0; d < 3; ++d) b.dims[d] = dims[d];
+  b.ncmp = ncmp;
+  b.precision = precision;
+  b.geom = make_geom(ndim, dims);
+  b.stream = static_cast<hipStream_t>(stream);
+  b.esz = precision == RHY_MIXED ? sizeof(float) : sizeof(double);
+  b.cells = cells;
+  b.extra = static_cast<unsigned char*>(cells) + b.esz * (size_t)b.geom.plane * (ndim + 2);
+  const size_t flux_bytes = align256(b.esz * (size_t)b.geom.plane * (ndim + 2) * ndim);
+  b.flux = base;
+  const int64_t ncell = (int64_t)b.geom.n[0] * b.geom.n[1] * b.geom.n[2];
+  int64_t faces = 0;
+  for (int a = 0; a < ndim; ++a) faces += ncell / b.geom.n[a] * (b.geom.n[a] + 1);
+  b.defer_cap = (uint32_t)std::min<int64_t>(std::max<int64_t>(faces / 8, 4096), 1 << 22);
+  b.defer_items = reinterpret_cast<uint64_t*>(base + flux_bytes);
+  const size_t list_bytes = align256(sizeof(uint64_t) * (size_t)b.defer_cap);
+  b.defer_count = reinterpret_cast<uint32_t*>(base + flux_bytes + list_bytes);
+  b.dt = reinterpret_cast<double*>(base + flux_bytes + list_bytes + 256);
+  b.profiling = false;
+  rc = mh_solve_enqueue(&b, p, dx, dt);
+  b.cells = b.flux = nullptr;  // nothing here is the view's to free
+  return rc;
 }
 
 int rhy_run_async(rhy_box_t* b, const rhy_params_t* p, const double dx[3], int32_t nsteps) {
