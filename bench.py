@@ -17,9 +17,12 @@ two stream synchronisations; zone-updates/s = 256^3 * K / elapsed.
 Also reported:
   roofline      algorithmic HBM bytes of the step (2 * 5 * 8 = 80 B per
                 zone-update: one read and one write of the FP64 state) over
-                the measured time of one step's kernels, against 8 TB/s;
+                the measured time of one step's kernels (achieved) and over
+                the wall time of a step (achieved_wall), against 8 TB/s;
                 `traffic` is what the step's kernels actually move per step,
-                from calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes.
+                from calibrated rocprofv3 FETCH_SIZE/WRITE_SIZE passes
+                summarised by --pmc-summary, flagged stale when the library
+                sources changed since.
   cpu_baseline  the CPU oracle (a restatement of the reference Fortran path,
                 both loop structures) on the host cores, on a bounded sample
                 of the same workload.
@@ -35,20 +38,141 @@ sys.path.insert(0, ROOT)
 
 METRIC = "zone-updates/s at 3D 256^3 (FP64 vs mixed), % HBM roofline; serve p99 latency"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes that one step's kernels move per zone-update at 3D 256^3, from
-# the two rocprofv3 --pmc passes of this bench (FETCH_SIZE, then WRITE_SIZE)
-# summarised in profiles/blast256_fp64_mixed_kernels.md, each scaled by what
-# the counter reads for kernels of known traffic run in the same passes
-# (--copy-probe): FETCH_SIZE reads 0.500 KiB per KiB a 1 GiB scaled copy
-# reads, WRITE_SIZE 1.000 per KiB written.
-PMC_TRAFFIC_PROFILE = "profiles/blast256_fp64_mixed_kernels.md"
-PMC_TRAFFIC_B_PER_ZONE = {"fp64": 547.1, "mixed": 273.7}
-PMC_TRAFFIC_N = 256
-# The flux kernels are bound by VALU issue, not HBM: issue cycles (4 per
-# wave64 VALU instruction, 16 per transcendental) over the SIMD cycles of
-# their run at 2.4 GHz, mean of the x, y, z launches, from the SQ counter
-# passes in the same profile.
-PMC_VALU_ISSUE = {"fp64": 0.80, "mixed": 0.89}
+# Counter figures of the step (HBM bytes per zone-update from calibrated
+# FETCH_SIZE/WRITE_SIZE passes, VALU issue of the walk kernels from SQ
+# passes), written by `bench.py --pmc-summary` from rocprofv3 --pmc runs of
+# this bench and stamped with the hash of the sources they were taken with;
+# a line whose sources hash differently marks them stale.
+PMC_SUMMARY = "profiles/blast256_counters.json"
+# gfx950 issue model: a wave64 VALU instruction holds its SIMD 4 cycles, a
+# transcendental 16; 256 CUs x 4 SIMDs at the 2.4 GHz peak clock
+SIMDS, CLOCK_GHZ = 1024, 2.4
+
+
+def source_hash():
+    """sha256 over the library's sources and build recipe (what the counters
+    depend on): rhyme_amd/csrc/*.hip|hpp, include/rhyme_hip.h, rhyme_amd/build.py."""
+    import hashlib
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "rhyme_amd", "csrc")
+    files = sorted(os.path.join(csrc, f) for f in os.listdir(csrc) if f.endswith((".hip", ".hpp")))
+    files += [os.path.join(ROOT, "include", "rhyme_hip.h"),
+              os.path.join(ROOT, "rhyme_amd", "build.py")]
+    for f in files:
+        h.update(os.path.relpath(f, ROOT).encode())
+        with open(f, "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def load_pmc_summary():
+    path = os.path.join(ROOT, PMC_SUMMARY)
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        d = json.load(f)
+    d["stale"] = d.get("source_hash") != source_hash()
+    return d
+
+
+# Kernels of one step of each leg, for the counter summary (names as
+# rocprofv3 prints them; kernels shared by the legs of both precisions,
+# such as dt_kernel, enter at their mean over all dispatches).
+STEP_KERNELS = {
+    "fp64": r"^void rhy::(flux_kernel<3, 0, \d, double|acc_walk_kernel<3, \d, \d, double|"
+            r"acc_fixup_kernel<3, \d, double|flux_fixup_kernel<3, double|update_kernel<3, \d, \w+, "
+            r"double)|^rhy::(dt_kernel|step_reset_kernel)",
+    "mixed": r"^void rhy::(flux_kernel<3, 0, \d, float|acc_walk_kernel<3, \d, \d, float|"
+             r"acc_fixup_kernel<3, \d, float|flux_fixup_kernel<3, float|update_kernel<3, \d, \w+, "
+             r"float)|^rhy::(dt_kernel|step_reset_kernel)",
+    "batched_fp64": r"batch_(flux|fixup|update)_kernel<3, [^>]*double|batch_dt_kernel|"
+                    r"batch_step_counter_kernel|^rhy::set_u32_kernel",
+    "batched_mixed": r"batch_(flux|fixup|update)_kernel<3, [^>]*float|batch_dt_kernel|"
+                     r"batch_step_counter_kernel|^rhy::set_u32_kernel",
+}
+# the kernels that walk faces (VALU-issue bound) in each leg
+WALK_KERNELS = r"flux_kernel<|acc_walk_kernel<|batch_flux_kernel<"
+
+
+def pmc_summary(root):
+    """Summarise the rocprofv3 --pmc passes under `root` (valu/, misc/,
+    fetch/, write/, each holding one *_counter_collection.csv, and the bench
+    line each pass printed as <pass>.json) into the figures the bench line
+    reports.  FETCH_SIZE and WRITE_SIZE (KiB per dispatch) are scaled by
+    what they read for the --copy-probe kernels of known traffic (a 1 GiB
+    fill and a 1 GiB scaled copy)."""
+    import collections
+    import csv
+    import glob
+    import re
+
+    def load(name):
+        path = glob.glob(os.path.join(root, name, "*_counter_collection.csv"))[0]
+        per = collections.defaultdict(lambda: collections.defaultdict(float))
+        disp = collections.defaultdict(dict)
+        for r in csv.DictReader(open(path)):
+            k = r["Kernel_Name"]
+            per[k][r["Counter_Name"]] += float(r["Counter_Value"])
+            disp[k][r["Dispatch_Id"]] = int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+        return per, disp
+
+    passes = {n: load(n) for n in ("valu", "misc", "fetch", "write")}
+    with open(os.path.join(root, "fetch.json")) as f:
+        line = json.loads(f.read().strip().splitlines()[-1])
+
+    def only(per, pat):
+        ks = [k for k in per if re.search(pat, k)]
+        assert len(ks) == 1, (pat, ks)
+        return ks[0]
+    fetch, _ = passes["fetch"]
+    write, _ = passes["write"]
+    gib_kib = float(1 << 20)
+    f_cal = fetch[only(fetch, r"AUnaryFunctor<double")]["FETCH_SIZE"] / gib_kib
+    w_cal = write[only(write, r"FillFunctor<double")]["WRITE_SIZE"] / gib_kib
+
+    def kernel_row(k):
+        vp, vd = passes["valu"]
+        mp, md = passes["misc"]
+        n = len(vd[k])
+        waves = vp[k]["SQ_WAVES"] / n
+        per_wave = {c.replace("SQ_INSTS_", "").lower(): v / vp[k]["SQ_WAVES"]
+                    for c, v in list(vp[k].items()) + list(mp[k].items())
+                    if c.startswith("SQ_INSTS")}
+        dur_us = sum(vd[k].values()) / n / 1e3
+        trans = per_wave.get("valu_trans_f64", 0.0) + per_wave.get("valu_trans_f32", 0.0)
+        issue = waves * (4 * per_wave["valu"] + 12 * trans) / (dur_us * 1e-6 * CLOCK_GHZ * 1e9
+                                                               * SIMDS)
+        f64 = sum(per_wave.get("valu_%s_f64" % o, 0.0) for o in ("add", "mul", "fma", "trans"))
+        return dict(dispatches=n, avg_us=dur_us, waves=waves,
+                    read_bytes=fetch[k]["FETCH_SIZE"] / len(passes["fetch"][1][k]) / f_cal * 1024,
+                    written_bytes=write[k]["WRITE_SIZE"] / len(passes["write"][1][k]) / w_cal
+                    * 1024,
+                    valu_per_wave=per_wave["valu"], f64_arith_per_wave=f64,
+                    f64_arith_share=f64 / per_wave["valu"] if per_wave["valu"] else 0.0,
+                    int32_per_wave=per_wave.get("valu_int32"),
+                    int64_per_wave=per_wave.get("valu_int64"),
+                    cvt_per_wave=per_wave.get("valu_cvt"), trans_per_wave=trans,
+                    valu_issue=issue)
+
+    out = dict(source_hash=line.get("source_hash"),
+               calibration=dict(fetch_kib_per_kib_read=f_cal, write_kib_per_kib_written=w_cal))
+    zones = {"fp64": line["config"]["grid"][0] ** 3, "mixed": line["config"]["grid"][0] ** 3}
+    if "batched" in line:
+        zones["batched_fp64"] = line["batched"]["fp64"]["zones_per_step"]
+        zones["batched_mixed"] = line["batched"]["mixed"]["zones_per_step"]
+    for leg, z in zones.items():
+        ks = [k for k in passes["valu"][0] if re.search(STEP_KERNELS[leg], k)]
+        rows = {k: kernel_row(k) for k in ks}
+        step_bytes = sum(r["read_bytes"] + r["written_bytes"] for r in rows.values())
+        walks = [r for k, r in rows.items() if re.search(WALK_KERNELS, k)]
+        wt = sum(r["avg_us"] for r in walks)
+        out[leg] = dict(zones=z, bytes_per_step=step_bytes, bytes_per_zone=step_bytes / z,
+                        step_kernel_us=sum(r["avg_us"] for r in rows.values()),
+                        walk_valu_issue=sum(r["valu_issue"] * r["avg_us"] for r in walks) / wt,
+                        walk_f64_arith_share=sum(r["f64_arith_share"] * r["avg_us"]
+                                                 for r in walks) / wt,
+                        kernels=rows)
+    return out
 
 
 def blast_params(hydro, solver):
@@ -312,7 +436,19 @@ def main():
                     help="first run two torch kernels of known traffic, a 1 GiB "
                          "fill (writes 1 GiB) and a 1 GiB scaled copy (reads 1 GiB, "
                          "writes 1 GiB), that scale rocprof FETCH/WRITE_SIZE")
+    ap.add_argument("--pmc-summary", metavar="DIR",
+                    help="no run: summarise the rocprofv3 --pmc passes of this bench under DIR "
+                         "(valu/, misc/, fetch/, write/ and their printed lines) into "
+                         + PMC_SUMMARY)
     a = ap.parse_args()
+    if a.pmc_summary:
+        d = pmc_summary(a.pmc_summary)
+        with open(os.path.join(ROOT, PMC_SUMMARY), "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+            f.write("\n")
+        print(json.dumps({leg: {k: v for k, v in d[leg].items() if k != "kernels"}
+                          for leg in d if isinstance(d[leg], dict) and "zones" in d[leg]}))
+        return 0
     if a.gpus != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
               "(--gpus %d requested)" % a.gpus, file=sys.stderr)
@@ -337,17 +473,22 @@ def main():
     flux_ms = st[0][1] / max(st[0][0], 1)
     bytes_alg = 80.0 * zones
     achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
-    traffic = (PMC_TRAFFIC_B_PER_ZONE["fp64"] * zones if a.n == PMC_TRAFFIC_N else None)
+    pmc = load_pmc_summary()
+    traffic = pmc["fp64"]["bytes_per_zone"] * zones if pmc and a.n ** 3 == pmc["fp64"]["zones"] \
+        else None
+    achieved_wall = bytes_alg / (ms_step * 1e-3) / 1e9
     roofline = dict(bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=achieved / HBM_PEAK_GBS, traffic=traffic,
                     traffic_unit="HBM bytes per step (FETCH_SIZE, WRITE_SIZE, calibrated)",
-                    traffic_source=PMC_TRAFFIC_PROFILE if traffic else None,
+                    counters_source=PMC_SUMMARY if pmc else None,
+                    counters_stale=pmc["stale"] if pmc else None,
+                    achieved_wall=achieved_wall, frac_wall=achieved_wall / HBM_PEAK_GBS,
                     flux_kernel_bound="valu issue",
-                    flux_kernel_valu_issue_frac=PMC_VALU_ISSUE["fp64"],
-                    kernel="one step: dt_kernel + deferred-list reset + 3 x flux_kernel + "
-                           "flux_fixup_kernel + update_kernel",
-                    step_kernel_ms=kern_ms, flux_kernel_avg_ms=flux_ms,
-                    update_kernel_avg_ms=st[1][1] / max(st[1][0], 1),
+                    flux_kernel_valu_issue_frac=pmc["fp64"]["walk_valu_issue"] if pmc else None,
+                    kernel="one step: dt_kernel + deferred-list reset + x flux walk + its fixup "
+                           "+ y and z accumulating walks + their fixups (achieved: over the "
+                           "HIP-event sum of these kernels; achieved_wall: over ms_per_step)",
+                    step_kernel_ms=kern_ms, walk_kernel_avg_ms=flux_ms,
                     bytes_per_zone_update=80)
     line = dict(metric=METRIC, value=zu, unit="zone-updates/s", n_gpus=1, steps=a.steps,
                 warmup=a.warmup, ms_per_step=ms_step, higher_is_better=True, scaling="weak",
@@ -357,7 +498,7 @@ def main():
                             "exact RS tol 1e-6 / 100 it" % a.n,
                             grid=[a.n] * 3, precision="fp64", state_finite=g["finite"],
                             dt_first=float(g["dts"][0]), dt_last=float(g["dts"][-1])),
-                roofline=roofline)
+                roofline=roofline, source_hash=source_hash())
     if not a.skip_mixed:
         import numpy as np
         m = gpu_leg(a.n, a.steps, a.warmup, a.seed, precision=1)
@@ -371,14 +512,12 @@ def main():
             rel_l1_vs_fp64=dict(zip(["rho", "rho_u", "rho_v", "rho_w", "e_tot"], rel)),
             rel_l1_note="against the FP64 device path after the same warmup+steps; the FP64 "
                         "path matches the FP64 oracle to <= 1e-12 (tests/test_gpu_parity.py)",
-            flux_kernel_avg_ms=m["stats"][0][1] / max(m["stats"][0][0], 1),
-            update_kernel_avg_ms=m["stats"][1][1] / max(m["stats"][1][0], 1),
+            walk_kernel_avg_ms=m["stats"][0][1] / max(m["stats"][0][0], 1),
             roofline_achieved_gbs=40.0 * zones / (sum(m["stats"][k][1] for k in (0, 1, 4, 5))
                                                    / max(a.steps, 1) * 1e-3) / 1e9,
             bytes_per_zone_update=40, state_finite=m["finite"],
-            traffic=(PMC_TRAFFIC_B_PER_ZONE["mixed"] * zones if a.n == PMC_TRAFFIC_N
-                     else None),
-            flux_kernel_valu_issue_frac=PMC_VALU_ISSUE["mixed"])
+            traffic=pmc["mixed"]["bytes_per_zone"] * zones if traffic else None,
+            flux_kernel_valu_issue_frac=pmc["mixed"]["walk_valu_issue"] if pmc else None)
     if not a.skip_batched:
         line["batched"] = batched_leg(a.batched_steps, a.warmup, a.seed)
     if not a.skip_amr:

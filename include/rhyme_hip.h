@@ -269,6 +269,52 @@ typedef struct rhy_ic {
 /* paint the box's device state (the host mirror is not touched) */
 int rhy_box_paint(rhy_box_t *box, const rhy_ic_t *ic);
 
+/* ---- device drawing: rhyme_drawing_init (src/rhyme_drawing/src/
+ * rhyme_drawing_init.f90:19-78) for a uniform canvas, sharp shapes and
+ * harmonic perturbations, in the reference's order ---- */
+#define RHY_DRAW_SPHERE 1
+#define RHY_DRAW_CUBOID 2
+#define RHY_DRAW_MAX_SHAPES 256
+#define RHY_DRAW_MAX_HARMONICS 8
+/* A shape painted over the canvas (later shapes over earlier ones) with
+ * the conserved form of its colour.  Sphere: the cells whose centre
+ * (q_d - 1/2 for 1-based index q_d, in cell units) lies within `radius`
+ * of `centre`, sum over axes of (x_d - centre_d)^2 <= radius^2 in axis
+ * order.  Cuboid: the cells with lo[d] <= q_d <= hi[d] on every axis
+ * (1-based, inclusive: the lb..ub range of rhyme_drawing_sharp_cuboid.f90
+ * on a base-level box). */
+typedef struct rhy_shape {
+    int32_t kind;
+    int32_t lo[3], hi[3];
+    int32_t pad;
+    double centre[3];
+    double radius;
+    double rho, v[3], p;
+} rhy_shape_t;
+/* A harmonic perturbation (rhyme_drawing_apply_perturbations.f90:51-185):
+ * profile[q-1] = A cos(2 pi / lambda * (q - 1/2)), evaluated on the host
+ * for the dims[axis] cells along the 0-based axis; every cell then gets
+ * conv_prim_to_cons(h) added to its hydro components, with h the sum over
+ * the drawing's perturbations (from 0) of profile * (rho, v, p). */
+typedef struct rhy_harmonic {
+    int32_t axis;
+    int32_t pad;
+    const double *profile;
+    double rho, v[3], p;
+} rhy_harmonic_t;
+/* The drawing: canvas colour (rhyme_drawing_uniform_canvas.f90), shapes,
+ * perturbations; host arrays.  Ghost cells and non-hydro components are
+ * written 0. */
+typedef struct rhy_drawing {
+    double rho, v[3], p;
+    int32_t nshapes, nharmonics;
+    const rhy_shape_t *shapes;
+    const rhy_harmonic_t *harmonics;
+} rhy_drawing_t;
+/* paint the box's device state from the drawing (gamma from p; the host
+ * mirror is not touched).  Blocks until done. */
+int rhy_box_draw(rhy_box_t *box, const rhy_params_t *p, const rhy_drawing_t *d);
+
 /* ---- unit-level batches (host arrays, NVAR = ndim+2 values per record,
  * record-major) so the reference's known-answer tests can run on the GPU ---- */
 
@@ -412,12 +458,29 @@ int rhy_batch_last_deferred(rhy_batch_t *batch, int64_t *n);
  *   that lie in another fine box's interior from that box.
  * rhy_amr_restrict: every coarse cell under a fine box takes the mean of
  *   its 2^ndim children (summed with i fastest, times 2^-ndim).
- * The prolong and restrict calls wait for both sides' streams. */
+ * rhy_amr_flux_register: after a fine substep, for every coarse face on the
+ *   boundary of every fine box, the sum S of the fluxes of its 2^(ndim-1)
+ *   fine faces (the last step's flux planes), pairwise: (F_00 + F_10) +
+ *   (F_01 + F_11), first index on the lower transverse axis, is stored in
+ *   the batch's flux register (reset = 1: after the first substep of a
+ *   coarse step) or added to it (reset = 0).
+ * rhy_amr_reflux: after the coarse step (rhy_mh_solve, whose flux planes
+ *   hold the coarse fluxes) and both substeps, every coarse cell next to a
+ *   fine box takes the correction of its face flux, F_c replaced by the
+ *   fine fluxes' mean over the two substeps and the 2^(ndim-1) fine faces:
+ *   U += dt/dx_a (F_c - <F_f>) on the low side of a box, dt/dx_a (<F_f> -
+ *   F_c) on the high side, applied axis by axis, low side then high side.
+ *   Cells under another fine box get it too and are then restricted over.
+ * The prolong, restrict, register and reflux calls wait for both sides'
+ * streams. */
 int rhy_batch_cfl_dts(rhy_batch_t *batch, const rhy_params_t *p, double *dts);
 int rhy_batch_run_dt_async(rhy_batch_t *batch, const rhy_params_t *p, double dt,
                            int32_t nsteps);
 int rhy_amr_prolong(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, int32_t half);
 int rhy_amr_restrict(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo);
+int rhy_amr_flux_register(rhy_batch_t *fine, const int32_t *lo, int32_t reset);
+int rhy_amr_reflux(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, double dt,
+                   const double dx[3]);
 
 /* Snapshot payload of a whole level (a batch or pool): the active boxes in
  * slot order, each laid out as rhy_box_snapshot_pack lays out one box; the

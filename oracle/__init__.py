@@ -109,6 +109,8 @@ def lib():
         L.orc_bc_init_ghosts.argtypes = [pp, ip, dp]
         L.orc_mh_solve.restype = C.c_int
         L.orc_mh_solve.argtypes = [pp, ip, dp, dp, C.c_double]
+        L.orc_mh_solve_fluxes.restype = C.c_int
+        L.orc_mh_solve_fluxes.argtypes = [pp, ip, dp, dp, C.c_double, dp]
         L.orc_run.restype = C.c_int
         L.orc_run.argtypes = [pp, ip, dp, dp, C.c_int32, dp]
         L.orc_num_threads.restype = C.c_int
@@ -330,6 +332,21 @@ def mh_solve(p, dims, cells, dx, dt):
     rc = lib().orc_mh_solve(C.byref(p), _ip(_dims(dims)), _dp(cells), _dp(dx), dt)
     if rc:
         raise RuntimeError("orc_mh_solve failed: %d" % rc)
+
+
+def mh_solve_fluxes(p, dims, cells, dx, dt):
+    """mh_solve that also returns the step's intercell fluxes, an array of
+    shape (ndim, nvar) + the ghosted box (the face between a cell and its
+    neighbour above along an axis stored at the cell)."""
+    _check_cells(p, dims, cells)
+    nd = p.ndim
+    dx = np.ascontiguousarray(np.resize(np.asarray(dx, dtype=np.float64), 3))
+    fr = np.zeros((nd, nd + 2) + cells.shape[1:])
+    rc = lib().orc_mh_solve_fluxes(C.byref(p), _ip(_dims(dims)), _dp(cells), _dp(dx), dt,
+                                   _dp(fr))
+    if rc:
+        raise RuntimeError("orc_mh_solve_fluxes failed: %d" % rc)
+    return fr
 
 
 def run(p, dims, cells, dx, nsteps):

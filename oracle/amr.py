@@ -18,9 +18,20 @@ reproduce this step bit for bit in FP64:
    (linear in time).  Then the face ghosts that lie in another fine box's
    interior take that box's cells instead (same level and time).  Fine boxes
    have no boundary fill of their own.
-4. Every coarse cell under a fine box takes the mean of its 2^ndim children
-   (summed with i fastest, then times 2^-ndim).  There is no refluxing, so
-   mass and energy are conserved only up to the coarse-fine flux mismatch.
+4. Refluxing (when on): for every coarse face on the boundary of a fine box,
+   the fluxes of its 2^(ndim-1) fine faces are summed over both substeps
+   (register: S_1 + S_2, S_k the pairwise sum (F_00 + F_10) + (F_01 + F_11)
+   of substep k's fine faces, first index on the lower transverse axis, so
+   that equal fluxes sum exactly), and the coarse cell outside the box takes
+   U += dt0/dx0_a (F_c - reg 2^-ndim) on the box's low side along a and
+   U += dt0/dx0_a (reg 2^-ndim - F_c) on its high side, F_c the coarse
+   step's flux on that face; passes in order a = 0.., low side then high
+   side, each over the boxes in order.  The fine fluxes through a box's
+   boundary then replace the coarse ones, and mass and energy are conserved
+   to rounding (the reference has no coarse-fine coupling, so this step has
+   no reference counterpart either).
+5. Every coarse cell under a fine box takes the mean of its 2^ndim children
+   (summed with i fastest, then times 2^-ndim).
 
 Placement: lo[d] is the coarse cell (1-based) that holds fine cells 1 and 2
 along d; fine extents are even; every fine box with its ghost layers lies in
@@ -28,7 +39,7 @@ the coarse interior (2 <= lo[d], lo[d] + n_f[d] / 2 <= n_c[d]).
 """
 import numpy as np
 
-from . import bc_set_boundaries, cfl_time_step, mh_solve
+from . import bc_set_boundaries, cfl_time_step, mh_solve, mh_solve_fluxes
 
 
 def _axis(ndim, d):
@@ -131,6 +142,67 @@ def restrict(ndim, fine, lo, coarse):
         coarse[c][csel] = s * w
 
 
+def _transverse(ndim, a):
+    return [d for d in range(ndim) if d != a]
+
+
+def register(ndim, fr, lo, reg, box, reset):
+    """Add one fine box's boundary fluxes fr (mh_solve_fluxes of its
+    substep) to the register entries reg[(box, a, side)] (see step 4)."""
+    nv = ndim + 2
+    n_f = [fr.shape[2 + ndim - 1 - d] - 4 for d in range(ndim)]
+    for a in range(ndim):
+        tr = _transverse(ndim, a)
+        for side in (0, 1):
+            fa = n_f[a] if side else 0
+
+            def face(c, x, y):
+                sel = [None] * ndim
+                sel[a] = slice(fa + 1, fa + 2)
+                for k, d in enumerate(tr):
+                    o = x if k == 0 else y
+                    sel[d] = slice(2 + o, 2 + o + n_f[d], 2)
+                return fr[a][c][tuple(sel[d] for d in reversed(range(ndim)))]
+            key = (box, a, side)
+            out = []
+            for c in range(nv):
+                if ndim == 1:
+                    s = face(c, 0, 0)
+                elif ndim == 2:
+                    s = face(c, 0, 0) + face(c, 1, 0)
+                else:
+                    s = (face(c, 0, 0) + face(c, 1, 0)) + (face(c, 0, 1) + face(c, 1, 1))
+                out.append(s.copy() if reset else reg[key][c] + s)
+            reg[key] = out
+
+
+def reflux(ndim, coarse, cflux, fine_dims, lo, reg, dt, dx):
+    """The coarse cells next to the fine boxes take the flux corrections
+    (see step 4)."""
+    nv = ndim + 2
+    w = {1: 0.5, 2: 0.25, 3: 0.125}[ndim]
+    for a in range(ndim):
+        dtdx = dt / dx[a]
+        for side in (0, 1):
+            for i, (n_f, l) in enumerate(zip(fine_dims, lo)):
+                out = l[a] + n_f[a] // 2 if side else l[a] - 1
+                low = out - 1 if side else out
+                so, sl = [None] * ndim, [None] * ndim
+                for d in range(ndim):
+                    if d == a:
+                        so[d] = slice(out + 1, out + 2)
+                        sl[d] = slice(low + 1, low + 2)
+                    else:
+                        so[d] = sl[d] = slice(l[d] + 1, l[d] + 1 + n_f[d] // 2)
+                so = tuple(so[d] for d in reversed(range(ndim)))
+                sl = tuple(sl[d] for d in reversed(range(ndim)))
+                for c in range(nv):
+                    favg = reg[(i, a, side)][c] * w
+                    fc = cflux[a][c][sl]
+                    dd = dtdx * (favg - fc) if side else dtdx * (fc - favg)
+                    coarse[c][so] = coarse[c][so] + dd
+
+
 def fine_params(p):
     """The fine level's parameters: the coarse ones with every face type 0
     (ghosts set by the hierarchy)."""
@@ -140,7 +212,7 @@ def fine_params(p):
     return q
 
 
-def step(pc, pf, coarse_dims, coarse, dx0, fines, lo):
+def step(pc, pf, coarse_dims, coarse, dx0, fines, lo, refluxing=True):
     """One coarse step of the hierarchy, in place; returns dt0.  pc / pf are
     oracle parameters of the coarse and fine levels, fines the fine boxes'
     host arrays, lo their placements."""
@@ -152,13 +224,21 @@ def step(pc, pf, coarse_dims, coarse, dx0, fines, lo):
         dt0 = min(dt0, 2.0 * cfl_time_step(pf, fd, f, dx1))
     old = coarse.copy()
     bc_set_boundaries(pc, coarse_dims, coarse)
-    mh_solve(pc, coarse_dims, coarse, dx0, dt0)
+    cflux = mh_solve_fluxes(pc, coarse_dims, coarse, dx0, dt0) if refluxing else \
+        mh_solve(pc, coarse_dims, coarse, dx0, dt0)
+    reg = {}
     for half in (0, 1):
         for f, l in zip(fines, lo):
             prolong(ndim, f, l, coarse, old, half)
         fill_siblings(ndim, fines, lo)
-        for fd, f in zip(fdims, fines):
-            mh_solve(pf, fd, f, dx1, 0.5 * dt0)
+        for i, (fd, f) in enumerate(zip(fdims, fines)):
+            if refluxing:
+                register(ndim, mh_solve_fluxes(pf, fd, f, dx1, 0.5 * dt0), lo[i], reg, i,
+                         half == 0)
+            else:
+                mh_solve(pf, fd, f, dx1, 0.5 * dt0)
+    if refluxing:
+        reflux(ndim, coarse, cflux, fdims, lo, reg, dt0, dx0)
     for f, l in zip(fines, lo):
         restrict(ndim, f, l, coarse)
     return dt0

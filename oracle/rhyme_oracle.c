@@ -842,7 +842,8 @@ void orc_bc_init_ghosts(const orc_params *p, const int32_t *dims,
  * full ghosted box (rhyme_mh_workspace_check.f90:36-75), stored here as
  * [dir][var][cell]. */
 static int mh_memory_intensive(const orc_params *p, const geom *G,
-                               double *cells, const double *dx, double dt)
+                               double *cells, const double *dx, double dt,
+                               double *fr_out)
 {
     const int nd = p->ndim, nv = nd + 2;
     const gconst k = make_gconst(p->gamma);
@@ -925,6 +926,8 @@ static int mh_memory_intensive(const orc_params *p, const geom *G,
                 }
             }
 #undef WS
+    if (fr_out)
+        memcpy(fr_out, fr, sz * sizeof(double));
     free(ul);
     free(ur);
     free(fr);
@@ -1004,8 +1007,32 @@ int orc_mh_solve(const orc_params *p, const int32_t *dims, double *cells,
     if (p->solver_type == 1)
         return mh_cpu_intensive(p, &G, cells, dx, dt);
     if (p->solver_type == 2)
-        return mh_memory_intensive(p, &G, cells, dx, dt);
+        return mh_memory_intensive(p, &G, cells, dx, dt, NULL);
     return -2; /* logger%err('Unknown solver type') */
+}
+
+/* orc_mh_solve that also returns the intercell fluxes of the step in
+ * fr_out[dir][var][cell] (ndim * nvar planes of the ghosted box, the face
+ * between cells o and o + s_dir at the low cell o): the workspace array
+ * ws%fr of the memory_intensive solve (…_memory_intensive.f90:110-116).
+ * Both solver types solve the same interfaces, so for cpu_intensive the
+ * fluxes come from a memory_intensive solve of a copy of the state. */
+int orc_mh_solve_fluxes(const orc_params *p, const int32_t *dims, double *cells,
+                        const double *dx, double dt, double *fr_out)
+{
+    geom G = make_geom(p->ndim, dims);
+    if (p->solver_type == 2)
+        return mh_memory_intensive(p, &G, cells, dx, dt, fr_out);
+    if (p->solver_type != 1)
+        return -2;
+    const size_t n = (size_t)G.plane * (size_t)(p->ndim + 2);
+    double *copy = (double *)malloc(n * sizeof(double));
+    if (!copy)
+        return -1;
+    memcpy(copy, cells, n * sizeof(double));
+    int rc = mh_memory_intensive(p, &G, copy, dx, dt, fr_out);
+    free(copy);
+    return rc ? rc : mh_cpu_intensive(p, &G, cells, dx, dt);
 }
 
 int orc_run(const orc_params *p, const int32_t *dims, double *cells,

@@ -118,6 +118,8 @@ void orc_bc_init_ghosts(const orc_params *p, const int32_t *dims,
 /* returns 0, or -1 if scratch allocation failed */
 int orc_mh_solve(const orc_params *p, const int32_t *dims, double *cells,
                  const double *dx, double dt);
+int orc_mh_solve_fluxes(const orc_params *p, const int32_t *dims, double *cells,
+                        const double *dx, double dt, double *fr_out);
 /* The main-loop step (src/main.f90:109-136): dt = CFL; BC; MUSCL-Hancock.
  * dts[s] receives the dt of step s. Returns 0 on success. */
 int orc_run(const orc_params *p, const int32_t *dims, double *cells,

@@ -41,7 +41,8 @@ EXPORTS = [
     "rhy_batch_snapshot_pack", "rhy_batch_snapshot_unpack",
     "rhy_build_options", "rhy_riemann_solve_batch", "rhy_riemann_star_batch",
     "rhy_riemann_sampling_batch", "rhy_riemann_region_batch", "rhy_irs_exec_batch",
-    "rhy_irs_guess_batch", "rhy_pow_batch", "rhy_mh_solve_noalloc",
+    "rhy_irs_guess_batch", "rhy_pow_batch", "rhy_mh_solve_noalloc", "rhy_box_draw",
+    "rhy_amr_flux_register", "rhy_amr_reflux",
 ]
 # size of the static device scratch of rhy_mh_solve_noalloc
 NOALLOC_SCRATCH_BYTES = 64 << 20
@@ -114,6 +115,50 @@ class RhyIC(C.Structure):
     ]
 
 
+RHY_DRAW_SPHERE, RHY_DRAW_CUBOID = 1, 2
+DRAW_MAX_SHAPES, DRAW_MAX_HARMONICS = 256, 8
+
+
+class RhyShape(C.Structure):
+    """rhy_shape_t: a sharp sphere or cuboid of a drawing."""
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("lo", C.c_int32 * 3),
+        ("hi", C.c_int32 * 3),
+        ("pad", C.c_int32),
+        ("centre", C.c_double * 3),
+        ("radius", C.c_double),
+        ("rho", C.c_double),
+        ("v", C.c_double * 3),
+        ("p", C.c_double),
+    ]
+
+
+class RhyHarmonic(C.Structure):
+    """rhy_harmonic_t: a harmonic perturbation and its host profile."""
+    _fields_ = [
+        ("axis", C.c_int32),
+        ("pad", C.c_int32),
+        ("profile", C.POINTER(C.c_double)),
+        ("rho", C.c_double),
+        ("v", C.c_double * 3),
+        ("p", C.c_double),
+    ]
+
+
+class RhyDrawing(C.Structure):
+    """rhy_drawing_t: canvas colour, shapes and perturbations."""
+    _fields_ = [
+        ("rho", C.c_double),
+        ("v", C.c_double * 3),
+        ("p", C.c_double),
+        ("nshapes", C.c_int32),
+        ("nharmonics", C.c_int32),
+        ("shapes", C.POINTER(RhyShape)),
+        ("harmonics", C.POINTER(RhyHarmonic)),
+    ]
+
+
 class RhyDiag(C.Structure):
     """rhy_diag_t: the diagnostics of one box."""
     _fields_ = [
@@ -145,6 +190,48 @@ def make_ic(ic):
     for k, v in ic.items():
         setattr(out, k, v)
     return out
+
+
+def make_drawing(d):
+    """RhyDrawing from the dict layout of ics.draw; returns it with the
+    arrays it points to (keep them alive while the call runs)."""
+    import numpy as np
+    shapes = d.get("shapes", [])
+    harms = d.get("harmonics", [])
+    sa = (RhyShape * max(1, len(shapes)))()
+    for i, sh in enumerate(shapes):
+        s = sa[i]
+        s.kind = sh["kind"]
+        for a in range(3):
+            s.lo[a] = int(sh.get("lo", [0, 0, 0])[a])
+            s.hi[a] = int(sh.get("hi", [0, 0, 0])[a])
+            s.centre[a] = float(sh.get("centre", [0.0, 0.0, 0.0])[a])
+            s.v[a] = float(sh["v"][a]) if a < len(sh["v"]) else 0.0
+        s.radius = float(sh.get("radius", 0.0))
+        s.rho = float(sh["rho"])
+        s.p = float(sh["p"])
+    ha = (RhyHarmonic * max(1, len(harms)))()
+    profiles = []
+    for i, h in enumerate(harms):
+        prof = np.ascontiguousarray(h["profile"], dtype=np.float64)
+        profiles.append(prof)
+        x = ha[i]
+        x.axis = int(h["axis"])
+        x.profile = prof.ctypes.data_as(C.POINTER(C.c_double))
+        x.rho = float(h["rho"])
+        for a in range(3):
+            x.v[a] = float(h["v"][a]) if a < len(h["v"]) else 0.0
+        x.p = float(h["p"])
+    out = RhyDrawing()
+    out.rho = float(d["rho"])
+    for a in range(3):
+        out.v[a] = float(d["v"][a]) if a < len(d["v"]) else 0.0
+    out.p = float(d["p"])
+    out.nshapes = len(shapes)
+    out.nharmonics = len(harms)
+    out.shapes = C.cast(sa, C.POINTER(RhyShape))
+    out.harmonics = C.cast(ha, C.POINTER(RhyHarmonic))
+    return out, (sa, ha, profiles)
 
 
 class RhyError(RuntimeError):
@@ -222,6 +309,7 @@ def open_library(path):
                                           C.c_double, dp, dp]),
         "rhy_flux_batch": (C.c_int, [pp, C.c_int64, dp, ip, dp]),
         "rhy_box_paint": (C.c_int, [vp, C.POINTER(RhyIC)]),
+        "rhy_box_draw": (C.c_int, [vp, C.POINTER(RhyParams), C.POINTER(RhyDrawing)]),
         "rhy_box_diagnostics": (C.c_int, [vp, C.POINTER(RhyDiag)]),
         "rhy_box_last_deferred": (C.c_int, [vp, C.POINTER(C.c_int64)]),
         "rhy_debug_defer": (C.c_int, [C.c_int32, C.c_int64]),

@@ -11,11 +11,16 @@ coarse step:
    set from their coarse parents, piecewise constant in space: U_old before
    the first, 0.5 * U_old + 0.5 * U_new before the second; ghosts inside
    another fine box's interior are copied from that box instead.
-4. Every coarse cell under a fine box takes the mean of its 2^ndim children.
+4. Refluxing (on by default): after each substep the fine fluxes through
+   every fine box's boundary are summed per coarse face into a flux
+   register (rhy_amr_flux_register); after the second, every coarse cell
+   next to a fine box has its coarse face flux replaced by the fine fluxes'
+   mean (rhy_amr_reflux), so mass and energy are conserved to rounding.
+5. Every coarse cell under a fine box takes the mean of its 2^ndim children.
 
-No refluxing: mass and energy are conserved up to the coarse-fine flux
-mismatch.  Every kernel is the library's (the box and batch steps, and
-rhy_amr_prolong / rhy_amr_restrict); the orchestration is on the host.  The
+Every kernel is the library's (the box and batch steps, rhy_amr_prolong,
+rhy_amr_flux_register, rhy_amr_reflux, rhy_amr_restrict); the orchestration
+is on the host.  The
 ghosts for the first substep are set before the coarse step, so no copy of
 U_old is kept (the second fill interpolates in place), and the first fine
 substep runs on the batch's stream while the coarse step runs on the box's.
@@ -55,8 +60,9 @@ class AmrHierarchy:
     holds fine cells 1 and 2 along each axis."""
 
     def __init__(self, ndim, base_dims, base_dx, fine_lo, fine_dims, precision=hydro.FP64,
-                 ncmp=None):
+                 ncmp=None, refluxing=True):
         self.ndim = ndim
+        self.refluxing = refluxing
         self.base_dims = list(base_dims)[:ndim]
         self.dx0 = [float(x) for x in list(base_dx)[:ndim]]
         self.dx1 = [x / 2.0 for x in self.dx0]
@@ -117,6 +123,17 @@ class AmrHierarchy:
                                      self._lo.ctypes.data_as(C.POINTER(C.c_int32))),
                "rhy_amr_restrict")
 
+    def _register(self, reset):
+        _check(_L().rhy_amr_flux_register(self.fine._h,
+                                          self._lo.ctypes.data_as(C.POINTER(C.c_int32)),
+                                          1 if reset else 0), "rhy_amr_flux_register")
+
+    def _reflux(self, dt0):
+        dx = (C.c_double * 3)(*(self.dx0 + [0.0] * (3 - self.ndim)))
+        _check(_L().rhy_amr_reflux(self.coarse.handle, self.fine._h,
+                                   self._lo.ctypes.data_as(C.POINTER(C.c_int32)), dt0, dx),
+               "rhy_amr_reflux")
+
     def step(self, params):
         """One coarse step; returns dt0."""
         fp = fine_params(params)
@@ -124,10 +141,15 @@ class AmrHierarchy:
         dt0 = min(dt0, 2.0 * float(self.fine.cfl_dts(fp).min()))
         self._prolong(0)                              # U_old: before the coarse step
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)      # overlaps the coarse step
+        if self.refluxing:
+            self._register(True)                      # the batch's stream, after substep 1
         hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
         hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
         self._prolong(1)                              # after substep 1 (stream order)
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)
+        if self.refluxing:
+            self._register(False)
+            self._reflux(dt0)                         # waits for the coarse box's stream
         self._restrict()                              # waits for the batch's stream
         return dt0
 

@@ -46,6 +46,9 @@ static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
 constexpr int kMaxDevices = 64;
 void* g_scratch[kMaxDevices] = {};
 
+// Round up to a 256-byte boundary (parts of one device buffer).
+size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
 constexpr int kThreads = 256;
 static_assert(kThreads <= kWalkLanes, "the walks' LDS ring has one slot column per thread");
 // workgroups of the fixup kernel (grid-stride over the deferred faces)
@@ -454,8 +457,11 @@ template <int ND, int LIM, typename R, class PW, class P>
 void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
                          int order) {
   static_assert(ND > 1, "1D boxes step through the flux planes");
+  TimedLaunch tr;
+  timed_begin(b, 5, tr);
   hipLaunchKernelGGL(step_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 4,
                      b->partial + b->acc_blocks);
+  timed_end(b, tr);
   const DeferList dlx = defer_list_axis(b, 0);
   launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx, dlx);
   if constexpr (PW::kDefer) {
@@ -985,6 +991,108 @@ int rhy_box_paint(rhy_box_t* b, const rhy_ic_t* ic) {
   return RHY_OK;
 }
 
+int rhy_box_draw(rhy_box_t* b, const rhy_params_t* p, const rhy_drawing_t* d) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (!p || !d) return fail(RHY_ERR_ARG, "params or drawing is NULL");
+  if (b->ncmp < b->ndim + 2) return fail(RHY_ERR_ARG, "box has too few components");
+  if (d->nshapes < 0 || d->nshapes > RHY_DRAW_MAX_SHAPES || (d->nshapes && !d->shapes))
+    return fail(RHY_ERR_ARG, "shape count out of range (0.." +
+                                 std::to_string(RHY_DRAW_MAX_SHAPES) + ") or shapes NULL");
+  if (d->nharmonics < 0 || d->nharmonics > RHY_DRAW_MAX_HARMONICS ||
+      (d->nharmonics && !d->harmonics))
+    return fail(RHY_ERR_ARG, "harmonic count out of range (0.." +
+                                 std::to_string(RHY_DRAW_MAX_HARMONICS) +
+                                 ") or harmonics NULL");
+  if (!(p->gamma > 1.0)) return fail(RHY_ERR_ARG, "gamma must exceed 1");
+  DrawCanvas cv{};
+  const double cw[5] = {d->rho, d->v[0], d->v[1], d->v[2], d->p};
+  // the colour of a box of fewer axes keeps its own velocity slots: (rho,
+  // v[0..ndim), p) packed into w[0..ndim+1], p last at w[4]
+  auto pack = [&](const double* src, double* w) {
+    for (int c = 0; c < 5; ++c) w[c] = 0.0;
+    w[0] = src[0];
+    for (int a = 0; a < b->ndim; ++a) w[1 + a] = src[1 + a];
+    w[4] = src[4];
+  };
+  pack(cw, cv.w);
+  cv.gm1 = p->gamma - 1.0;
+  cv.nshapes = d->nshapes;
+  cv.nharmonics = d->nharmonics;
+  std::vector<DrawShape> hs((size_t)d->nshapes);
+  for (int s = 0; s < d->nshapes; ++s) {
+    const rhy_shape_t& S = d->shapes[s];
+    if (S.kind != RHY_DRAW_SPHERE && S.kind != RHY_DRAW_CUBOID)
+      return fail(RHY_ERR_ARG, "unknown shape kind " + std::to_string(S.kind));
+    DrawShape& T = hs[s];
+    T.kind = S.kind;
+    for (int a = 0; a < 3; ++a) {
+      T.lo[a] = S.lo[a];
+      T.hi[a] = S.hi[a];
+      T.c[a] = S.centre[a];
+    }
+    T.r2 = S.radius * S.radius;
+    const double sw[5] = {S.rho, S.v[0], S.v[1], S.v[2], S.p};
+    pack(sw, T.w);
+  }
+  std::vector<DrawHarmonic> hh((size_t)d->nharmonics);
+  int64_t nprof = 0;
+  for (int m = 0; m < d->nharmonics; ++m) {
+    const rhy_harmonic_t& H = d->harmonics[m];
+    if (H.axis < 0 || H.axis >= b->ndim || !H.profile)
+      return fail(RHY_ERR_ARG, "harmonic axis out of range or profile NULL");
+    hh[m].axis = H.axis;
+    hh[m].off = nprof;
+    const double hw[5] = {H.rho, H.v[0], H.v[1], H.v[2], H.p};
+    pack(hw, hh[m].w);
+    nprof += b->dims[H.axis];
+  }
+  std::vector<double> prof((size_t)std::max<int64_t>(nprof, 1));
+  for (int m = 0; m < d->nharmonics; ++m)
+    std::memcpy(prof.data() + hh[m].off, d->harmonics[m].profile,
+                sizeof(double) * (size_t)b->dims[d->harmonics[m].axis]);
+  // the descriptors and profiles, in one short-lived device buffer
+  const size_t o1 = align256(sizeof(DrawShape) * std::max<size_t>(hs.size(), 1));
+  const size_t o2 = o1 + align256(sizeof(DrawHarmonic) * std::max<size_t>(hh.size(), 1));
+  const size_t bytes = o2 + sizeof(double) * prof.size();
+  unsigned char* dev = nullptr;
+  HIP_TRY(dev_alloc(&dev, bytes));
+  std::vector<unsigned char> host(bytes, 0);
+  std::memcpy(host.data(), hs.data(), sizeof(DrawShape) * hs.size());
+  std::memcpy(host.data() + o1, hh.data(), sizeof(DrawHarmonic) * hh.size());
+  std::memcpy(host.data() + o2, prof.data(), sizeof(double) * prof.size());
+  hipError_t e = hipMemcpyAsync(dev, host.data(), bytes, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) {
+    const Geom& G = b->geom;
+    const dim3 grid(grid_for(G.plane));
+    const bool mixed = b->precision == RHY_MIXED;
+    const DrawShape* ds = reinterpret_cast<const DrawShape*>(dev);
+    const DrawHarmonic* dh = reinterpret_cast<const DrawHarmonic*>(dev + o1);
+    const double* dp = reinterpret_cast<const double*>(dev + o2);
+#define RHY_DRAW(ND)                                                                           \
+  do {                                                                                         \
+    if (mixed)                                                                                 \
+      hipLaunchKernelGGL((draw_kernel<ND, float>), grid, dim3(kThreads), 0, b->stream, G, cv,  \
+                         ds, dh, dp, b->ncmp, static_cast<float*>(b->cells),                   \
+                         static_cast<float*>(b->extra));                                       \
+    else                                                                                       \
+      hipLaunchKernelGGL((draw_kernel<ND, double>), grid, dim3(kThreads), 0, b->stream, G, cv, \
+                         ds, dh, dp, b->ncmp, static_cast<double*>(b->cells),                  \
+                         static_cast<double*>(b->extra));                                      \
+  } while (0)
+    if (b->ndim == 1) RHY_DRAW(1);
+    else if (b->ndim == 2) RHY_DRAW(2);
+    else RHY_DRAW(3);
+#undef RHY_DRAW
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+  }
+  (void)hipFree(dev);
+  if (e != hipSuccess)
+    return fail(RHY_ERR_HIP, std::string("rhy_box_draw: ") + hipGetErrorString(e));
+  return RHY_OK;
+}
+
 int rhy_bc_set_face(rhy_box_t* b, const rhy_params_t* p, int32_t face) {
   int rc;
   if ((rc = check_box(b))) return rc;
@@ -1045,9 +1153,6 @@ int mh_solve_enqueue(rhy_box* b, const rhy_params_t* p, const double* dx, double
   }
   return rc;
 }
-
-// Round up to the 256-byte alignment of the scratch's parts.
-size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 
 }  // namespace
 
@@ -1529,6 +1634,7 @@ int box_view(rhy_box_t* b, BoxView* v) {
   if ((rc = check_box(b))) return rc;
   HIP_TRY(hipStreamSynchronize(b->stream));
   v->cells = b->cells;
+  v->flux = b->flux;
   v->geom = b->geom;
   v->ndim = b->ndim;
   v->ncmp = b->ncmp;

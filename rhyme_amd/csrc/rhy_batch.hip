@@ -70,8 +70,8 @@ struct BTile {
 // A segment is the run of tiles that covers one phase of one box: tiles
 // [start, next segment's start) take blockDim.x consecutive threads each of
 // the `threads` threads of box `box`'s phase `kind`.  Segments are stored in
-// tile order (all flux phases, then the updates), a few per box, so the
-// table changes size with the boxes and not with their zones.
+// tile order (all flux phases, then the updates), a few per box; a per-tile
+// table of segment indices (4 bytes per tile) finds a tile's segment.
 struct BSeg {
   int32_t start;
   int32_t box;
@@ -80,18 +80,10 @@ struct BSeg {
   int64_t threads;
 };
 
-// Tile ti by binary search over the segments (uniform in the workgroup).
-__device__ __forceinline__ BTile tile_of(const BSeg* __restrict__ segs, int32_t nseg,
-                                         int32_t ti) {
-  int lo = 0, hi = nseg;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (segs[mid].start <= ti)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const BSeg S = segs[lo];
+// Tile ti: its segment from the per-tile index (one load), then its share.
+__device__ __forceinline__ BTile tile_of(const BSeg* __restrict__ segs,
+                                         const int32_t* __restrict__ tile_seg, int32_t ti) {
+  const BSeg S = segs[tile_seg[ti]];
   BTile T;
   T.box = S.box;
   T.kind = S.kind;
@@ -105,13 +97,14 @@ __device__ __forceinline__ BTile tile_of(const BSeg* __restrict__ segs, int32_t 
 template <int ND, int LIM, typename R, class PW>
 __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void batch_flux_kernel(Consts k, const BBox* __restrict__ boxes,
                                                          const BSeg* __restrict__ segs,
-                                                         int32_t nseg, int32_t ntiles,
+                                                         const int32_t* __restrict__ tile_seg,
+                                                         int32_t ntiles,
                                                          R* __restrict__ arena,
                                                          const double* __restrict__ dt,
                                                          int32_t nf_x, int32_t nf_yz,
                                                          DeferList dl) {
   for (int32_t ti = blockIdx.x; ti < ntiles; ti += gridDim.x) {
-    const BTile T = tile_of(segs, nseg, ti);
+    const BTile T = tile_of(segs, tile_seg, ti);
     if ((int32_t)threadIdx.x >= T.count) continue;
     const BBox& B = boxes[T.box];
     Consts kb = k;
@@ -176,14 +169,15 @@ __global__ __launch_bounds__(256) void batch_fixup_kernel(Consts k, const BBox* 
 template <int ND, int ORDER, typename R>
 __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox* __restrict__ boxes,
                                                            const BSeg* __restrict__ segs,
-                                                           int32_t nseg, int32_t first,
+                                                           const int32_t* __restrict__ tile_seg,
+                                                           int32_t first,
                                                            int32_t ntiles,
                                                            R* __restrict__ arena,
                                                            const double* __restrict__ dt,
                                                            double* __restrict__ partial) {
   constexpr int NV = ND + 2;
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
-    const BTile T = tile_of(segs, nseg, ti);
+    const BTile T = tile_of(segs, tile_seg, ti);
     const BBox& B = boxes[T.box];
     const Geom& G = B.G;
     double speed = 0.0;
@@ -236,12 +230,13 @@ __global__ __launch_bounds__(256) void batch_update_kernel(Consts k, const BBox*
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void batch_cfl_kernel(Consts k, const BBox* __restrict__ boxes,
                                                         const BSeg* __restrict__ segs,
-                                                        int32_t nseg, int32_t first,
+                                                        const int32_t* __restrict__ tile_seg,
+                                                        int32_t first,
                                                         int32_t ntiles,
                                                         const R* __restrict__ arena,
                                                         double* __restrict__ partial) {
   for (int32_t ti = first + blockIdx.x; ti < first + ntiles; ti += gridDim.x) {
-    const BTile T = tile_of(segs, nseg, ti);
+    const BTile T = tile_of(segs, tile_seg, ti);
     const BBox& B = boxes[T.box];
     const Geom& G = B.G;
     double v = 0.0;
@@ -448,6 +443,115 @@ __global__ __launch_bounds__(256) void amr_restrict_kernel(const AmrItem* __rest
   }
 }
 
+// Coarse faces on the boundary of the fine boxes: an item is one (box,
+// axis a = slo[0], side s = slo[1]) with its Pi_{d != a} n_d / 2 coarse faces;
+// dst_off is its first register entry (ND + 2 doubles per face), src_off the
+// box's flux planes, lo its placement.  Thread t of the item: coarse face
+// (t_u, t_v) on the transverse axes u < v, u fastest.
+template <int ND>
+__device__ __forceinline__ void reflux_face(const AmrItem& it, int64_t t, int& a, int& s,
+                                            int tr[3]) {
+  a = it.slo[0];
+  s = it.slo[1];
+  int64_t q = t;
+  for (int d = 0; d < 3; ++d) {
+    tr[d] = 0;
+    if (d == a || d >= ND) continue;
+    const int m = it.GD.n[d] / 2;
+    tr[d] = (int)(q % m);
+    q /= m;
+  }
+}
+
+// Flux register of one substep: S = the sum of the coarse face's fine-face
+// fluxes, pairwise ((F_00 + F_10) + (F_01 + F_11), first index on the lower
+// transverse axis), so that equal fluxes sum exactly; reg = S (reset) or
+// reg + S.
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_register_kernel(const AmrItem* __restrict__ items,
+                                                           int nitems, int64_t total,
+                                                           const R* __restrict__ arena,
+                                                           double* __restrict__ reg, int reset) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
+  const int64_t t = t0 - it.first;
+  int a, s, tr[3];
+  reflux_face<ND>(it, t, a, s, tr);
+  const Geom& GF = it.GD;
+  const R* F = arena + it.src_off + (int64_t)a * (ND + 2) * GF.plane;
+  double* r = reg + it.dst_off + t * (ND + 2);
+  const int fa = s ? GF.n[a] : 0;  // the fine face's low cell along a
+  // offsets of the fine faces (x, y): x on the lower transverse axis
+  int64_t of[2][2];
+  for (int y = 0; y < 2; ++y)
+    for (int x = 0; x < 2; ++x) {
+      int idx[3] = {1, 1, 1};
+      int k = 0;
+      for (int d = 0; d < ND; ++d) {
+        if (d == a) {
+          idx[d] = fa;
+        } else {
+          idx[d] = 2 * tr[d] + 1 + (k == 0 ? x : y);
+          ++k;
+        }
+      }
+      of[y][x] = goff(GF, idx[0], idx[1], idx[2]);
+    }
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) {
+    const R* Fc = F + (int64_t)c * GF.plane;
+    double sum;
+    if (ND == 1)
+      sum = (double)Fc[of[0][0]];
+    else if (ND == 2)
+      sum = (double)Fc[of[0][0]] + (double)Fc[of[0][1]];
+    else
+      sum = ((double)Fc[of[0][0]] + (double)Fc[of[0][1]]) +
+            ((double)Fc[of[1][0]] + (double)Fc[of[1][1]]);
+    r[c] = reset ? sum : r[c] + sum;
+  }
+}
+
+// Reflux of one (axis, side) pass: the coarse cell outside each listed face
+// takes dtdx (F_c - <F_f>) (low side) or dtdx (<F_f> - F_c) (high side),
+// <F_f> = register * 2^-ndim (the mean over 2 substeps and 2^(ndim-1) faces).
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_reflux_kernel(const AmrItem* __restrict__ items,
+                                                         int nitems, int64_t total,
+                                                         const double* __restrict__ reg,
+                                                         Geom GC, R* __restrict__ coarse,
+                                                         const R* __restrict__ cflux,
+                                                         double dtdx) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
+  const int64_t t = t0 - it.first;
+  int a, s, tr[3];
+  reflux_face<ND>(it, t, a, s, tr);
+  int out[3] = {1, 1, 1}, low[3] = {1, 1, 1};
+  for (int d = 0; d < ND; ++d) {
+    if (d == a) {
+      out[d] = s ? it.lo[d] + it.GD.n[d] / 2 : it.lo[d] - 1;
+      low[d] = s ? out[d] - 1 : out[d];
+    } else {
+      out[d] = low[d] = it.lo[d] + tr[d];
+    }
+  }
+  const double w = ND == 3 ? 0.125 : ND == 2 ? 0.25 : 0.5;
+  const double* r = reg + it.dst_off + t * (ND + 2);
+  const int64_t oo = goff(GC, out[0], out[1], out[2]);
+  const int64_t of = goff(GC, low[0], low[1], low[2]) + (int64_t)a * (ND + 2) * GC.plane;
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) {
+    const double favg = r[c] * w;
+    const double fc = (double)cflux[of + c * GC.plane];
+    const double d = s ? dtdx * (favg - fc) : dtdx * (fc - favg);
+    R* u = coarse + oo + c * GC.plane;
+    *u = (R)((double)*u + d);
+  }
+}
+
 unsigned cdiv(int64_t a, int64_t b) { return (unsigned)((a + b - 1) / b); }
 
 constexpr int64_t kAlign = 64;  // arena blocks start on 64-element (>= 256 B) boundaries
@@ -482,6 +586,8 @@ struct rhy_batch {
   int32_t* dactive = nullptr;
   BSeg* dsegs = nullptr;
   size_t dsegs_cap = 0;
+  int32_t* dtile_seg = nullptr;  // segment index of every tile
+  size_t dtile_seg_cap = 0;
   double* partial = nullptr;
   size_t partial_cap = 0;
   double* dt = nullptr;
@@ -500,6 +606,9 @@ struct rhy_batch {
   int64_t snap_elems = 0;
   AmrItem* amr_items = nullptr;      // work items of the hierarchy launches (lazy, grows)
   int64_t amr_cap = 0;
+  double* amr_reg = nullptr;         // flux register of the coarse-fine faces (lazy)
+  int64_t amr_reg_len = 0;           // doubles in use
+  int64_t amr_reg_cap = 0;
   std::vector<AmrItem> amr_host;     // their host copy, kept until the stream is idle
 };
 
@@ -621,6 +730,15 @@ int build_tiles(rhy_batch* b) {
   }
   b->nupd_tiles = start - b->nflux_tiles;
   int rc;
+  std::vector<int32_t> tseg((size_t)std::max(1, start));
+  for (size_t i = 0; i < b->hsegs.size(); ++i) {
+    const int32_t end = i + 1 < b->hsegs.size() ? b->hsegs[i + 1].start : start;
+    for (int32_t t = b->hsegs[i].start; t < end; ++t) tseg[t] = (int32_t)i;
+  }
+  if ((rc = grow((void**)&b->dtile_seg, &b->dtile_seg_cap, tseg.size(), sizeof(int32_t))))
+    return rc;
+  BHIP_TRY(hipMemcpy(b->dtile_seg, tseg.data(), sizeof(int32_t) * tseg.size(),
+                     hipMemcpyHostToDevice));
   if ((rc = grow((void**)&b->dsegs, &b->dsegs_cap, b->hsegs.size(), sizeof(BSeg))) ||
       (rc = grow((void**)&b->partial, &b->partial_cap, (size_t)std::max(1, b->nupd_tiles),
                  sizeof(double))))
@@ -668,7 +786,7 @@ void launch_flux_phase(rhy_batch* b, const Consts& k, int lim) {
   hipLaunchKernelGGL(set_u32_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 0u);
 #define RHY_BATCH_FLUX(L)                                                                   \
   hipLaunchKernelGGL((batch_flux_kernel<ND, L, R, PW>), dim3(g), dim3(b->block), 0, b->stream, \
-                     k, b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles, a, b->dt, \
+                     k, b->dboxes, b->dsegs, b->dtile_seg, b->nflux_tiles, a, b->dt, \
                      b->nf_x, b->nf_yz, dl)
   switch (lim) {
     case 0: RHY_BATCH_FLUX(0); break;
@@ -690,11 +808,11 @@ void launch_update_phase(rhy_batch* b, const Consts& k, int order) {
   R* a = static_cast<R*>(b->arena);
   if (order == RHY_MEMORY_INTENSIVE)
     hipLaunchKernelGGL((batch_update_kernel<ND, 2, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                       b->dboxes, b->dsegs, b->dtile_seg, b->nflux_tiles,
                        b->nupd_tiles, a, b->dt, b->partial);
   else
     hipLaunchKernelGGL((batch_update_kernel<ND, 1, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                       b->dboxes, b->dsegs, b->dtile_seg, b->nflux_tiles,
                        b->nupd_tiles, a, b->dt, b->partial);
 }
 
@@ -740,7 +858,7 @@ int prologue(rhy_batch* b, const Consts& k, bool need_partials) {
   if (need_partials && (!b->pool || !b->partials_valid)) {
     const unsigned g = persistent_grid(b, b->nupd_tiles);
     hipLaunchKernelGGL((batch_cfl_kernel<ND, R>), dim3(g), dim3(b->block), 0, b->stream, k,
-                       b->dboxes, b->dsegs, (int32_t)b->hsegs.size(), b->nflux_tiles,
+                       b->dboxes, b->dsegs, b->dtile_seg, b->nflux_tiles,
                      b->nupd_tiles,
                        static_cast<const R*>(b->arena), b->partial);
   }
@@ -1094,9 +1212,10 @@ int rhy_batch_destroy(rhy_batch_t* b) {
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {(void*)b->arena, (void*)b->dboxes, (void*)b->dactive, (void*)b->dsegs,
-                  (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
+                  (void*)b->dtile_seg, (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow, (void*)b->diag, (void*)b->defer_count,
-                  (void*)b->defer_items, (void*)b->snap, (void*)b->amr_items})
+                  (void*)b->defer_items, (void*)b->snap, (void*)b->amr_items,
+                  (void*)b->amr_reg})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1497,6 +1616,134 @@ int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
     }
   }
 #undef RHY_RESTRICT
+  BHIP_TRY(hipGetLastError());
+  BHIP_TRY(hipStreamSynchronize(f->stream));
+  return RHY_OK;
+}
+
+// The register items of every fine box, grouped by (axis, side) pass in
+// pass order a = 0..ndim-1, low side then high side; pass_begin[p] and
+// pass_total[p] locate pass p's items and threads.  Returns the register
+// length in doubles.
+static int64_t reflux_items(rhy_batch_t* f, const int32_t* lo, int pass_begin[7],
+                            int64_t pass_total[6]) {
+  const int nd = f->ndim, nb = (int)f->slots.size();
+  f->amr_host.clear();
+  int64_t reg = 0;
+  for (int a = 0; a < nd; ++a)
+    for (int s = 0; s < 2; ++s) {
+      const int pass = 2 * a + s;
+      pass_begin[pass] = (int)f->amr_host.size();
+      int64_t first = 0;
+      for (int i = 0; i < nb; ++i) {
+        const BBox& B = f->slots[i].B;
+        AmrItem it{};
+        it.first = first;
+        it.dst_off = reg;
+        it.src_off = B.flux_off;
+        it.GD = B.G;
+        it.slo[0] = a;
+        it.slo[1] = s;
+        for (int d = 0; d < 3; ++d) it.lo[d] = lo[3 * i + d];
+        int64_t m = 1;
+        for (int d = 0; d < nd; ++d)
+          if (d != a) m *= B.G.n[d] / 2;
+        f->amr_host.push_back(it);
+        first += m;
+        reg += m * (nd + 2);
+      }
+      pass_total[pass] = first;
+    }
+  pass_begin[2 * nd] = (int)f->amr_host.size();
+  return reg;
+}
+
+int rhy_amr_flux_register(rhy_batch_t* f, const int32_t* lo, int32_t reset) {
+  if (!f || !lo) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (f->pool) return bfail(RHY_ERR_ARG, "the fine level is a fixed batch, not a pool");
+  for (int i = 0; i < (int)f->slots.size(); ++i)
+    for (int d = 0; d < f->ndim; ++d)
+      if (f->slots[i].B.G.n[d] % 2) return bfail(RHY_ERR_SHAPE, "fine extents must be even");
+  int pb[7];
+  int64_t pt[6];
+  const int64_t len = reflux_items(f, lo, pb, pt);
+  if (!reset && len != f->amr_reg_len)
+    return bfail(RHY_ERR_ARG, "flux register not started (reset = 1) for these boxes");
+  if (len > f->amr_reg_cap) {
+    BHIP_TRY(hipStreamSynchronize(f->stream));
+    if (f->amr_reg) (void)hipFree(f->amr_reg);
+    f->amr_reg = nullptr;
+    f->amr_reg_cap = 0;
+    if (dev_alloc(&f->amr_reg, sizeof(double) * (size_t)len) != hipSuccess) {
+      f->amr_reg = nullptr;
+      f->amr_reg_len = 0;
+      return bfail(RHY_ERR_OOM, "flux register allocation failed");
+    }
+    f->amr_reg_cap = len;
+  }
+  f->amr_reg_len = len;
+  int rc;
+  if ((rc = amr_upload(f))) return rc;
+  const int nd = f->ndim;
+  for (int pass = 0; pass < 2 * nd; ++pass) {
+    const int ni = pb[pass + 1] - pb[pass];
+    if (!pt[pass] || !ni) continue;
+    const AmrItem* items = f->amr_items + pb[pass];
+#define RHY_REG(ND, R)                                                                         \
+  hipLaunchKernelGGL((amr_register_kernel<ND, R>), dim3(cdiv(pt[pass], 256)), dim3(256), 0,   \
+                     f->stream, items, ni, pt[pass], static_cast<const R*>(f->arena),         \
+                     f->amr_reg, reset ? 1 : 0)
+    if (f->precision == RHY_MIXED) {
+      if (nd == 1) RHY_REG(1, float);
+      else if (nd == 2) RHY_REG(2, float);
+      else RHY_REG(3, float);
+    } else {
+      if (nd == 1) RHY_REG(1, double);
+      else if (nd == 2) RHY_REG(2, double);
+      else RHY_REG(3, double);
+    }
+#undef RHY_REG
+  }
+  BHIP_TRY(hipGetLastError());
+  BHIP_TRY(hipStreamSynchronize(f->stream));
+  return RHY_OK;
+}
+
+int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double dt,
+                   const double dx[3]) {
+  BoxView vc;
+  int rc;
+  if (!coarse || !dx) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vc)))
+    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if ((rc = amr_check(f, vc, lo))) return rc;
+  int pb[7];
+  int64_t pt[6];
+  const int64_t len = reflux_items(f, lo, pb, pt);
+  if (len != f->amr_reg_len || !f->amr_reg)
+    return bfail(RHY_ERR_ARG, "flux register not filled for these boxes");
+  if ((rc = amr_upload(f))) return rc;
+  const int nd = f->ndim;
+  for (int pass = 0; pass < 2 * nd; ++pass) {
+    const int ni = pb[pass + 1] - pb[pass];
+    if (!pt[pass] || !ni) continue;
+    const AmrItem* items = f->amr_items + pb[pass];
+    const double dtdx = dt / dx[pass / 2];
+#define RHY_REFLUX(ND, R)                                                                      \
+  hipLaunchKernelGGL((amr_reflux_kernel<ND, R>), dim3(cdiv(pt[pass], 256)), dim3(256), 0,     \
+                     f->stream, items, ni, pt[pass], f->amr_reg, vc.geom,                     \
+                     static_cast<R*>(vc.cells), static_cast<const R*>(vc.flux), dtdx)
+    if (f->precision == RHY_MIXED) {
+      if (nd == 1) RHY_REFLUX(1, float);
+      else if (nd == 2) RHY_REFLUX(2, float);
+      else RHY_REFLUX(3, float);
+    } else {
+      if (nd == 1) RHY_REFLUX(1, double);
+      else if (nd == 2) RHY_REFLUX(2, double);
+      else RHY_REFLUX(3, double);
+    }
+#undef RHY_REFLUX
+  }
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));
   return RHY_OK;

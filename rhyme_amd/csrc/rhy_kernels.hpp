@@ -117,6 +117,7 @@ int64_t defer_cap_setting();
 // the box's stream).
 struct BoxView {
   void* cells;
+  void* flux;  // flux planes of the last rhy_mh_solve
   Geom geom;
   int32_t ndim, ncmp, precision;
 };
@@ -212,13 +213,16 @@ __device__ __forceinline__ void write_ghost_images(const Geom& G, const BcTypes&
   }
 }
 
-// |v| + c_s of one cell: calc_u of rhyme_cfl_time_step.f90:52-59.
+// |v| + c_s of one cell: calc_u of rhyme_cfl_time_step.f90:52-59.  (A
+// wave of cells at rest with rho > 0 skips the square root and division:
+// sqrt(+0) / rho is +0.)
 template <int ND>
 __device__ __forceinline__ double cfl_speed(const Consts& k, const double* u) {
   double s = 0.0;
 #pragma unroll
   for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
-  return sqrt(s) / u[0] + speed_of_sound<ND>(k, u);
+  const double v = wave_all(s == 0.0 && u[0] > 0.0) ? 0.0 : sqrt(s) / u[0];
+  return v + speed_of_sound<ND>(k, u);
 }
 
 // Block-wide max of non-negative values; thread 0 of the block writes the
@@ -601,6 +605,47 @@ __device__ __forceinline__ void acc_cell(const Geom& G, const R* __restrict__ fx
                                     dtdx0, dtdxA, order);
 }
 
+// What the epilogue of one cell reads from HBM: F_x at the cell and below
+// it (kAccMid, and kAccLast of a 2D box), or Q at the cell (kAccLast, 3D).
+template <int ND, typename R, int MODE>
+struct AccLoads {
+  static constexpr bool kFx = MODE == kAccMid || ND == 2;
+  R a[ND + 2];  // F_x[c - 1] or Q[c]
+  R b[ND + 2];  // F_x[c]
+  __device__ __forceinline__ void load(const Geom& G, const R* __restrict__ fx,
+                                       const R* __restrict__ acc, int64_t oc) {
+#pragma unroll
+    for (int c = 0; c < ND + 2; ++c) {
+      const int64_t q = oc + c * G.plane;
+      if constexpr (kFx) {
+        a[c] = fx[q - G.s[0]];
+        b[c] = fx[q];
+      } else {
+        a[c] = acc[q];
+      }
+    }
+  }
+};
+
+// acc_one with the cell's loads already in registers (same arithmetic).
+template <int ND, typename R, int MODE>
+__device__ __forceinline__ R acc_one_loaded(R* __restrict__ out, int64_t q, R uc, R fl, R fu,
+                                            R xa, R xb, R dtdx0, R dtdxA, int order) {
+  const R b = dtdxA * (fl - fu);
+  R v;
+  if constexpr (MODE == kAccMid || ND == 2) {
+    const R a0 = dtdx0 * (xa - xb);
+    if constexpr (MODE == kAccMid)
+      v = order == RHY_MEMORY_INTENSIVE ? a0 + b : (uc + a0) + b;
+    else
+      v = order == RHY_MEMORY_INTENSIVE ? uc + (a0 + b) : (uc + a0) + b;
+  } else {
+    v = order == RHY_MEMORY_INTENSIVE ? uc + (xa + b) : xa + b;
+  }
+  out[q] = v;
+  return v;
+}
+
 template <int ND, int A, int LIM, typename R, class PW, int MODE>
 __device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
                                                 const R* __restrict__ cells,
@@ -659,6 +704,80 @@ __device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
   R nx[NV];
   load_state<ND, R>(cells, G.plane, o + 2 * s, nx);
   put(2, nx);
+  if constexpr (sizeof(R) == sizeof(float)) {
+    // The mixed form's walk is software-pipelined: the cell between faces
+    // m-2 and m-1 is finished at face m, between that face's half step and
+    // its Riemann solve.  Its loads (AccLoads) are issued at the top of the
+    // iteration and arrive during the half step; its upper face's flux waits
+    // in registers across the half step, its lower one in the lane's
+    // walk_fprev slot; nothing more than the plain walk holds is live across
+    // a Riemann solve.  (With FP64 values the held loads and flux spill: the
+    // FP64 walk finishes each cell right after its upper face, below.)
+    auto finish = [&](int j, const R* fu, const AccLoads<ND, R, MODE>& ld, bool deferred) {
+      const int64_t oc = o + (int64_t)j * s;
+      R un[NV];
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        const R fl = fprev[c * kWalkLanes + lane];
+        fprev[c * kWalkLanes + lane] = fu[c];
+        const R uc = ring[((j & 3) * NV + c) * kWalkLanes + lane];
+        un[c] = acc_one_loaded<ND, R, MODE>(out, oc + c * G.plane, uc, fl, fu[c], ld.a[c],
+                                            AccLoads<ND, R, MODE>::kFx ? ld.b[c] : R(0), dtdx0,
+                                            dtdxA, order);
+      }
+      if constexpr (MODE == kAccLast) {
+        int ci[3] = {idx[0], idx[1], idx[2]};
+        ci[A] = f0 + j;
+        write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
+        if (!deferred) {  // a deferred face: the fixup recomputes the speed
+          double ud[NV];
+#pragma unroll
+          for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
+          const double sp = cfl_speed<ND>(k, ud);
+          smax = sp > smax ? sp : smax;
+        }
+      }
+    };
+    R fh[NV];         // flux of face m - 1, held for the finish at face m
+    bool dh = false;  // face m - 1 deferred
+    bool dp = false;  // face m - 2 deferred (its flux in fprev)
+#pragma unroll 1
+    for (int m = 0; m < nf; ++m) {
+      AccLoads<ND, R, MODE> ld;
+      if (m >= 2) ld.load(G, fx, acc, o + (int64_t)(m - 1) * s);
+      if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
+      get(m, um);
+      get(m + 1, u0);
+      get(m + 2, up);
+      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+      if (m >= 2) {  // cell f0 + m - 1 (its ring slot is overwritten by the put)
+        finish(m - 1, fh, ld, dp || dh);
+        dp = dh;
+      }
+      if (m + 1 < nf) put(m + 3, nx);
+      R Fr[NV];
+      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
+      const bool dm = defer_take<PW>(k, dl, 0u, A, o + (int64_t)m * s);
+#pragma unroll
+      for (int c = 0; c < NV; ++c) {
+        rp[c] = rn[c];
+        if (m == 0)
+          fprev[c * kWalkLanes + lane] = Fr[c];
+        else
+          fh[c] = Fr[c];
+      }
+      if (m == 0)
+        dp = dm;
+      else
+        dh = dm;
+    }
+    {  // the last cell, f0 + nf - 1 (faces nf - 2 and nf - 1; nf >= 2)
+      AccLoads<ND, R, MODE> ld;
+      ld.load(G, fx, acc, o + (int64_t)(nf - 1) * s);
+      finish(nf - 1, fh, ld, dp || dh);
+    }
+    return smax;
+  }
   bool dprev = false;
 #pragma unroll 1
   for (int m = 0; m < nf; ++m) {
@@ -1390,6 +1509,103 @@ __global__ __launch_bounds__(256) void paint_kernel(Geom G, PaintIC ic, int32_t 
     }
     comp_plane<R>(cells, extra, ND + 2, c, G.plane)[t] = (R)v;
   }
+}
+
+// Drawing (rhyme_drawing_init.f90:19-78): canvas, sharp shapes in order,
+// then the harmonic perturbations added in conserved form.
+struct DrawShape {
+  int32_t kind;
+  int32_t lo[3], hi[3];
+  double c[3], r2;
+  double w[5];  // rho, v[3], p
+};
+struct DrawHarmonic {
+  int32_t axis;
+  int64_t off;  // first value of its profile in the profile array
+  double w[5];
+};
+struct DrawCanvas {
+  double w[5];
+  double gm1;
+  int32_t nshapes, nharmonics;
+};
+
+// conv_prim_vars_to_cons (rhyme_ideal_gas_primitive_vars_to_conserved.f90:9-11)
+// of w = (rho, v[ND], p) into u
+template <int ND>
+__device__ __forceinline__ void draw_cons(const double* w, double gm1, double* u) {
+  double s = 0.0;
+#pragma unroll
+  for (int d = 0; d < ND; ++d) s = s + w[1 + d] * w[1 + d];
+  u[0] = w[0];
+#pragma unroll
+  for (int d = 0; d < ND; ++d) u[1 + d] = w[0] * w[1 + d];
+  u[ND + 1] = .5 * w[0] * s + w[4] / gm1;
+}
+
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void draw_kernel(Geom G, DrawCanvas cv,
+                                                   const DrawShape* __restrict__ shapes,
+                                                   const DrawHarmonic* __restrict__ harms,
+                                                   const double* __restrict__ profiles,
+                                                   int32_t ncmp, R* __restrict__ cells,
+                                                   R* __restrict__ extra) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= G.plane) return;
+  int q[3];
+  bool inside = true;
+  int64_t rem = t;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int ext = G.n[d] + 2 * G.g[d];
+    q[d] = (int)(rem % ext) - G.g[d];
+    rem /= ext;
+    inside = inside && q[d] >= 0 && q[d] < G.n[d];
+  }
+  double u[ND + 2];
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) u[c] = 0.0;
+  if (inside) {
+    double w[5];
+#pragma unroll
+    for (int c = 0; c < 5; ++c) w[c] = cv.w[c];
+    for (int s = 0; s < cv.nshapes; ++s) {
+      const DrawShape& S = shapes[s];
+      bool in = true;
+      if (S.kind == 1) {
+        double r2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+          const double x = ((double)(q[d] + 1) - 0.5) - S.c[d];
+          r2 = d == 0 ? x * x : r2 + x * x;
+        }
+        in = r2 <= S.r2;
+      } else {
+#pragma unroll
+        for (int d = 0; d < ND; ++d) in = in && q[d] + 1 >= S.lo[d] && q[d] + 1 <= S.hi[d];
+      }
+      if (in) {
+#pragma unroll
+        for (int c = 0; c < 5; ++c) w[c] = S.w[c];
+      }
+    }
+    draw_cons<ND>(w, cv.gm1, u);
+    if (cv.nharmonics > 0) {
+      double h[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+      for (int m = 0; m < cv.nharmonics; ++m) {
+        const DrawHarmonic& H = harms[m];
+        const double a = profiles[H.off + q[H.axis]];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) h[c] = h[c] + a * H.w[c];
+      }
+      double up[ND + 2];
+      draw_cons<ND>(h, cv.gm1, up);
+#pragma unroll
+      for (int c = 0; c < ND + 2; ++c) u[c] = u[c] + up[c];
+    }
+  }
+  for (int c = 0; c < ncmp; ++c)
+    comp_plane<R>(cells, extra, ND + 2, c, G.plane)[t] = (R)(c < ND + 2 ? u[c] : 0.0);
 }
 
 // ------------------------------------------------------ unit batches ----

@@ -96,15 +96,25 @@ __device__ __forceinline__ double root<double>(double x) { return sqrt(x); }
 template <>
 __device__ __forceinline__ float root<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// True on every active lane of the wave when p holds on all of them (a
+// wave-uniform value: a branch on it skips its body for the whole wave).
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
+
 // ---------------------------------------------------------------- EOS ----
 
-// rhyme_hydro_base_specific_kinetic_energy.f90:9
+// rhyme_hydro_base_specific_kinetic_energy.f90:9.  For a state at rest the
+// numerator is +0 and, for rho*rho >= the smallest normal number of R (the
+// FP32 quotient multiplies by a reciprocal that must stay finite), the
+// quotient is +0: a wave whose lanes all hold such states skips the
+// division, and every other wave divides on every lane, as before.
 template <int ND, typename R = double>
 __device__ __forceinline__ R sp_kin_e(const R* u) {
   R s = R(0);
 #pragma unroll
   for (int d = 0; d < ND; ++d) s += u[1 + d] * u[1 + d];
-  return rdiv<R>(R(.5) * s, u[0] * u[0]);
+  const R num = R(.5) * s, den = u[0] * u[0];
+  if (wave_all(num == R(0) && den >= tiny_of<R>())) return R(0);
+  return rdiv<R>(num, den);
 }
 
 // rhyme_hydro_base_specific_internal_energy.f90:9
@@ -348,6 +358,10 @@ struct PowFast {
 
   __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
                                              double& xb) {
+    if (wave_all(x == 1.0)) {  // ln 1 = 0 and e**(+-0) = 1 exactly below
+      xa = xb = 1.0;
+      return;
+    }
     if (!(x > 0.0) || !(x < INFINITY)) {
       xa = pow(x, a);
       xb = pow(x, b);
@@ -434,6 +448,7 @@ struct Deferring : P {
   __device__ __forceinline__ static void pw2(double x, double a, double b, double& xa,
                                              double& xb) {
     xa = xb = 1.0;
+    if (wave_all(x == 1.0)) return;  // P's result at x = 1, as in P::pw2
     if (!(x > 0.0) || !(x < INFINITY)) {
       defer_mark();
       return;
@@ -736,11 +751,14 @@ __device__ __forceinline__ void riemann_sample(const Consts& k, const RI* l, con
   } else if (vac_b) {
     // vacuum
   } else {
-    double s_starl = lv[A] - PW::dv(2 * lc, k.gm1);
-    double s_starr = rv[A] + PW::dv(2 * rc, k.gm1);
-    double du = rv[A] - lv[A];
-    double du_crit = PW::dv(2 * (lc + rc), k.gm1);
-    if (du_crit < du) {  // generation of vacuum (:83-98)
+    const double du = rv[A] - lv[A];
+    // du_crit = 2 (c_L + c_R) / (gamma - 1) > 0 (c_K >= tiny), so
+    // du_crit < du needs du > 0: a wave with no such lane skips the division
+    bool vacuum_gen = false;
+    if (!wave_all(!(du > 0.0))) vacuum_gen = PW::dv(2 * (lc + rc), k.gm1) < du;
+    if (vacuum_gen) {  // generation of vacuum (:83-98)
+      const double s_starl = lv[A] - PW::dv(2 * lc, k.gm1);
+      const double s_starr = rv[A] + PW::dv(2 * rc, k.gm1);
       if (dxdt > s_starr) {
         if (dxdt > rv[A] + rc)
           take(r_rho, rv, rp);
@@ -773,7 +791,11 @@ __device__ __forceinline__ void riemann_sample(const Consts& k, const RI* l, con
       double kv[ND];
 #pragma unroll
       for (int d = 0; d < ND; ++d) kv[d] = right ? rv[d] : lv[d];
-      const double ps_pk = PW::dv(ps, kp);
+      // p* / p_K is exactly 1 at a contact (p* = p_K, finite and > 0) under
+      // IEEE division: a wave of contacts skips the division
+      const double ps_pk = !PW::kFast && wave_all(ps == kp && kp > 0.0 && kp < INFINITY)
+                               ? 1.0
+                               : PW::dv(ps, kp);
       take(kr, kv, kp);
       if (ps > kp) {  // shock on side K
         double speed = kv[A] + sc * PW::sq(k.gp1_2g * ps_pk + k.gm1_2g);

@@ -244,6 +244,15 @@ class SamrBox:
         c_ic = _lib.make_ic(ic)
         check(_lib.load().rhy_box_paint(self._h, C.byref(c_ic)), "rhy_box_paint")
 
+    def draw(self, params, drawing):
+        """Paint the device state from a drawing (rhyme_drawing_init: uniform
+        canvas, sharp shapes in order, harmonic perturbations; the dict
+        layout of ics.draw); gamma from params.  The host mirror is not
+        touched (download() to see it)."""
+        d, keep = _lib.make_drawing(drawing)
+        check(_lib.load().rhy_box_draw(self._h, C.byref(params), C.byref(d)), "rhy_box_draw")
+        del keep
+
     # -- timing helpers used by bench.py
     def set_profiling(self, on):
         check(_lib.load().rhy_box_set_profiling(self._h, 1 if on else 0),

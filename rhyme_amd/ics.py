@@ -57,27 +57,101 @@ def two_state_1d(n=1024, seed=0, gamma=5.0 / 3.0, ncmp=None):
     return _fill(1, [n], ncmp, gamma, rho, v, p), dict(rho_l=rl, p_l=pl, rho_r=rr, p_r=pr)
 
 
-def spheres_2d(n=1024, seed=0, gamma=5.0 / 3.0, nspheres=16, amp=0.05, lam=32.0, ncmp=None):
-    """2D spheres workload: canvas rho = p = 1 at rest; harmonic perturbation
-    of p along x with amplitude amp and wavelength lam cells; then nspheres
-    sharp discs at seeded centres, radius U[8, 32] cells, rho and p
-    U[0.1, 10]; later discs overwrite earlier ones."""
+SPHERE, CUBOID = 1, 2
+
+
+def harmonic_profile(n, amp, lam):
+    """A cos(kx) of a harmonic perturbation at the n cell centres x = q - 1/2
+    of an axis, kx = 2 pi / lambda * x as rhyme_drawing_apply_perturbations.f90
+    :115 evaluates it (pi = 4 atan(1))."""
+    x = np.arange(1, n + 1, dtype=np.float64) - 0.5
+    return amp * np.cos(2.0 * np.pi / lam * x)
+
+
+def draw(drawing, ndim, dims, gamma=5.0 / 3.0, ncmp=None):
+    """Host array of the state rhy_box_draw paints (rhyme_drawing_init.f90
+    :19-78 for a uniform canvas, sharp shapes and harmonic perturbations):
+
+    * canvas: every cell gets conv_prim_to_cons of (rho, v, p);
+    * shapes in order, later over earlier: a sphere takes the cells whose
+      centre x_d = q_d - 1/2 (1-based q) satisfies sum_d (x_d - c_d)^2 <=
+      radius^2 (axis order), a cuboid the cells with lo <= q <= hi;
+    * then every cell gets conv_prim_to_cons(h) added to its hydro
+      components, h = 0 + sum over the perturbations of profile * (rho, v, p)
+      (rhyme_drawing_apply_perturbations.f90:51-60, 115, 184).
+
+    drawing: dict(rho, v, p, shapes=[dict(kind, centre, radius | lo, hi, rho,
+    v, p)], harmonics=[dict(axis, profile, rho, v, p)])."""
+    ncmp = ncmp or ncmp_for(ndim)
+    dims = list(dims)[:ndim]
+    shape = dims[::-1]
+    pos = [(np.arange(1, n + 1, dtype=np.float64) - 0.5) for n in dims]
+    grids = np.meshgrid(*pos[::-1], indexing="ij")[::-1]  # x, y, z arrays
+    qs = np.meshgrid(*[np.arange(1, n + 1) for n in dims[::-1]], indexing="ij")[::-1]
+
+    def vel(c):
+        v = list(c["v"]) + [0.0] * 3
+        return v[:ndim]
+    rho = np.full(shape, float(drawing["rho"]))
+    v = [np.full(shape, x) for x in vel(drawing)]
+    p = np.full(shape, float(drawing["p"]))
+    for sh in drawing.get("shapes", []):
+        if sh["kind"] == SPHERE:
+            r2 = None
+            for d in range(ndim):
+                x = grids[d] - sh["centre"][d]
+                r2 = x * x if r2 is None else r2 + x * x
+            m = r2 <= sh["radius"] * sh["radius"]
+        else:
+            m = np.ones(shape, dtype=bool)
+            for d in range(ndim):
+                m &= (qs[d] >= sh["lo"][d]) & (qs[d] <= sh["hi"][d])
+        rho[m] = sh["rho"]
+        for d, x in enumerate(vel(sh)):
+            v[d][m] = x
+        p[m] = sh["p"]
+    cons = prim_to_cons(gamma, rho, np.array(v).reshape([ndim] + shape), p)
+    harms = drawing.get("harmonics", [])
+    if harms:
+        h = [np.zeros(shape) for _ in range(ndim + 2)]  # rho, v[ndim], p
+        for hm in harms:
+            prof = np.asarray(hm["profile"], dtype=np.float64)
+            bshape = [1] * ndim
+            bshape[ndim - 1 - hm["axis"]] = dims[hm["axis"]]
+            a = prof.reshape(bshape)
+            base = [hm["rho"]] + vel(hm) + [hm["p"]]
+            for c in range(ndim + 2):
+                h[c] = h[c] + a * base[c]
+        up = prim_to_cons(gamma, h[0], np.array(h[1:ndim + 1]).reshape([ndim] + shape), h[-1])
+        cons = [c + u for c, u in zip(cons, up)]
+    cells = np.zeros(box_shape(ndim, dims, ncmp), dtype=np.float64)
+    sl = tuple(slice(2, 2 + n) for n in dims[::-1])
+    for c, arr in enumerate(cons):
+        cells[(c,) + sl] = arr
+    return cells
+
+
+def spheres_2d_drawing(n=1024, seed=0, nspheres=16, amp=0.05, lam=32.0):
+    """The 2D spheres workload as a drawing: canvas rho = p = 1 at rest;
+    nspheres sharp spheres at seeded centres U[0, n)^2 (cell units), radius
+    U[8, 32] cells, rho and p U[0.1, 10], later over earlier; then a harmonic
+    perturbation of p along x, amplitude amp, wavelength lam cells (the
+    reference's order: shapes, then perturbations)."""
     g = rng(seed)
-    ncmp = ncmp or ncmp_for(2)
-    i = np.arange(1, n + 1, dtype=np.float64)
-    x = np.broadcast_to(i[None, :], (n, n))
-    y = np.broadcast_to(i[:, None], (n, n))
-    rho = np.ones((n, n))
-    p = np.ones((n, n)) + amp * np.cos(2.0 * np.pi * (x - 0.5) / lam)
+    shapes = []
     for _ in range(nspheres):
-        cx, cy = g.uniform(1, n, size=2)
+        cx, cy = g.uniform(0.0, n, size=2)
         rad = g.uniform(8, 32)
         r_s, p_s = g.uniform(0.1, 10.0, size=2)
-        m = (x - cx) ** 2 + (y - cy) ** 2 <= rad * rad
-        rho[m] = r_s
-        p[m] = p_s
-    v = np.zeros((2, n, n))
-    return _fill(2, [n, n], ncmp, gamma, rho, v, p)
+        shapes.append(dict(kind=SPHERE, centre=[float(cx), float(cy), 0.0], radius=float(rad),
+                           rho=float(r_s), v=[0.0, 0.0], p=float(p_s)))
+    harm = dict(axis=0, profile=harmonic_profile(n, amp, lam), rho=0.0, v=[0.0, 0.0], p=1.0)
+    return dict(rho=1.0, v=[0.0, 0.0], p=1.0, shapes=shapes, harmonics=[harm])
+
+
+def spheres_2d(n=1024, seed=0, gamma=5.0 / 3.0, nspheres=16, amp=0.05, lam=32.0, ncmp=None):
+    """2D spheres workload (host array of spheres_2d_drawing)."""
+    return draw(spheres_2d_drawing(n, seed, nspheres, amp, lam), 2, [n, n], gamma, ncmp)
 
 
 def blast_3d(n=256, seed=0, gamma=5.0 / 3.0, radius=3.0, jitter=0.01, p0=1e-5, dims=None,

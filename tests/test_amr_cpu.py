@@ -1,7 +1,8 @@
 """The two-level hierarchy's CPU mirror (oracle/amr.py): placement rules,
 prolongation and restriction against explicit loops over the cells, and
 whole hierarchy steps that must keep a uniform state uniform, bit for bit,
-on both levels."""
+on both levels; refluxing's register and corrections against loops over the
+faces, and its conservation of mass and energy to rounding."""
 import numpy as np
 import pytest
 
@@ -188,3 +189,108 @@ def test_split_fine_box_identical(ndim):
     inner_w = (slice(0, ndim + 2),) + (slice(2, -2),) * (ndim - 1)
     assert np.array_equal(fw[inner_w + (slice(2, 2 + left[0]),)], fl[inner_w + (slice(2, -2),)])
     assert np.array_equal(fw[inner_w + (slice(2 + left[0], -2),)], fr[inner_w + (slice(2, -2),)])
+
+
+def total(ndim, cells, c):
+    """Interior sum of component c, compensated (math.fsum)."""
+    import math
+    inner = (c,) + (slice(2, -2),) * ndim
+    return math.fsum(cells[inner].ravel())
+
+
+REFLUX_CASES = {
+    # periodic coarse box, two fine boxes sharing a face and one apart
+    2: ([32, 24], [[8, 8], [8, 8], [6, 4]], [[5, 6], [9, 6], [22, 15]]),
+    3: ([16, 16, 12], [[8, 6, 4], [4, 6, 4], [4, 4, 4]], [[3, 4, 4], [7, 4, 4], [11, 11, 7]]),
+}
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_refluxing_conserves_mass_and_energy(ndim):
+    """With a periodic coarse box, the hierarchy with refluxing keeps the
+    coarse level's (after restriction, the whole hierarchy's) total mass
+    and energy to rounding; without it the coarse-fine flux mismatch moves
+    them by orders of magnitude more."""
+    cdims, fdims, lo = REFLUX_CASES[ndim]
+    oamr.check_placement(ndim, cdims, fdims, lo)
+    p = mk(ndim, hydro.PERIODIC)
+    pf = oamr.fine_params(p)
+    dx0 = [1.0 / cdims[0]] * ndim
+    coarse0 = ics.random_smooth(ndim, cdims, seed=40 + ndim, vamp=0.8)
+    fines0 = []
+    for d, l in zip(fdims, lo):
+        f = np.zeros(hydro.box_shape(ndim, d, coarse0.shape[0]))
+        idx = [l[a] + (np.arange(-1, d[a] + 3) + 1) // 2 for a in reversed(range(ndim))]
+        f[...] = coarse0[(slice(None),) + np.ix_(*idx)]
+        fines0.append(f)
+    drift = {}
+    for refl in (True, False):
+        coarse = coarse0.copy()
+        fines = [f.copy() for f in fines0]
+        for f, l in zip(fines, lo):   # the coarse level starts as the restriction
+            oamr.restrict(ndim, f, l, coarse)
+        m0, e0 = total(ndim, coarse, 0), total(ndim, coarse, ndim + 1)
+        for _ in range(4):
+            oamr.step(p, pf, cdims, coarse, dx0, fines, lo, refluxing=refl)
+        drift[refl] = (abs(total(ndim, coarse, 0) - m0) / abs(m0),
+                       abs(total(ndim, coarse, ndim + 1) - e0) / abs(e0))
+    assert max(drift[True]) <= 1e-14, drift
+    assert min(drift[False]) > 1e3 * max(max(drift[True]), 1e-16), drift
+
+
+def test_register_and_reflux_against_loops():
+    """3D: the register (pairwise sums over the 4 fine faces of each coarse
+    face, both substeps) and the corrections of the coarse cells outside
+    each face, against loops over the faces with the indices written out."""
+    ndim, cdims = 3, [12, 10, 10]
+    fdims, lo = [[6, 4, 8], [4, 4, 4]], [[3, 3, 2], [8, 6, 6]]
+    nv = ndim + 2
+    g = np.random.default_rng(8)
+    fr = [[g.uniform(-1, 1, (ndim, nv) + tuple(n + 4 for n in d[::-1])) for d in fdims]
+          for _ in range(2)]
+    reg = {}
+    for k in range(2):
+        for i, l in enumerate(lo):
+            oamr.register(ndim, fr[k][i], l, reg, i, k == 0)
+    cflux = g.uniform(-1, 1, (ndim, nv) + tuple(n + 4 for n in cdims[::-1]))
+    coarse = g.uniform(1, 2, (nv,) + tuple(n + 4 for n in cdims[::-1]))
+    want = coarse.copy()
+    dt, dx = 0.01, [0.1, 0.2, 0.25]
+    oamr.reflux(ndim, coarse, cflux, fdims, lo, reg, dt, dx)
+
+    def at(arr, x, y, z):  # Fortran indices -> array element (x last)
+        return arr[z + 1, y + 1, x + 1]
+    for a in range(ndim):
+        for side in (0, 1):
+            for i, (d, l) in enumerate(zip(fdims, lo)):
+                tr = [e for e in range(ndim) if e != a]
+                for tu in range(d[tr[0]] // 2):
+                    for tv in range(d[tr[1]] // 2):
+                        for c in range(nv):
+                            s = [0.0, 0.0]
+                            for k in range(2):
+                                f = []
+                                for y in range(2):
+                                    for x in range(2):
+                                        q = [0, 0, 0]
+                                        q[a] = d[a] if side else 0
+                                        q[tr[0]] = 2 * tu + 1 + x
+                                        q[tr[1]] = 2 * tv + 1 + y
+                                        f.append(at(fr[k][i][a][c], *q))
+                                s[k] = (f[0] + f[1]) + (f[2] + f[3])
+                            r = s[0] + s[1]
+                            ix = tuple(0 if e == a else (tu if e == tr[0] else tv)
+                                       for e in (2, 1, 0))
+                            assert reg[(i, a, side)][c][ix] == r
+                            out = [0, 0, 0]
+                            out[a] = l[a] + d[a] // 2 if side else l[a] - 1
+                            out[tr[0]] = l[tr[0]] + tu
+                            out[tr[1]] = l[tr[1]] + tv
+                            low = list(out)
+                            low[a] = out[a] - 1 if side else out[a]
+                            fc = at(cflux[a][c], *low)
+                            favg = r * 0.125
+                            dd = dt / dx[a] * (favg - fc) if side else dt / dx[a] * (fc - favg)
+                            z, y, x = out[2] + 1, out[1] + 1, out[0] + 1
+                            want[c, z, y, x] = want[c, z, y, x] + dd
+    np.testing.assert_array_equal(coarse, want)

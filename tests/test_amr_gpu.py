@@ -1,6 +1,7 @@
 """Two-level hierarchy on the device (rhyme_amd.amr) against its CPU mirror
 (oracle/amr.py): ratio-2 prolongation of the fine boxes' face ghosts (linear
-in time over the two substeps), subcycled fine steps, restriction.  The
+in time over the two substeps), subcycled fine steps, refluxing of the
+coarse cells next to the fine boxes, restriction.  The
 reference has no coarse-fine coupling (src/main.f90:118-136 advances every
 box alone), so oracle/amr.py defines the semantics; the kernels under it are
 the single-box and batch steps whose parity the other tests establish."""
@@ -229,3 +230,52 @@ def test_bad_placement_rejected_by_library():
                                 lo.ctypes.data_as(C.POINTER(C.c_int32)))
         assert rc == _lib.RHY_ERR_SHAPE
     h.close()
+
+
+def total(ndim, cells, c):
+    import math
+    return math.fsum(cells[(c,) + (slice(2, -2),) * ndim].ravel())
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_refluxing_conserves_on_device(ndim, precision):
+    """Periodic coarse box, adjacent and separate fine boxes: with refluxing
+    the device hierarchy keeps total mass and energy (the coarse level after
+    restriction) to rounding in FP64 and to FP32 rounding in the mixed form,
+    and matches the oracle's hierarchy; without refluxing both move."""
+    cases = {
+        2: ([32, 24], [[8, 8], [8, 8], [6, 4]], [[5, 6], [9, 6], [22, 15]]),
+        3: ([16, 16, 12], [[8, 6, 4], [4, 6, 4], [4, 4, 4]], [[3, 4, 4], [7, 4, 4], [11, 11, 7]]),
+    }
+    cdims, fdims, lo = cases[ndim]
+    p = params(ndim, hydro.MINMOD, hydro.PERIODIC, courant=0.2)
+    dx0 = [1.0 / cdims[0]] * ndim
+    coarse = ics.random_smooth(ndim, cdims, seed=40 + ndim, vamp=0.8)
+    fines = [amr.fine_from_coarse(ndim, coarse, l, d) for l, d in zip(lo, fdims)]
+    for f, l in zip(fines, lo):
+        oamr.restrict(ndim, f, l, coarse)
+    drift = {}
+    for refl in (True, False):
+        say("reflux conservation", ndim, "precision", precision, "refluxing", refl)
+        h = amr.AmrHierarchy(ndim, cdims, dx0, lo, fdims, precision=precision, refluxing=refl)
+        h.upload(coarse, fines)
+        dc0, _ = h.download()
+        m0, e0 = total(ndim, dc0, 0), total(ndim, dc0, ndim + 1)
+        dts = h.run(p, 4)
+        dc, dfs = h.download()
+        h.close()
+        drift[refl] = max(abs(total(ndim, dc, 0) - m0) / abs(m0),
+                          abs(total(ndim, dc, ndim + 1) - e0) / abs(e0))
+        rc, rfs = coarse.copy(), [f.copy() for f in fines]
+        pc = oracle.params_from(p)
+        rdts = [oamr.step(pc, oamr.fine_params(pc), cdims, rc, dx0, rfs, lo, refluxing=refl)
+                for _ in range(4)]
+        tol = 1e-12 if precision == hydro.FP64 else 1e-5
+        assert np.max(np.abs(dts - rdts) / rdts) <= tol
+        assert max(rel_l1(dc, rc, ndim)) <= tol
+        for a, b in zip(dfs, rfs):
+            assert max(rel_l1(a, b, ndim)) <= tol
+    say("  drift with / without refluxing", drift[True], drift[False])
+    assert drift[True] <= (1e-14 if precision == hydro.FP64 else 1e-6)
+    assert drift[False] > 10 * drift[True]
