@@ -26,7 +26,10 @@ own (state planes + flux scratch, batch.box_bytes) stay within the budget and
 its pool's arena has a free block large enough; otherwise it waits, and so
 does every later job (first come, first served).  Each pool's arena is sized
 to the whole budget, so the budget is the only limit that binds except for
-fragmentation.
+fragmentation -- and the device holds two arenas of the budget's size (one
+per dimensionality), 2 x budget_bytes of HBM, of which at most the budget is
+ever in use; the constructor fails with the library's out-of-memory error
+when the device cannot hold both.
 """
 import collections
 import time

@@ -79,6 +79,33 @@ def test_pool_arena_first_fit_and_refusal():
     pool.close()
 
 
+def test_pool_fragmentation_refuses_then_coalesces():
+    """A box that fits the free bytes but no single free block is refused
+    (PoolFull, as the server's admission then defers it); freeing the block
+    between the two holes coalesces them and the same box is placed."""
+    say("pool fragmentation")
+    p = serve.job_params(3)
+    dims, big = [10, 12, 9], [10, 12, 18]
+    nb, nbig = box_bytes(3, dims), box_bytes(3, big)
+    assert nb < nbig <= 2 * nb
+    pool = SamrPool(3, 4 * nb + 100, max_boxes=8)
+    cells = np.zeros(hydro.box_shape(3, dims, p.ncmp))
+    s = [pool.add(dims, [0.1] * 3, cells, hydro.MINMOD) for _ in range(4)]
+    pool.remove(s[0])
+    pool.remove(s[2])
+    free, largest = pool.free_bytes()
+    assert free >= nbig > largest >= nb
+    bcells = np.zeros(hydro.box_shape(3, big, p.ncmp))
+    with pytest.raises(PoolFull):
+        pool.add(big, [0.1] * 3, bcells, hydro.MINMOD)
+    pool.remove(s[1])
+    free, largest = pool.free_bytes()
+    assert largest >= 3 * nb
+    slot = pool.add(big, [0.1] * 3, bcells, hydro.MINMOD)
+    np.testing.assert_array_equal(pool.download(slot), bcells)
+    pool.close()
+
+
 def test_pool_slot_limit():
     say("pool slot limit")
     dims = [8, 8]
