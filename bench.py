@@ -400,6 +400,7 @@ def serving_leg(njobs, rate, budget_gib, quantum, seed, precision=0):
     srv.close()
     rep.update(rate_jobs_per_s=rate, budget_gib=budget_gib, quantum_steps=quantum,
                peak_resident_gib=srv.peak_bytes / float(1 << 30), admission_deferrals=srv.deferrals,
+               admission=srv.admission, admitted_past_head=srv.bypassed,
                quanta=srv.quanta, wall_s=wall, host_seconds=srv.seconds, unit="zone-updates/s (aggregate), ms latency",
                precision="mixed" if precision else "fp64")
     return rep

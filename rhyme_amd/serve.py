@@ -23,8 +23,13 @@ final diagnostics.
 
 Admission: a job is admitted when the bytes of all resident jobs plus its
 own (state planes + flux scratch, batch.box_bytes) stay within the budget and
-its pool's arena has a free block large enough; otherwise it waits, and so
-does every later job (first come, first served).  Each pool's arena is sized
+its pool's arena has a free block large enough.  With admission="fifo" a
+job that does not fit holds every later one (first come, first served).
+With admission="backfill" (the default) the jobs behind it that fit are
+admitted past it -- up to `lookahead` of them per admission pass -- until
+it has waited `max_bypass_s` seconds since it first failed to fit; from then
+on nothing passes it, so a large job is delayed by at most that long plus
+the time the resident set needs to drain.  Each pool's arena is sized
 to the whole budget, so the budget is the only limit that binds except for
 fragmentation -- and the device holds two arenas of the budget's size (one
 per dimensionality), 2 x budget_bytes of HBM, of which at most the budget is
@@ -66,6 +71,7 @@ class Job:
     dts: list = field(default_factory=list)
     diag: dict = None           # rhy_diag_t of the final (or failing) state
     failed: str = None          # why the job was ended early, if it was
+    blocked_since: float = None  # when it first failed to fit as the head job
 
     def zones(self):
         return int(np.prod(self.dims))
@@ -141,7 +147,13 @@ class HydroServer:
 
     def __init__(self, budget_bytes, precision=FP64, quantum=10, params=None, max_boxes=4096,
                  tiles=None, keep_results=True, record_dts=False, check_every=0,
-                 diagnose=False):
+                 diagnose=False, admission="backfill", lookahead=64, max_bypass_s=1.0):
+        if admission not in ("fifo", "backfill"):
+            raise ValueError("admission must be 'fifo' or 'backfill'")
+        self.admission = admission
+        self.lookahead = int(lookahead)
+        self.max_bypass_s = float(max_bypass_s)
+        self.bypassed = 0  # admissions past a waiting head job
         self.budget = int(budget_bytes)
         self.precision = precision
         self.quantum = int(quantum)
@@ -228,10 +240,26 @@ class HydroServer:
         while left:
             ta = clock()
             now = ta - t0
-            # admission, first come first served
+            # admission: in arrival order; with backfill, arrived jobs that
+            # fit pass a head job that does not, for up to max_bypass_s
             while queue and queue[0].arrival <= now:
                 if not self._try_admit(queue[0], now):
                     self.deferrals += 1
+                    head = queue[0]
+                    if head.blocked_since is None:
+                        head.blocked_since = now
+                    if (self.admission == "backfill" and
+                            now - head.blocked_since < self.max_bypass_s):
+                        passed = []
+                        for i in range(1, min(len(queue), self.lookahead + 1)):
+                            j = queue[i]
+                            if j.arrival > now:
+                                break
+                            if self._try_admit(j, now):
+                                passed.append(i)
+                        for i in reversed(passed):
+                            del queue[i]
+                        self.bypassed += len(passed)
                     break
                 queue.popleft()
             tb = clock()

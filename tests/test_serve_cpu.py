@@ -1,6 +1,7 @@
 """Host logic of the serving loop, without a device: the workload generator
 (sizes, halves, arrivals, determinism), first-come-first-served admission
-under the byte budget, quanta that never overshoot a job's step count, and
+under the byte budget, backfill admission and its bound on how long a
+blocked job is passed, quanta that never overshoot a job's step count, and
 the latency report.  The device pools are replaced by a stand-in that keeps
 the same byte accounting (rhy_pool_box_bytes, a host-only call)."""
 import numpy as np
@@ -92,7 +93,7 @@ def test_fifo_admission_under_budget(fake_pools):
     jobs = serve.serving_jobs(njobs=30, rate=1e9, seed=2, steps=7, n2d=(8, 32), n3d=(6, 16))
     sizes = [batch.box_bytes(j.ndim, j.dims) for j in jobs]
     budget = max(sizes) * 2
-    srv = serve.HydroServer(budget, quantum=3, keep_results=False)
+    srv = serve.HydroServer(budget, quantum=3, keep_results=False, admission="fifo")
     srv.serve(jobs)
     assert srv.peak_bytes <= budget and srv.deferrals > 0
     # admission order is arrival order
@@ -105,6 +106,44 @@ def test_fifo_admission_under_budget(fake_pools):
     for pool in FakePool.instances:
         for q, _ in pool.runs:
             assert 1 <= q <= 3
+
+
+def test_backfill_passes_a_blocked_head_for_a_bounded_time(fake_pools):
+    """Backfill admits arrived jobs that fit past a head job that does not,
+    only while the head has been blocked for less than max_bypass_s (on the
+    server's clock); every job still completes, within the budget."""
+    def mk(jid, nd, n):
+        return serve.Job(jid=jid, ndim=nd, dims=[n] * nd, dx=[1.0 / n] * nd,
+                         limiter=hydro.MINMOD, bc_types=[hydro.OUTFLOW] * 6, steps=6,
+                         arrival=0.0, kind="blast", seed=jid, ic=ics.ic_blast(jid))
+    jobs = [mk(0, 3, 24), mk(1, 3, 24)] + [mk(2 + i, 2, 12) for i in range(40)]
+    nb = [batch.box_bytes(j.ndim, j.dims) for j in jobs]
+    budget = nb[0] + 3 * nb[2]
+    t = [0.0]
+
+    def clock():  # every reading of the clock advances it by 10 ms
+        t[0] += 0.01
+        return t[0]
+    window = 0.3
+    srv = serve.HydroServer(budget, quantum=3, keep_results=False, max_bypass_s=window)
+    srv.serve(jobs, clock=clock, sleep=lambda s: None)
+    head = jobs[1]
+    assert head.blocked_since is not None and srv.bypassed > 0
+    passed = [j for j in jobs[2:] if j.admitted < head.admitted]
+    assert passed and all(j.admitted < head.blocked_since + window + 0.05 for j in passed)
+    assert any(j.admitted > head.admitted for j in jobs[2:])  # the rest waited for it
+    assert srv.peak_bytes <= budget
+    assert all(j.done == j.steps for j in jobs)
+    # FIFO on the same stream passes nothing
+    t[0] = 0.0
+    for j in jobs:
+        j.admitted = j.finished = j.blocked_since = None
+        j.done = 0
+    fifo = serve.HydroServer(budget, quantum=3, keep_results=False, admission="fifo")
+    fifo.serve(jobs, clock=clock, sleep=lambda s: None)
+    assert fifo.bypassed == 0
+    order = sorted(jobs, key=lambda j: (j.admitted, j.jid))
+    assert [j.jid for j in order] == list(range(len(jobs)))
 
 
 def test_quanta_bounded_when_nothing_waits(fake_pools):

@@ -173,6 +173,37 @@ def test_served_jobs_match_oracle():
     srv.close()
 
 
+@pytest.mark.parametrize("admission", ["fifo", "backfill"])
+def test_backfill_admits_past_a_blocked_head(admission):
+    """A large job that does not fit while another runs holds a small one
+    behind it under FIFO admission; with backfill the small one is admitted
+    past it at once.  Either way every result equals the job run alone."""
+    big = [dict(ndim=3, dims=[24] * 3), dict(ndim=3, dims=[24] * 3)]
+    small = dict(ndim=2, dims=[12, 12])
+    jobs = []
+    for i, d in enumerate(big + [small]):
+        n = d["dims"][0]
+        jobs.append(serve.Job(jid=i, ndim=d["ndim"], dims=d["dims"], dx=[1.0 / n] * d["ndim"],
+                              limiter=hydro.MINMOD, bc_types=[hydro.OUTFLOW] * 6, steps=6,
+                              arrival=0.0, kind="blast", seed=i,
+                              ic=ics.ic_blast(100 + i)))
+    nb = [box_bytes(j.ndim, j.dims) for j in jobs]
+    budget = nb[0] + nb[2] + 64  # the first big job and the small one, not both big ones
+    say("admission", admission, "bytes", nb, "budget", budget)
+    srv = serve.HydroServer(budget, quantum=3, admission=admission, max_bypass_s=60.0)
+    srv.serve(jobs)
+    a, b, c = jobs
+    assert a.admitted <= c.admitted and a.admitted <= b.admitted
+    if admission == "backfill":
+        assert c.admitted < b.admitted and srv.bypassed == 1
+    else:
+        assert c.admitted >= b.admitted and srv.bypassed == 0
+    for j in jobs:
+        ref, _ = serve.run_alone(j, srv.params[j.ndim])
+        np.testing.assert_array_equal(j.result, ref)
+    srv.close()
+
+
 def test_serving_report_small_stream():
     jobs = serve.serving_jobs(njobs=40, rate=400.0, seed=3, steps=10, n2d=(32, 96),
                               n3d=(16, 40))
