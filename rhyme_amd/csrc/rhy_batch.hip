@@ -609,6 +609,12 @@ struct rhy_batch {
   double* amr_reg = nullptr;         // flux register of the coarse-fine faces (lazy)
   int64_t amr_reg_len = 0;           // doubles in use
   int64_t amr_reg_cap = 0;
+  // the register's work items on the device, built once per placement
+  AmrItem* reg_items = nullptr;
+  int64_t reg_items_cap = 0;
+  std::vector<int64_t> reg_key;      // placement and box extents they were built for
+  int reg_pb[7] = {0, 0, 0, 0, 0, 0, 0};
+  int64_t reg_pt[6] = {0, 0, 0, 0, 0, 0};
   std::vector<AmrItem> amr_host;     // their host copy, kept until the stream is idle
 };
 
@@ -1215,7 +1221,7 @@ int rhy_batch_destroy(rhy_batch_t* b) {
                   (void*)b->dtile_seg, (void*)b->partial, (void*)b->dt, (void*)b->dtlog, (void*)b->step,
                   (void*)b->inflow, (void*)b->diag, (void*)b->defer_count,
                   (void*)b->defer_items, (void*)b->snap, (void*)b->amr_items,
-                  (void*)b->amr_reg})
+                  (void*)b->amr_reg, (void*)b->reg_items})
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -1658,40 +1664,77 @@ static int64_t reflux_items(rhy_batch_t* f, const int32_t* lo, int pass_begin[7]
   return reg;
 }
 
+// The register/reflux work items for this placement on the device (built
+// and uploaded when the placement or the boxes change, synchronously, so
+// the calls themselves only enqueue kernels); returns the register length.
+static int reflux_prepare(rhy_batch_t* f, const int32_t* lo, int64_t* len) {
+  std::vector<int64_t> key;
+  for (int i = 0; i < (int)f->slots.size(); ++i)
+    for (int d = 0; d < 3; ++d) {
+      key.push_back(lo[3 * i + d]);
+      key.push_back(f->slots[i].B.G.n[d]);
+    }
+  if (key == f->reg_key && f->reg_items) {
+    *len = f->amr_reg_len;
+    return RHY_OK;
+  }
+  BHIP_TRY(hipStreamSynchronize(f->stream));
+  const int64_t n = reflux_items(f, lo, f->reg_pb, f->reg_pt);
+  const int64_t ni = (int64_t)f->amr_host.size();
+  if (ni > f->reg_items_cap) {
+    if (f->reg_items) (void)hipFree(f->reg_items);
+    f->reg_items = nullptr;
+    f->reg_items_cap = 0;
+    if (dev_alloc(&f->reg_items, sizeof(AmrItem) * (size_t)std::max<int64_t>(ni, 1)) !=
+        hipSuccess) {
+      f->reg_items = nullptr;
+      return bfail(RHY_ERR_OOM, "flux register items allocation failed");
+    }
+    f->reg_items_cap = ni;
+  }
+  if (ni)
+    BHIP_TRY(hipMemcpy(f->reg_items, f->amr_host.data(), sizeof(AmrItem) * (size_t)ni,
+                       hipMemcpyHostToDevice));
+  if (n > f->amr_reg_cap) {
+    if (f->amr_reg) (void)hipFree(f->amr_reg);
+    f->amr_reg = nullptr;
+    f->amr_reg_cap = 0;
+    if (dev_alloc(&f->amr_reg, sizeof(double) * (size_t)std::max<int64_t>(n, 1)) !=
+        hipSuccess) {
+      f->amr_reg = nullptr;
+      f->reg_key.clear();
+      return bfail(RHY_ERR_OOM, "flux register allocation failed");
+    }
+    f->amr_reg_cap = n;
+  }
+  f->amr_reg_len = n;
+  f->reg_key = key;
+  *len = n;
+  return RHY_OK;
+}
+
 int rhy_amr_flux_register(rhy_batch_t* f, const int32_t* lo, int32_t reset) {
   if (!f || !lo) return bfail(RHY_ERR_ARG, "NULL argument");
   if (f->pool) return bfail(RHY_ERR_ARG, "the fine level is a fixed batch, not a pool");
   for (int i = 0; i < (int)f->slots.size(); ++i)
     for (int d = 0; d < f->ndim; ++d)
       if (f->slots[i].B.G.n[d] % 2) return bfail(RHY_ERR_SHAPE, "fine extents must be even");
-  int pb[7];
-  int64_t pt[6];
-  const int64_t len = reflux_items(f, lo, pb, pt);
-  if (!reset && len != f->amr_reg_len)
-    return bfail(RHY_ERR_ARG, "flux register not started (reset = 1) for these boxes");
-  if (len > f->amr_reg_cap) {
-    BHIP_TRY(hipStreamSynchronize(f->stream));
-    if (f->amr_reg) (void)hipFree(f->amr_reg);
-    f->amr_reg = nullptr;
-    f->amr_reg_cap = 0;
-    if (dev_alloc(&f->amr_reg, sizeof(double) * (size_t)len) != hipSuccess) {
-      f->amr_reg = nullptr;
-      f->amr_reg_len = 0;
-      return bfail(RHY_ERR_OOM, "flux register allocation failed");
-    }
-    f->amr_reg_cap = len;
-  }
-  f->amr_reg_len = len;
+  const std::vector<int64_t> before = f->reg_key;
+  int64_t len = 0;
   int rc;
-  if ((rc = amr_upload(f))) return rc;
+  if ((rc = reflux_prepare(f, lo, &len))) return rc;
+  if (!reset && f->reg_key != before)
+    return bfail(RHY_ERR_ARG, "flux register not started (reset = 1) for these boxes");
+  // enqueued on the batch's stream behind the substep; no wait, so the
+  // coarse step on the box's stream overlaps the fine substep
   const int nd = f->ndim;
   for (int pass = 0; pass < 2 * nd; ++pass) {
-    const int ni = pb[pass + 1] - pb[pass];
-    if (!pt[pass] || !ni) continue;
-    const AmrItem* items = f->amr_items + pb[pass];
-#define RHY_REG(ND, R)                                                                         \
-  hipLaunchKernelGGL((amr_register_kernel<ND, R>), dim3(cdiv(pt[pass], 256)), dim3(256), 0,   \
-                     f->stream, items, ni, pt[pass], static_cast<const R*>(f->arena),         \
+    const int ni = f->reg_pb[pass + 1] - f->reg_pb[pass];
+    if (!f->reg_pt[pass] || !ni) continue;
+    const AmrItem* items = f->reg_items + f->reg_pb[pass];
+#define RHY_REG(ND, R)                                                                          \
+  hipLaunchKernelGGL((amr_register_kernel<ND, R>), dim3(cdiv(f->reg_pt[pass], 256)), dim3(256), \
+                     0, f->stream, items, ni, f->reg_pt[pass], static_cast<const R*>(f->arena), \
                      f->amr_reg, reset ? 1 : 0)
     if (f->precision == RHY_MIXED) {
       if (nd == 1) RHY_REG(1, float);
@@ -1705,7 +1748,6 @@ int rhy_amr_flux_register(rhy_batch_t* f, const int32_t* lo, int32_t reset) {
 #undef RHY_REG
   }
   BHIP_TRY(hipGetLastError());
-  BHIP_TRY(hipStreamSynchronize(f->stream));
   return RHY_OK;
 }
 
@@ -1717,21 +1759,20 @@ int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double 
   if ((rc = box_view(coarse, &vc)))
     return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
   if ((rc = amr_check(f, vc, lo))) return rc;
-  int pb[7];
-  int64_t pt[6];
-  const int64_t len = reflux_items(f, lo, pb, pt);
-  if (len != f->amr_reg_len || !f->amr_reg)
+  const std::vector<int64_t> before = f->reg_key;
+  int64_t len = 0;
+  if ((rc = reflux_prepare(f, lo, &len))) return rc;
+  if (f->reg_key != before || !f->amr_reg)
     return bfail(RHY_ERR_ARG, "flux register not filled for these boxes");
-  if ((rc = amr_upload(f))) return rc;
   const int nd = f->ndim;
   for (int pass = 0; pass < 2 * nd; ++pass) {
-    const int ni = pb[pass + 1] - pb[pass];
-    if (!pt[pass] || !ni) continue;
-    const AmrItem* items = f->amr_items + pb[pass];
+    const int ni = f->reg_pb[pass + 1] - f->reg_pb[pass];
+    if (!f->reg_pt[pass] || !ni) continue;
+    const AmrItem* items = f->reg_items + f->reg_pb[pass];
     const double dtdx = dt / dx[pass / 2];
-#define RHY_REFLUX(ND, R)                                                                      \
-  hipLaunchKernelGGL((amr_reflux_kernel<ND, R>), dim3(cdiv(pt[pass], 256)), dim3(256), 0,     \
-                     f->stream, items, ni, pt[pass], f->amr_reg, vc.geom,                     \
+#define RHY_REFLUX(ND, R)                                                                       \
+  hipLaunchKernelGGL((amr_reflux_kernel<ND, R>), dim3(cdiv(f->reg_pt[pass], 256)), dim3(256), 0, \
+                     f->stream, items, ni, f->reg_pt[pass], f->amr_reg, vc.geom,               \
                      static_cast<R*>(vc.cells), static_cast<const R*>(vc.flux), dtdx)
     if (f->precision == RHY_MIXED) {
       if (nd == 1) RHY_REFLUX(1, float);
