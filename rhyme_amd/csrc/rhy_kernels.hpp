@@ -704,124 +704,74 @@ __device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
   R nx[NV];
   load_state<ND, R>(cells, G.plane, o + 2 * s, nx);
   put(2, nx);
-  if constexpr (sizeof(R) == sizeof(float)) {
-    // The mixed form's walk is software-pipelined: the cell between faces
-    // m-2 and m-1 is finished at face m, between that face's half step and
-    // its Riemann solve.  Its loads (AccLoads) are issued at the top of the
-    // iteration and arrive during the half step; its upper face's flux waits
-    // in registers across the half step, its lower one in the lane's
-    // walk_fprev slot; nothing more than the plain walk holds is live across
-    // a Riemann solve.  (With FP64 values the held loads and flux spill: the
-    // FP64 walk finishes each cell right after its upper face, below.)
-    auto finish = [&](int j, const R* fu, const AccLoads<ND, R, MODE>& ld, bool deferred) {
-      const int64_t oc = o + (int64_t)j * s;
-      R un[NV];
+  // The walk is software-pipelined: the cell between faces m-2 and m-1 is
+  // finished at face m, between that face's half step and its Riemann
+  // solve.  Its loads (AccLoads) are issued at the top of the iteration and
+  // arrive during the half step; its upper face's flux waits in registers
+  // across the half step, its lower one in the lane's walk_fprev slot;
+  // nothing more than a plain walk holds is live across a Riemann solve.
+  auto finish = [&](int j, const R* fu, const AccLoads<ND, R, MODE>& ld, bool deferred) {
+    const int64_t oc = o + (int64_t)j * s;
+    R un[NV];
 #pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        const R fl = fprev[c * kWalkLanes + lane];
-        fprev[c * kWalkLanes + lane] = fu[c];
-        const R uc = ring[((j & 3) * NV + c) * kWalkLanes + lane];
-        un[c] = acc_one_loaded<ND, R, MODE>(out, oc + c * G.plane, uc, fl, fu[c], ld.a[c],
-                                            AccLoads<ND, R, MODE>::kFx ? ld.b[c] : R(0), dtdx0,
-                                            dtdxA, order);
-      }
-      if constexpr (MODE == kAccLast) {
-        int ci[3] = {idx[0], idx[1], idx[2]};
-        ci[A] = f0 + j;
-        write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
-        if (!deferred) {  // a deferred face: the fixup recomputes the speed
-          double ud[NV];
-#pragma unroll
-          for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
-          const double sp = cfl_speed<ND>(k, ud);
-          smax = sp > smax ? sp : smax;
-        }
-      }
-    };
-    R fh[NV];         // flux of face m - 1, held for the finish at face m
-    bool dh = false;  // face m - 1 deferred
-    bool dp = false;  // face m - 2 deferred (its flux in fprev)
-#pragma unroll 1
-    for (int m = 0; m < nf; ++m) {
-      AccLoads<ND, R, MODE> ld;
-      if (m >= 2) ld.load(G, fx, acc, o + (int64_t)(m - 1) * s);
-      if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
-      get(m, um);
-      get(m + 1, u0);
-      get(m + 2, up);
-      half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
-      if (m >= 2) {  // cell f0 + m - 1 (its ring slot is overwritten by the put)
-        finish(m - 1, fh, ld, dp || dh);
-        dp = dh;
-      }
-      if (m + 1 < nf) put(m + 3, nx);
-      R Fr[NV];
-      riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
-      const bool dm = defer_take<PW>(k, dl, 0u, A, o + (int64_t)m * s);
-#pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        rp[c] = rn[c];
-        if (m == 0)
-          fprev[c * kWalkLanes + lane] = Fr[c];
-        else
-          fh[c] = Fr[c];
-      }
-      if (m == 0)
-        dp = dm;
-      else
-        dh = dm;
+    for (int c = 0; c < NV; ++c) {
+      const R fl = fprev[c * kWalkLanes + lane];
+      fprev[c * kWalkLanes + lane] = fu[c];
+      const R uc = ring[((j & 3) * NV + c) * kWalkLanes + lane];
+      un[c] = acc_one_loaded<ND, R, MODE>(out, oc + c * G.plane, uc, fl, fu[c], ld.a[c],
+                                          AccLoads<ND, R, MODE>::kFx ? ld.b[c] : R(0), dtdx0,
+                                          dtdxA, order);
     }
-    {  // the last cell, f0 + nf - 1 (faces nf - 2 and nf - 1; nf >= 2)
-      AccLoads<ND, R, MODE> ld;
-      ld.load(G, fx, acc, o + (int64_t)(nf - 1) * s);
-      finish(nf - 1, fh, ld, dp || dh);
+    if constexpr (MODE == kAccLast) {
+      int ci[3] = {idx[0], idx[1], idx[2]};
+      ci[A] = f0 + j;
+      write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
+      if (!deferred) {  // a deferred face: the fixup recomputes the speed
+        double ud[NV];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
+        const double sp = cfl_speed<ND>(k, ud);
+        smax = sp > smax ? sp : smax;
+      }
     }
-    return smax;
-  }
-  bool dprev = false;
+  };
+  R fh[NV];         // flux of face m - 1, held for the finish at face m
+  bool dh = false;  // face m - 1 deferred
+  bool dp = false;  // face m - 2 deferred (its flux in fprev)
 #pragma unroll 1
   for (int m = 0; m < nf; ++m) {
+    AccLoads<ND, R, MODE> ld;
+    if (m >= 2) ld.load(G, fx, acc, o + (int64_t)(m - 1) * s);
     if (m + 1 < nf) load_state<ND, R>(cells, G.plane, o + (m + 3) * s, nx);
     get(m, um);
     get(m + 1, u0);
     get(m + 2, up);
     half_step<ND, A, LIM, R>(k, um, u0, up, hdtdx, lw, rn);
+    if (m >= 2) {  // cell f0 + m - 1 (its ring slot is overwritten by the put)
+      finish(m - 1, fh, ld, dp || dh);
+      dp = dh;
+    }
     if (m + 1 < nf) put(m + 3, nx);
     R Fr[NV];
     riemann_flux<ND, A, PW, R>(k, rp, lw, Fr);
-    const int64_t of = o + m * s;
-    const bool dm = defer_take<PW>(k, dl, 0u, A, of);
+    const bool dm = defer_take<PW>(k, dl, 0u, A, o + (int64_t)m * s);
 #pragma unroll
-    for (int c = 0; c < NV; ++c) rp[c] = rn[c];
-    if (m >= 1) {  // cell f0 + m: faces m - 1 (held) and m
-      // one component at a time: its lower flux out of the lane's slot, the
-      // upper one in, so Fr dies as the new values come
-      R un[NV];
-#pragma unroll
-      for (int c = 0; c < NV; ++c) {
-        const R fl = fprev[c * kWalkLanes + lane];
+    for (int c = 0; c < NV; ++c) {
+      rp[c] = rn[c];
+      if (m == 0)
         fprev[c * kWalkLanes + lane] = Fr[c];
-        const R uc = ring[((m & 3) * NV + c) * kWalkLanes + lane];
-        un[c] = acc_one<ND, A, R, MODE>(G, fx, acc, out, of + c * G.plane, uc, fl, Fr[c], dtdx0,
-                                        dtdxA, order);
-      }
-      if constexpr (MODE == kAccLast) {
-        int ci[3] = {idx[0], idx[1], idx[2]};
-        ci[A] = f0 + m;
-        write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
-        if (!(dm || dprev)) {
-          double ud[NV];
-#pragma unroll
-          for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
-          const double sp = cfl_speed<ND>(k, ud);
-          smax = sp > smax ? sp : smax;
-        }
-      }
-    } else {
-#pragma unroll
-      for (int c = 0; c < NV; ++c) fprev[c * kWalkLanes + lane] = Fr[c];
+      else
+        fh[c] = Fr[c];
     }
-    dprev = dm;
+    if (m == 0)
+      dp = dm;
+    else
+      dh = dm;
+  }
+  {  // the last cell, f0 + nf - 1 (faces nf - 2 and nf - 1; nf >= 2)
+    AccLoads<ND, R, MODE> ld;
+    ld.load(G, fx, acc, o + (int64_t)(nf - 1) * s);
+    finish(nf - 1, fh, ld, dp || dh);
   }
   return smax;
 }
