@@ -38,6 +38,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "zone-updates/s at 3D 256^3 (FP64 vs mixed), % HBM roofline; serve p99 latency"
 HBM_PEAK_GBS = 8000.0
+EVENT_EVERY = 5  # HIP events around the launches of every 5th timed step
 # Counter figures of the step (HBM bytes per zone-update from calibrated
 # FETCH_SIZE/WRITE_SIZE passes, VALU issue of the walk kernels from SQ
 # passes), written by `bench.py --pmc-summary` from rocprofv3 --pmc runs of
@@ -184,7 +185,7 @@ def blast_params(hydro, solver):
         cfl=hydro.Cfl(0.2), bc=hydro.SamrBc(types=[hydro.REFLECTIVE] * 6))
 
 
-def gpu_leg(n, steps, warmup, seed, precision=0):
+def gpu_leg(n, steps, warmup, seed, precision=0, event_every=EVENT_EVERY):
     import numpy as np
     from rhyme_amd import hydro, ics
 
@@ -196,8 +197,10 @@ def gpu_leg(n, steps, warmup, seed, precision=0):
     if warmup:
         hydro.run(params, box, dx, warmup)
     # timed region: K steps enqueued, bracketed by stream syncs; HIP events
-    # recorded around every launch on the box's stream
-    box.set_profiling(True)
+    # recorded on the box's stream around every launch of the steps 0, E,
+    # 2E, ... of the K (E = event_every; events around every launch of
+    # every step cost ~4% of the step)
+    box.set_profiling(event_every)
     box.sync()
     t0 = time.perf_counter()
     hydro.run_async(params, box, dx, steps)
@@ -437,6 +440,9 @@ def main():
                     help="first run two torch kernels of known traffic, a 1 GiB "
                          "fill (writes 1 GiB) and a 1 GiB scaled copy (reads 1 GiB, "
                          "writes 1 GiB), that scale rocprof FETCH/WRITE_SIZE")
+    ap.add_argument("--event-every", type=int, default=EVENT_EVERY,
+                    help="HIP events around the launches of every E-th timed step "
+                         "(1: every step)")
     ap.add_argument("--pmc-summary", metavar="DIR",
                     help="no run: summarise the rocprofv3 --pmc passes of this bench under DIR "
                          "(valu/, misc/, fetch/, write/ and their printed lines) into "
@@ -463,14 +469,14 @@ def main():
         torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
-    g = gpu_leg(a.n, a.steps, a.warmup, a.seed)
+    g = gpu_leg(a.n, a.steps, a.warmup, a.seed, event_every=a.event_every)
     zones = a.n ** 3
     zu = zones * a.steps / g["elapsed"]
     ms_step = 1e3 * g["elapsed"] / a.steps
     st = g["stats"]
     # kernels of one step: dt (4), deferred-face list reset and fixup (5),
     # flux x/y/z (0), update (1)
-    kern_ms = sum(st[k][1] for k in (0, 1, 4, 5)) / max(a.steps, 1)
+    kern_ms = sum(st[k][1] for k in (0, 1, 4, 5)) / max(st[4][0], 1)  # per timed step
     flux_ms = st[0][1] / max(st[0][0], 1)
     bytes_alg = 80.0 * zones
     achieved = bytes_alg / (kern_ms * 1e-3) / 1e9
@@ -489,6 +495,9 @@ def main():
                     kernel="one step: dt_kernel + deferred-list reset + x flux walk + its fixup "
                            "+ y and z accumulating walks + their fixups (achieved: over the "
                            "HIP-event sum of these kernels; achieved_wall: over ms_per_step)",
+                    events="HIP events on the box stream around every launch of timed steps "
+                           "0, %d, %d, ... (%d of the %d)" % (a.event_every, 2 * a.event_every,
+                                                              st[4][0], a.steps),
                     step_kernel_ms=kern_ms, walk_kernel_avg_ms=flux_ms,
                     bytes_per_zone_update=80)
     line = dict(metric=METRIC, value=zu, unit="zone-updates/s", n_gpus=1, steps=a.steps,
@@ -502,7 +511,7 @@ def main():
                 roofline=roofline, source_hash=source_hash())
     if not a.skip_mixed:
         import numpy as np
-        m = gpu_leg(a.n, a.steps, a.warmup, a.seed, precision=1)
+        m = gpu_leg(a.n, a.steps, a.warmup, a.seed, precision=1, event_every=a.event_every)
         zu_m = zones * a.steps / m["elapsed"]
         ref = g["state"]
         rel = [float(np.abs(m["state"][c] - ref[c]).sum() / np.abs(ref[c]).sum())
@@ -515,7 +524,7 @@ def main():
                         "path matches the FP64 oracle to <= 1e-12 (tests/test_gpu_parity.py)",
             walk_kernel_avg_ms=m["stats"][0][1] / max(m["stats"][0][0], 1),
             roofline_achieved_gbs=40.0 * zones / (sum(m["stats"][k][1] for k in (0, 1, 4, 5))
-                                                   / max(a.steps, 1) * 1e-3) / 1e9,
+                                                   / max(m["stats"][4][0], 1) * 1e-3) / 1e9,
             bytes_per_zone_update=40, state_finite=m["finite"],
             traffic=pmc["mixed"]["bytes_per_zone"] * zones if traffic else None,
             flux_kernel_valu_issue_frac=pmc["mixed"]["walk_valu_issue"] if pmc else None)

@@ -194,11 +194,14 @@ int rhy_run_async(rhy_box_t *box, const rhy_params_t *p, const double dx[3],
                   int32_t nsteps);
 int rhy_box_last_dts(rhy_box_t *box, int32_t n, double *dts);
 
-/* Per-kernel timing with HIP events recorded on the box's stream around every
- * launch while enabled.  rhy_box_kernel_stats returns, for kernel index
- * kidx (0: the flux kernels, 1: update, 2: bc, 3: cfl, 4: dt, 5: the
+/* Per-kernel timing with HIP events recorded on the box's stream.  on = 1:
+ * around every launch; on = N > 1: around every launch of the steps s of a
+ * rhy_run / rhy_run_async call with s % N == 0 (step 0, N, 2N, ...) and of
+ * nothing else, so that the events (a few microseconds each) perturb only
+ * the sampled steps; on = 0: off.  rhy_box_kernel_stats returns, for kernel
+ * index kidx (0: the flux kernels, 1: update, 2: bc, 3: cfl, 4: dt, 5: the
  * deferred-face list reset and fixup), the number of launches timed and their
- * summed milliseconds. */
+ * summed milliseconds.  RHY_ERR_ARG for on < 0. */
 int rhy_box_set_profiling(rhy_box_t *box, int32_t on);
 int rhy_box_kernel_stats(rhy_box_t *box, int32_t kidx, int64_t *launches,
                          double *total_ms);

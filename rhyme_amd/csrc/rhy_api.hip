@@ -99,7 +99,8 @@ struct rhy_box {
   float* snap = nullptr;          // staging of the FP32 snapshot payload (lazy)
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
-  bool profiling = false;
+  int32_t prof_every = 0;  // 0: no timing, 1: every launch, N: the steps s % N == 0
+  bool profiling = false;   // launches are being timed now
   std::vector<TimedLaunch> pending;
   int64_t launches[kNumKernelIds] = {0, 0, 0, 0, 0, 0};
   double total_ms[kNumKernelIds] = {0, 0, 0, 0, 0, 0};
@@ -524,6 +525,7 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
     if ((rc = launch_bc<ND>(b, p, -1, b->state[1 - b->cur]))) return rc;
   const double cmdx = p->courant * min_dx(ND, dx);
   for (int32_t s = 0; s < nsteps; ++s) {
+    if (b->prof_every > 1) b->profiling = s % b->prof_every == 0;
     // partials: the CFL pass's before the first step, the last walk's and
     // the fixup's slot after
     const int32_t np = ND > 1 && s > 0 ? b->acc_blocks + 1 : b->partial_n;
@@ -539,6 +541,7 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
       if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc))) return rc;
     }
   }
+  b->profiling = b->prof_every == 1;
   b->last_nsteps = nsteps;
   HIP_TRY(hipGetLastError());
   return RHY_OK;
@@ -1276,9 +1279,11 @@ int rhy_box_last_dts(rhy_box_t* b, int32_t n, double* dts) {
 int rhy_box_set_profiling(rhy_box_t* b, int32_t on) {
   int rc;
   if ((rc = check_box(b))) return rc;
+  if (on < 0) return fail(RHY_ERR_ARG, "profiling period must be >= 0");
   HIP_TRY(hipStreamSynchronize(b->stream));
   collect_timings(b);
-  b->profiling = on != 0;
+  b->prof_every = on;
+  b->profiling = on == 1;
   for (int i = 0; i < kNumKernelIds; ++i) {
     b->launches[i] = 0;
     b->total_ms[i] = 0.0;

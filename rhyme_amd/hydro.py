@@ -255,8 +255,9 @@ class SamrBox:
 
     # -- timing helpers used by bench.py
     def set_profiling(self, on):
-        check(_lib.load().rhy_box_set_profiling(self._h, 1 if on else 0),
-              "rhy_box_set_profiling")
+        """on: False/0 off, True/1 every launch, N > 1 every launch of the
+        steps s % N == 0 of each run (rhy_box_set_profiling)."""
+        check(_lib.load().rhy_box_set_profiling(self._h, int(on)), "rhy_box_set_profiling")
 
     def kernel_stats(self, kidx):
         n = C.c_int64(0)

@@ -131,6 +131,41 @@ def test_run_continues_across_calls():
     np.testing.assert_array_equal(interior(3, out), interior(3, one))
 
 
+def test_sampled_kernel_timing():
+    """rhy_box_set_profiling(N): events around the launches of steps 0, N,
+    2N, ... of each rhy_run call and nothing else; N = 1 every launch; the
+    state is the same whatever the timing."""
+    p = params(3, hydro.SUPERBEE, hydro.MEMORY_INTENSIVE, [1] * 6)
+    dims = [24, 20, 18]
+    cells = ics.blast_3d(dims=dims, seed=4)
+    dx = [1.0 / 24] * 3
+    ref, dref = run_loop(3, dims, cells, p, dx, 9, hydro.FP64)
+    box = hydro.SamrBox(3, dims, cells=cells.copy())
+    box.upload()
+    box.set_profiling(3)
+    d = [hydro.run(p, box, dx, 7)]           # steps 0, 3, 6 timed
+    say("sampled", [box.kernel_stats(k) for k in range(6)])
+    assert box.kernel_stats(4)[0] == 3       # one dt launch per step
+    assert box.kernel_stats(0)[0] == 9       # x walk, y walk, z walk
+    assert box.kernel_stats(3)[0] == 0       # the prologue's CFL pass: untimed
+    assert box.kernel_stats(2)[0] == 0
+    assert all(box.kernel_stats(k)[1] > 0.0 for k in (0, 4))
+    box.set_profiling(1)                     # every launch, prologue included
+    d.append(hydro.run(p, box, dx, 2))
+    say("every", [box.kernel_stats(k) for k in range(6)])
+    assert box.kernel_stats(4)[0] == 2
+    assert box.kernel_stats(0)[0] == 6
+    assert box.kernel_stats(3)[0] == 1
+    box.set_profiling(0)
+    assert box.kernel_stats(4)[0] == 0
+    with pytest.raises(_lib.RhyError):
+        box.set_profiling(-1)
+    out = box.download()
+    box.close()
+    np.testing.assert_array_equal(np.concatenate(d), dref)
+    np.testing.assert_array_equal(interior(3, out), interior(3, ref))
+
+
 def test_resize_then_step_matches_oracle():
     """rhy_box_resize (the device side of rhyme_mh_workspace_check's
     reallocation on a shape change, rhyme_mh_workspace_check.f90:51-59):
