@@ -518,15 +518,16 @@ __global__ __launch_bounds__(256) void amr_register_kernel(const AmrItem* __rest
 // <F_f> = register * 2^-ndim (the mean over 2 substeps and 2^(ndim-1) faces).
 template <int ND, typename R>
 __global__ __launch_bounds__(256) void amr_reflux_kernel(const AmrItem* __restrict__ items,
-                                                         int nitems, int64_t total,
+                                                         int nitems, int64_t total, int64_t base,
                                                          const double* __restrict__ reg,
                                                          Geom GC, R* __restrict__ coarse,
                                                          const R* __restrict__ cflux,
                                                          double dtdx) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= total) return;
-  const AmrItem& it = items[amr_item_of(items, nitems, t0)];
-  const int64_t t = t0 - it.first;
+  const int64_t tg = base + t0;  // the pass' items number threads from base
+  const AmrItem& it = items[amr_item_of(items, nitems, tg)];
+  const int64_t t = tg - it.first;
   int a, s, tr[3];
   reflux_face<ND>(it, t, a, s, tr);
   int out[3] = {1, 1, 1}, low[3] = {1, 1, 1};
@@ -1629,18 +1630,19 @@ int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
 
 // The register items of every fine box, grouped by (axis, side) pass in
 // pass order a = 0..ndim-1, low side then high side; pass_begin[p] and
-// pass_total[p] locate pass p's items and threads.  Returns the register
-// length in doubles.
+// pass_total[p] locate pass p's items and threads.  Items number their
+// threads across all passes (pass p's start after those of passes < p), so
+// one launch can take every pass.  Returns the register length in doubles.
 static int64_t reflux_items(rhy_batch_t* f, const int32_t* lo, int pass_begin[7],
                             int64_t pass_total[6]) {
   const int nd = f->ndim, nb = (int)f->slots.size();
   f->amr_host.clear();
-  int64_t reg = 0;
+  int64_t reg = 0, first = 0;
   for (int a = 0; a < nd; ++a)
     for (int s = 0; s < 2; ++s) {
       const int pass = 2 * a + s;
       pass_begin[pass] = (int)f->amr_host.size();
-      int64_t first = 0;
+      const int64_t pass_first = first;
       for (int i = 0; i < nb; ++i) {
         const BBox& B = f->slots[i].B;
         AmrItem it{};
@@ -1658,7 +1660,7 @@ static int64_t reflux_items(rhy_batch_t* f, const int32_t* lo, int pass_begin[7]
         first += m;
         reg += m * (nd + 2);
       }
-      pass_total[pass] = first;
+      pass_total[pass] = first - pass_first;
     }
   pass_begin[2 * nd] = (int)f->amr_host.size();
   return reg;
@@ -1725,28 +1727,28 @@ int rhy_amr_flux_register(rhy_batch_t* f, const int32_t* lo, int32_t reset) {
   if ((rc = reflux_prepare(f, lo, &len))) return rc;
   if (!reset && f->reg_key != before)
     return bfail(RHY_ERR_ARG, "flux register not started (reset = 1) for these boxes");
-  // enqueued on the batch's stream behind the substep; no wait, so the
-  // coarse step on the box's stream overlaps the fine substep
+  // one launch over the items of every pass (each thread writes its own
+  // register entry), enqueued on the batch's stream behind the substep; no
+  // wait, so the coarse step on the box's stream overlaps the fine substep
   const int nd = f->ndim;
-  for (int pass = 0; pass < 2 * nd; ++pass) {
-    const int ni = f->reg_pb[pass + 1] - f->reg_pb[pass];
-    if (!f->reg_pt[pass] || !ni) continue;
-    const AmrItem* items = f->reg_items + f->reg_pb[pass];
-#define RHY_REG(ND, R)                                                                          \
-  hipLaunchKernelGGL((amr_register_kernel<ND, R>), dim3(cdiv(f->reg_pt[pass], 256)), dim3(256), \
-                     0, f->stream, items, ni, f->reg_pt[pass], static_cast<const R*>(f->arena), \
+  const int ni = f->reg_pb[2 * nd];
+  int64_t total = 0;
+  for (int pass = 0; pass < 2 * nd; ++pass) total += f->reg_pt[pass];
+  if (!total || !ni) return RHY_OK;
+#define RHY_REG(ND, R)                                                                         \
+  hipLaunchKernelGGL((amr_register_kernel<ND, R>), dim3(cdiv(total, 256)), dim3(256), 0,      \
+                     f->stream, f->reg_items, ni, total, static_cast<const R*>(f->arena),     \
                      f->amr_reg, reset ? 1 : 0)
-    if (f->precision == RHY_MIXED) {
-      if (nd == 1) RHY_REG(1, float);
-      else if (nd == 2) RHY_REG(2, float);
-      else RHY_REG(3, float);
-    } else {
-      if (nd == 1) RHY_REG(1, double);
-      else if (nd == 2) RHY_REG(2, double);
-      else RHY_REG(3, double);
-    }
-#undef RHY_REG
+  if (f->precision == RHY_MIXED) {
+    if (nd == 1) RHY_REG(1, float);
+    else if (nd == 2) RHY_REG(2, float);
+    else RHY_REG(3, float);
+  } else {
+    if (nd == 1) RHY_REG(1, double);
+    else if (nd == 2) RHY_REG(2, double);
+    else RHY_REG(3, double);
   }
+#undef RHY_REG
   BHIP_TRY(hipGetLastError());
   return RHY_OK;
 }
@@ -1764,15 +1766,18 @@ int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double 
   if ((rc = reflux_prepare(f, lo, &len))) return rc;
   if (f->reg_key != before || !f->amr_reg)
     return bfail(RHY_ERR_ARG, "flux register not filled for these boxes");
+  // one launch per (axis, side) pass: faces of two passes can share their
+  // outside coarse cell (a corner between two fine boxes)
   const int nd = f->ndim;
-  for (int pass = 0; pass < 2 * nd; ++pass) {
+  int64_t base = 0;
+  for (int pass = 0; pass < 2 * nd; base += f->reg_pt[pass], ++pass) {
     const int ni = f->reg_pb[pass + 1] - f->reg_pb[pass];
     if (!f->reg_pt[pass] || !ni) continue;
     const AmrItem* items = f->reg_items + f->reg_pb[pass];
     const double dtdx = dt / dx[pass / 2];
 #define RHY_REFLUX(ND, R)                                                                       \
   hipLaunchKernelGGL((amr_reflux_kernel<ND, R>), dim3(cdiv(f->reg_pt[pass], 256)), dim3(256), 0, \
-                     f->stream, items, ni, f->reg_pt[pass], f->amr_reg, vc.geom,               \
+                     f->stream, items, ni, f->reg_pt[pass], base, f->amr_reg, vc.geom,         \
                      static_cast<R*>(vc.cells), static_cast<const R*>(vc.flux), dtdx)
     if (f->precision == RHY_MIXED) {
       if (nd == 1) RHY_REFLUX(1, float);
