@@ -423,7 +423,7 @@ DeferList defer_list_axis(rhy_box* b, int a) {
 // One accumulating walk (and its fixup) along A of an ND box: reads the
 // current state, writes Q (kAccMid) or the other state buffer (kAccLast).
 template <int ND, int A, int LIM, typename R, class PW, class P, int MODE>
-void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx,
                      int order) {
   const Geom& G = b->geom;
   const int nv = ND + 2;
@@ -439,13 +439,13 @@ void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx, const BcType
   timed_begin(b, 0, tl);
   hipLaunchKernelGGL((acc_walk_kernel<ND, A, LIM, R, PW, MODE>),
                      dim3(grid_for(acc_threads(G, A, kAccL))), dim3(kThreads), 0, b->stream, k, G,
-                     cur, fx, acc, out, bc, step_consts(b), kAccL, order, dl, b->partial);
+                     cur, fx, acc, out, step_consts(b), kAccL, order, dl, b->partial);
   timed_end(b, tl);
   if constexpr (PW::kDefer) {
     TimedLaunch tf;
     timed_begin(b, 5, tf);
     hipLaunchKernelGGL((acc_fixup_kernel<ND, A, R, P, MODE>), dim3(kFixupBlocks), dim3(kThreads),
-                       0, b->stream, k, G, cur, fx, acc, out, bc, b->dt, dx[0], dx[A], order, dl,
+                       0, b->stream, k, G, cur, fx, acc, out, b->dt, dx[0], dx[A], order, dl,
                        slot);
     timed_end(b, tf);
   }
@@ -455,7 +455,7 @@ void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx, const BcType
 // fluxes into the first flux planes (and their fixup), then the
 // accumulating walks; the new state lands in the other state buffer.
 template <int ND, int LIM, typename R, class PW, class P>
-void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx,
                          int order) {
   static_assert(ND > 1, "1D boxes step through the flux planes");
   TimedLaunch tr;
@@ -474,33 +474,33 @@ void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx, const Bc
     timed_end(b, tf);
   }
   if constexpr (ND == 3) {
-    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccMid>(b, k, dx, bc, order);
-    launch_acc_axis<ND, 2, LIM, R, PW, P, kAccLast>(b, k, dx, bc, order);
+    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccMid>(b, k, dx, order);
+    launch_acc_axis<ND, 2, LIM, R, PW, P, kAccLast>(b, k, dx, order);
   } else {
-    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccLast>(b, k, dx, bc, order);
+    launch_acc_axis<ND, 1, LIM, R, PW, P, kAccLast>(b, k, dx, order);
   }
   b->cur = 1 - b->cur;
   b->cells = b->state[b->cur];
 }
 
 template <int ND, typename R, class PW, class P>
-void launch_acc_step_r(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+void launch_acc_step_r(rhy_box* b, const Consts& k, const double* dx,
                        int order) {
   switch (k.limiter) {
-    case 1: launch_acc_step_lim<ND, 1, R, PW, P>(b, k, dx, bc, order); break;
-    case 2: launch_acc_step_lim<ND, 2, R, PW, P>(b, k, dx, bc, order); break;
-    case 3: launch_acc_step_lim<ND, 3, R, PW, P>(b, k, dx, bc, order); break;
-    default: launch_acc_step_lim<ND, 4, R, PW, P>(b, k, dx, bc, order); break;
+    case 1: launch_acc_step_lim<ND, 1, R, PW, P>(b, k, dx, order); break;
+    case 2: launch_acc_step_lim<ND, 2, R, PW, P>(b, k, dx, order); break;
+    case 3: launch_acc_step_lim<ND, 3, R, PW, P>(b, k, dx, order); break;
+    default: launch_acc_step_lim<ND, 4, R, PW, P>(b, k, dx, order); break;
   }
 }
 
 template <int ND>
-int launch_acc_step(rhy_box* b, const Consts& k, const double* dx, const BcTypes& bc,
+int launch_acc_step(rhy_box* b, const Consts& k, const double* dx,
                     int order) {
   if (b->precision == RHY_MIXED)
-    launch_acc_step_r<ND, float, MixedFlux, MixedMath>(b, k, dx, bc, order);
+    launch_acc_step_r<ND, float, MixedFlux, MixedMath>(b, k, dx, order);
   else
-    launch_acc_step_r<ND, double, PowF64Flux, PowF64Fix>(b, k, dx, bc, order);
+    launch_acc_step_r<ND, double, PowF64Flux, PowF64Fix>(b, k, dx, order);
   HIP_TRY(hipGetLastError());
   return RHY_OK;
 }
@@ -508,10 +508,10 @@ int launch_acc_step(rhy_box* b, const Consts& k, const double* dx, const BcTypes
 // The main loop's steps (src/main.f90:109-136) on the device.  1D boxes:
 // fluxes into planes, then the update with the fused epilogue (ghost
 // images and CFL partials of the new state).  2D/3D boxes: the
-// accumulating walks, whose last one writes the new state with the same
-// epilogue; the state buffers swap every step, so the prologue fills the
-// ghosts of both (the inflow faces, which the epilogue leaves alone, then
-// hold their values in either buffer).
+// accumulating walks, whose last one writes the new state's interior and
+// CFL partials into the other state buffer (the buffers swap every step),
+// then the ghost fill of the new state.  Writing the ghost images from the
+// last walk instead costs that walk more than the separate fill takes.
 template <int ND>
 int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nsteps) {
   const Consts k = make_consts(p);
@@ -521,8 +521,6 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
   // prologue: CFL of the current state and the ghost fill it is stepped with
   if ((rc = launch_cfl<ND>(b, k))) return rc;
   if ((rc = launch_bc<ND>(b, p, -1))) return rc;
-  if constexpr (ND > 1)
-    if ((rc = launch_bc<ND>(b, p, -1, b->state[1 - b->cur]))) return rc;
   const double cmdx = p->courant * min_dx(ND, dx);
   for (int32_t s = 0; s < nsteps; ++s) {
     if (b->prof_every > 1) b->profiling = s % b->prof_every == 0;
@@ -535,7 +533,8 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
                        b->dt_log + s, (double*)nullptr, dx[0], dx[1], dx[2], step_consts(b));
     timed_end(b, tl);
     if constexpr (ND > 1) {
-      if ((rc = launch_acc_step<ND>(b, k, dx, bc, p->solver_type))) return rc;
+      if ((rc = launch_acc_step<ND>(b, k, dx, p->solver_type))) return rc;
+      if ((rc = launch_bc<ND>(b, p, -1))) return rc;  // the new state's ghosts
     } else {
       if ((rc = launch_flux<ND>(b, k, dx))) return rc;
       if ((rc = launch_update<ND>(b, k, dx, p->solver_type, true, bc))) return rc;

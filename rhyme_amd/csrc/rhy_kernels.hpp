@@ -561,8 +561,9 @@ __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void flux_kernel(
 //            planes, B = dt/dx_y (F_y[c-1] - F_y[c]) from the walk;
 //   kAccLast (z of a 3D box, y of a 2D box): the new state u + (Q + C) or
 //            Q + C (3D), u + (A + B) or (u + A) + B (2D), written to the
-//            other state buffer with its ghost images for the next step's
-//            fill, and its CFL speed folded into the lane's maximum.
+//            other state buffer (interior only: a ghost fill of the new
+//            state follows the step), and its CFL speed folded into the
+//            lane's maximum.
 // These are the update's expressions operation for operation
 // (rhyme_muscl_hancock_solve_memory_intensive.f90:142-176,
 // …_solve_cpu_intensive.f90:117-136), so the state equals the flux-plane
@@ -651,7 +652,6 @@ __device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
                                                 const R* __restrict__ cells,
                                                 const R* __restrict__ fx,
                                                 const R* __restrict__ acc, R* __restrict__ out,
-                                                const BcTypes& bc,
                                                 const double* __restrict__ sc, int L,
                                                 int order, int64_t t, const DeferList& dl) {
   static_assert(A > 0 && A < ND, "the accumulating walks run along y or z");
@@ -723,9 +723,6 @@ __device__ __forceinline__ double flux_walk_acc(const Consts& k, const Geom& G,
                                           dtdxA, order);
     }
     if constexpr (MODE == kAccLast) {
-      int ci[3] = {idx[0], idx[1], idx[2]};
-      ci[A] = f0 + j;
-      write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
       if (!deferred) {  // a deferred face: the fixup recomputes the speed
         double ud[NV];
 #pragma unroll
@@ -787,10 +784,10 @@ __host__ __device__ inline int64_t acc_threads(const Geom& G, int A, int L) {
 template <int ND, int A, int LIM, typename R, class PW, int MODE>
 __global__ __launch_bounds__(256) RHY_FLUX_OCCUPANCY void acc_walk_kernel(
     Consts k, Geom G, const R* __restrict__ cells, const R* __restrict__ fx,
-    const R* __restrict__ acc, R* __restrict__ out, BcTypes bc, const double* __restrict__ sc,
+    const R* __restrict__ acc, R* __restrict__ out, const double* __restrict__ sc,
     int L, int order, DeferList dl, double* __restrict__ partial) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double sp = flux_walk_acc<ND, A, LIM, R, PW, MODE>(k, G, cells, fx, acc, out, bc, sc, L,
+  const double sp = flux_walk_acc<ND, A, LIM, R, PW, MODE>(k, G, cells, fx, acc, out, sc, L,
                                                           order, t, dl);
   if constexpr (MODE == kAccLast) block_max_to_partial(sp, partial);
 }
@@ -893,7 +890,7 @@ __global__ __launch_bounds__(256) void flux_fixup_kernel(Consts k, Geom G,
 // The cells an accumulating walk along A left beside deferred faces,
 // recomputed with the full policy PW: both faces of each such cell from
 // scratch, then the walk's own accumulation (acc_cell); kAccLast also
-// writes the ghost images and folds the CFL speed into *slot (an extra
+// folds the CFL speed into *slot (an extra
 // partial slot the dt reduction reads; speeds are >= 0, so their bit
 // patterns order as integers).  If the list overflowed, every cell of the
 // box.  Cells listed twice (two deferred faces) are written twice with the
@@ -902,7 +899,7 @@ template <int ND, int A, typename R, class PW, int MODE>
 __device__ __forceinline__ void acc_fix_cell(const Consts& k, const Geom& G,
                                              const R* __restrict__ cells,
                                              const R* __restrict__ fx, const R* __restrict__ acc,
-                                             R* __restrict__ out, const BcTypes& bc, double dt,
+                                             R* __restrict__ out, double dt,
                                              double dx0, double dxA, int order, int64_t oc,
                                              double* slot) {
   constexpr int NV = ND + 2;
@@ -921,7 +918,6 @@ __device__ __forceinline__ void acc_fix_cell(const Consts& k, const Geom& G,
   acc_cell<ND, A, R, MODE>(G, fx, acc, out, oc, uc, fl, fu, (R)(dt / dx0), (R)(dt / dxA),
                            order, un);
   if constexpr (MODE == kAccLast) {
-    write_ghost_images<ND, R>(G, bc, out, ci[0], ci[1], ci[2], un);
     double ud[NV];
 #pragma unroll
     for (int c = 0; c < NV; ++c) ud[c] = (double)un[c];
@@ -935,7 +931,7 @@ __device__ __forceinline__ void acc_fix_cell(const Consts& k, const Geom& G,
 template <int ND, int A, typename R, class PW, int MODE>
 __global__ __launch_bounds__(256) void acc_fixup_kernel(
     Consts k, Geom G, const R* __restrict__ cells, const R* __restrict__ fx,
-    const R* __restrict__ acc, R* __restrict__ out, BcTypes bc, const double* __restrict__ dtp,
+    const R* __restrict__ acc, R* __restrict__ out, const double* __restrict__ dtp,
     double dx0, double dxA, int order, DeferList dl, double* slot) {
   const uint32_t n = *dl.count;
   if (n == 0) return;
@@ -945,9 +941,9 @@ __global__ __launch_bounds__(256) void acc_fixup_kernel(
   if (n <= dl.cap) {
     for (int64_t i = t0; i < (int64_t)n; i += stride) {
       const int64_t of = (int64_t)(dl.items[i] & ((1ull << 40) - 1));
-      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order, of,
+      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, dt, dx0, dxA, order, of,
                                        slot);
-      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order,
+      acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, dt, dx0, dxA, order,
                                        of + G.s[A], slot);
     }
     return;
@@ -956,7 +952,7 @@ __global__ __launch_bounds__(256) void acc_fixup_kernel(
   for (int64_t q = t0; q < total; q += stride) {
     const int i = (int)(q % G.n[0]) + 1;
     const int64_t r = q / G.n[0];
-    acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, bc, dt, dx0, dxA, order,
+    acc_fix_cell<ND, A, R, PW, MODE>(k, G, cells, fx, acc, out, dt, dx0, dxA, order,
                                      goff(G, i, (int)(r % G.n[1]) + 1, (int)(r / G.n[1]) + 1),
                                      slot);
   }

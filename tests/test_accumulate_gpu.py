@@ -147,14 +147,15 @@ def test_sampled_kernel_timing():
     say("sampled", [box.kernel_stats(k) for k in range(6)])
     assert box.kernel_stats(4)[0] == 3       # one dt launch per step
     assert box.kernel_stats(0)[0] == 9       # x walk, y walk, z walk
+    assert box.kernel_stats(2)[0] == 3       # the new state's ghost fill
     assert box.kernel_stats(3)[0] == 0       # the prologue's CFL pass: untimed
-    assert box.kernel_stats(2)[0] == 0
     assert all(box.kernel_stats(k)[1] > 0.0 for k in (0, 4))
     box.set_profiling(1)                     # every launch, prologue included
     d.append(hydro.run(p, box, dx, 2))
     say("every", [box.kernel_stats(k) for k in range(6)])
     assert box.kernel_stats(4)[0] == 2
     assert box.kernel_stats(0)[0] == 6
+    assert box.kernel_stats(2)[0] == 3       # the prologue's fill and 2 steps
     assert box.kernel_stats(3)[0] == 1
     box.set_profiling(0)
     assert box.kernel_stats(4)[0] == 0
