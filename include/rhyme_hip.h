@@ -468,14 +468,25 @@ int rhy_batch_last_deferred(rhy_batch_t *batch, int64_t *n);
  *   the batch's flux register (reset = 1: after the first substep of a
  *   coarse step) or added to it (reset = 0).
  * rhy_amr_reflux: after the coarse step (rhy_mh_solve, whose flux planes
- *   hold the coarse fluxes) and both substeps, every coarse cell next to a
- *   fine box takes the correction of its face flux, F_c replaced by the
- *   fine fluxes' mean over the two substeps and the 2^(ndim-1) fine faces:
- *   U += dt/dx_a (F_c - <F_f>) on the low side of a box, dt/dx_a (<F_f> -
- *   F_c) on the high side, applied axis by axis, low side then high side.
- *   Cells under another fine box get it too and are then restricted over.
+ *   hold the coarse fluxes, or rhy_amr_coarse_step) and both substeps,
+ *   every coarse cell next to a fine box takes the correction of its face
+ *   flux, F_c replaced by the fine fluxes' mean over the two substeps and
+ *   the 2^(ndim-1) fine faces: U += dt/dx_a (F_c - <F_f>) on the low side
+ *   of a box, dt/dx_a (<F_f> - F_c) on the high side, applied axis by axis,
+ *   low side then high side.  Cells under another fine box get it too and
+ *   are then restricted over.
+ * rhy_amr_coarse_step: no counterpart in the reference (it has no coupled
+ *   levels); for callers of the hierarchy.  The coarse step as the ghost
+ *   fill and one step with the given dt (rhy_bc_set_boundaries +
+ *   rhy_mh_solve, bit for bit) through the accumulating walks of rhy_run
+ *   instead of the flux planes: only the x fluxes stay in the flux planes,
+ *   and the box keeps the state it stepped from in its second state buffer,
+ *   from which rhy_amr_reflux forms the y/z coarse fluxes it needs (as the
+ *   walks formed them, bit for bit).  Any later call that changes the box
+ *   (upload, step, paint, draw, resize, unpack) ends that; the box's ghost
+ *   faces are not filled after the step.  1D boxes take rhy_mh_solve.
  * The prolong, restrict, register and reflux calls wait for both sides'
- * streams. */
+ * streams; rhy_amr_coarse_step returns when its step is done. */
 int rhy_batch_cfl_dts(rhy_batch_t *batch, const rhy_params_t *p, double *dts);
 int rhy_batch_run_dt_async(rhy_batch_t *batch, const rhy_params_t *p, double dt,
                            int32_t nsteps);
@@ -484,6 +495,7 @@ int rhy_amr_restrict(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo);
 int rhy_amr_flux_register(rhy_batch_t *fine, const int32_t *lo, int32_t reset);
 int rhy_amr_reflux(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, double dt,
                    const double dx[3]);
+int rhy_amr_coarse_step(rhy_box_t *coarse, const rhy_params_t *p, const double dx[3], double dt);
 
 /* Snapshot payload of a whole level (a batch or pool): the active boxes in
  * slot order, each laid out as rhy_box_snapshot_pack lays out one box; the

@@ -6,7 +6,10 @@ coarse box and a fixed batch of fine boxes are coupled as follows, per
 coarse step:
 
 1. dt0 = min(coarse CFL dt, 2 * min fine CFL dt), each with its level's dx.
-2. The coarse box gets its ghost fill and one MUSCL-Hancock step with dt0.
+2. The coarse box gets its ghost fill and one MUSCL-Hancock step with dt0
+   (rhy_amr_coarse_step: the accumulating walks, bit for bit the flux-plane
+   step rhy_mh_solve; the reflux forms the coarse y/z interface fluxes
+   from the pre-step state the walks leave behind).
 3. Two fine substeps of dt0 / 2, each after the fine boxes' face ghosts are
    set from their coarse parents, piecewise constant in space: U_old before
    the first, 0.5 * U_old + 0.5 * U_new before the second; ghosts inside
@@ -60,9 +63,12 @@ class AmrHierarchy:
     holds fine cells 1 and 2 along each axis."""
 
     def __init__(self, ndim, base_dims, base_dx, fine_lo, fine_dims, precision=hydro.FP64,
-                 ncmp=None, refluxing=True):
+                 ncmp=None, refluxing=True, coarse_walks=True):
         self.ndim = ndim
         self.refluxing = refluxing
+        # the coarse step through rhy_amr_coarse_step (accumulating walks), or
+        # (False) through rhyme_samr_bc_set_boundaries + rhyme_muscl_hancock_solve
+        self.coarse_walks = coarse_walks
         self.base_dims = list(base_dims)[:ndim]
         self.dx0 = [float(x) for x in list(base_dx)[:ndim]]
         self.dx1 = [x / 2.0 for x in self.dx0]
@@ -143,8 +149,13 @@ class AmrHierarchy:
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)      # overlaps the coarse step
         if self.refluxing:
             self._register(True)                      # the batch's stream, after substep 1
-        hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
-        hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
+        if self.coarse_walks:
+            dx = (C.c_double * 3)(*(self.dx0 + [1.0] * (3 - self.ndim)))
+            _check(_L().rhy_amr_coarse_step(self.coarse.handle, C.byref(params), dx, dt0),
+                   "rhy_amr_coarse_step")
+        else:
+            hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
+            hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
         self._prolong(1)                              # after substep 1 (stream order)
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)
         if self.refluxing:

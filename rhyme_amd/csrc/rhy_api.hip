@@ -99,6 +99,9 @@ struct rhy_box {
   float* snap = nullptr;          // staging of the FP32 snapshot payload (lazy)
   int32_t dt_log_cap = 0;
   int32_t last_nsteps = 0;
+  bool coarse_walked = false;  // last step: rhy_amr_coarse_step (pre-step state kept)
+  Consts coarse_k{};           // its constants, dt and dx
+  double coarse_dt = 0.0, coarse_dx[3] = {1.0, 1.0, 1.0};
   int32_t prof_every = 0;  // 0: no timing, 1: every launch, N: the steps s % N == 0
   bool profiling = false;   // launches are being timed now
   std::vector<TimedLaunch> pending;
@@ -514,6 +517,7 @@ int launch_acc_step(rhy_box* b, const Consts& k, const double* dx,
 // last walk instead costs that walk more than the separate fill takes.
 template <int ND>
 int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nsteps) {
+  b->coarse_walked = false;
   const Consts k = make_consts(p);
   const BcTypes bc = make_bc(p);
   int rc;
@@ -723,6 +727,7 @@ int rhy_box_create(int32_t ndim, const int32_t dims[3], int32_t ncmp, int32_t pr
 int rhy_box_resize(rhy_box_t* b, const int32_t dims[3]) {
   int rc;
   if ((rc = check_box(b))) return rc;
+  b->coarse_walked = false;
   if ((rc = check_dims(b->ndim, dims))) return rc;
   bool same = true;
   for (int d = 0; d < 3; ++d) same = same && dims[d] == b->dims[d];
@@ -824,6 +829,7 @@ extern "C" {
 int rhy_box_upload(rhy_box_t* b, const double* cells) {
   int rc;
   if ((rc = check_box(b))) return rc;
+  b->coarse_walked = false;
   if (!cells) return fail(RHY_ERR_ARG, "cells is NULL");
   return box_copy(b, cells, nullptr);
 }
@@ -920,7 +926,9 @@ int rhy_box_snapshot_pack(rhy_box_t* b, float* data) {
 
 int rhy_box_snapshot_unpack(rhy_box_t* b, const float* data) {
   if (!data) return fail(RHY_ERR_ARG, "data is NULL");
-  return box_snapshot(b, nullptr, data);
+  const int rc = box_snapshot(b, nullptr, data);
+  if (rc == RHY_OK) b->coarse_walked = false;
+  return rc;
 }
 
 constexpr int kDiagBlocks = 256;
@@ -962,6 +970,7 @@ int rhy_box_diagnostics(rhy_box_t* b, rhy_diag_t* out) {
 int rhy_box_paint(rhy_box_t* b, const rhy_ic_t* ic) {
   int rc;
   if ((rc = check_box(b))) return rc;
+  b->coarse_walked = false;
   if (!ic || (ic->kind != RHY_IC_TWO_STATE && ic->kind != RHY_IC_BLAST))
     return fail(RHY_ERR_ARG, "unknown initial condition kind");
   if (ic->kind == RHY_IC_TWO_STATE && (ic->axis < 0 || ic->axis >= b->ndim))
@@ -996,6 +1005,7 @@ int rhy_box_paint(rhy_box_t* b, const rhy_ic_t* ic) {
 int rhy_box_draw(rhy_box_t* b, const rhy_params_t* p, const rhy_drawing_t* d) {
   int rc;
   if ((rc = check_box(b))) return rc;
+  b->coarse_walked = false;
   if (!p || !d) return fail(RHY_ERR_ARG, "params or drawing is NULL");
   if (b->ncmp < b->ndim + 2) return fail(RHY_ERR_ARG, "box has too few components");
   if (d->nshapes < 0 || d->nshapes > RHY_DRAW_MAX_SHAPES || (d->nshapes && !d->shapes))
@@ -1136,6 +1146,7 @@ namespace {
 // The launches of one MUSCL-Hancock solve of a box, enqueued on its stream.
 int mh_solve_enqueue(rhy_box* b, const rhy_params_t* p, const double* dx, double dt) {
   int rc;
+  b->coarse_walked = false;
   if ((rc = check_params(p, b->ndim))) return rc;
   if (!dx) return fail(RHY_ERR_ARG, "dx is NULL");
   if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
@@ -1164,6 +1175,34 @@ int rhy_mh_solve(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double
   int rc;
   if ((rc = check_box(b))) return rc;
   if ((rc = mh_solve_enqueue(b, p, dx, dt))) return rc;
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  return RHY_OK;
+}
+
+int rhy_amr_coarse_step(rhy_box_t* b, const rhy_params_t* p, const double dx[3], double dt) {
+  int rc;
+  if ((rc = check_box(b))) return rc;
+  if (b->ndim == 1) {  // 1D boxes step through the flux planes anyway
+    if ((rc = launch_bc<1>(b, p, -1))) return rc;
+    if ((rc = mh_solve_enqueue(b, p, dx, dt))) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return RHY_OK;
+  }
+  if ((rc = check_params(p, b->ndim))) return rc;
+  if (!dx) return fail(RHY_ERR_ARG, "dx is NULL");
+  if (p->solver_type != RHY_CPU_INTENSIVE && p->solver_type != RHY_MEMORY_INTENSIVE)
+    return fail(RHY_ERR_SOLVER, "Unknown solver type");
+  const Consts k = make_consts(p);
+  if ((rc = b->ndim == 2 ? launch_bc<2>(b, p, -1) : launch_bc<3>(b, p, -1))) return rc;
+  hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(64), 0, b->stream, b->dt, step_consts(b),
+                     dt, dx[0], b->ndim > 1 ? dx[1] : 1.0, b->ndim > 2 ? dx[2] : 1.0);
+  rc = b->ndim == 2 ? launch_acc_step<2>(b, k, dx, p->solver_type)
+                    : launch_acc_step<3>(b, k, dx, p->solver_type);
+  if (rc) return rc;
+  b->coarse_walked = true;  // after launch_acc_step: the state it stepped from is
+  b->coarse_k = k;          // state[1 - cur], its x fluxes in the first flux planes
+  b->coarse_dt = dt;
+  for (int d = 0; d < 3; ++d) b->coarse_dx[d] = d < b->ndim ? dx[d] : 1.0;
   HIP_TRY(hipStreamSynchronize(b->stream));
   return RHY_OK;
 }
@@ -1639,6 +1678,10 @@ int box_view(rhy_box_t* b, BoxView* v) {
   HIP_TRY(hipStreamSynchronize(b->stream));
   v->cells = b->cells;
   v->flux = b->flux;
+  v->prev = b->coarse_walked ? b->state[1 - b->cur] : nullptr;
+  v->k = b->coarse_k;
+  v->step_dt = b->coarse_dt;
+  for (int d = 0; d < 3; ++d) v->step_dx[d] = b->coarse_dx[d];
   v->geom = b->geom;
   v->ndim = b->ndim;
   v->ncmp = b->ncmp;

@@ -516,13 +516,18 @@ __global__ __launch_bounds__(256) void amr_register_kernel(const AmrItem* __rest
 // Reflux of one (axis, side) pass: the coarse cell outside each listed face
 // takes dtdx (F_c - <F_f>) (low side) or dtdx (<F_f> - F_c) (high side),
 // <F_f> = register * 2^-ndim (the mean over 2 substeps and 2^(ndim-1) faces).
-template <int ND, typename R>
+// F_c from the coarse flux planes (FROM_STATE = 0), or, for a pass along
+// axis FROM_STATE > 0 after rhy_amr_coarse_step, formed from the pre-step
+// state `prev` with policy P, the step's constants k, dt and dx.
+template <int ND, typename R, int FROM_STATE = 0, class P = PowF64Fix>
 __global__ __launch_bounds__(256) void amr_reflux_kernel(const AmrItem* __restrict__ items,
                                                          int nitems, int64_t total, int64_t base,
                                                          const double* __restrict__ reg,
                                                          Geom GC, R* __restrict__ coarse,
                                                          const R* __restrict__ cflux,
-                                                         double dtdx) {
+                                                         double dtdx, Consts k = Consts{},
+                                                         const R* __restrict__ prev = nullptr,
+                                                         double dt = 0.0, double dx = 1.0) {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= total) return;
   const int64_t tg = base + t0;  // the pass' items number threads from base
@@ -542,11 +547,21 @@ __global__ __launch_bounds__(256) void amr_reflux_kernel(const AmrItem* __restri
   const double w = ND == 3 ? 0.125 : ND == 2 ? 0.25 : 0.5;
   const double* r = reg + it.dst_off + t * (ND + 2);
   const int64_t oo = goff(GC, out[0], out[1], out[2]);
-  const int64_t of = goff(GC, low[0], low[1], low[2]) + (int64_t)a * (ND + 2) * GC.plane;
+  const int64_t ol = goff(GC, low[0], low[1], low[2]);
+  R Fs[ND + 2];
+  if constexpr (FROM_STATE > 0) {
+    // the coarse face's flux formed again from the state the coarse step
+    // started from, as the step's walk formed it
+    face_flux<ND, FROM_STATE, R, P>(k, GC, prev, dt, dx, ol, Fs);
+  }
 #pragma unroll
   for (int c = 0; c < ND + 2; ++c) {
     const double favg = r[c] * w;
-    const double fc = (double)cflux[of + c * GC.plane];
+    double fc;
+    if constexpr (FROM_STATE > 0)
+      fc = (double)Fs[c];
+    else
+      fc = (double)cflux[ol + (int64_t)a * (ND + 2) * GC.plane + c * GC.plane];
     const double d = s ? dtdx * (favg - fc) : dtdx * (fc - favg);
     R* u = coarse + oo + c * GC.plane;
     *u = (R)((double)*u + d);
@@ -1775,11 +1790,27 @@ int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double 
     if (!f->reg_pt[pass] || !ni) continue;
     const AmrItem* items = f->reg_items + f->reg_pb[pass];
     const double dtdx = dt / dx[pass / 2];
+    const int a = pass / 2;
 #define RHY_REFLUX(ND, R)                                                                       \
   hipLaunchKernelGGL((amr_reflux_kernel<ND, R>), dim3(cdiv(f->reg_pt[pass], 256)), dim3(256), 0, \
                      f->stream, items, ni, f->reg_pt[pass], base, f->amr_reg, vc.geom,         \
                      static_cast<R*>(vc.cells), static_cast<const R*>(vc.flux), dtdx)
-    if (f->precision == RHY_MIXED) {
+#define RHY_REFLUX_S(ND, A, R, P)                                                               \
+  hipLaunchKernelGGL((amr_reflux_kernel<ND, R, A, P>), dim3(cdiv(f->reg_pt[pass], 256)),       \
+                     dim3(256), 0, f->stream, items, ni, f->reg_pt[pass], base, f->amr_reg,    \
+                     vc.geom, static_cast<R*>(vc.cells), static_cast<const R*>(vc.flux), dtdx, \
+                     vc.k, static_cast<const R*>(vc.prev), vc.step_dt, vc.step_dx[a])
+    if (vc.prev && a > 0) {  // y/z coarse fluxes of an accumulating coarse step
+      if (f->precision == RHY_MIXED) {
+        if (nd == 2) RHY_REFLUX_S(2, 1, float, MixedMath);
+        else if (a == 1) RHY_REFLUX_S(3, 1, float, MixedMath);
+        else RHY_REFLUX_S(3, 2, float, MixedMath);
+      } else {
+        if (nd == 2) RHY_REFLUX_S(2, 1, double, PowF64Fix);
+        else if (a == 1) RHY_REFLUX_S(3, 1, double, PowF64Fix);
+        else RHY_REFLUX_S(3, 2, double, PowF64Fix);
+      }
+    } else if (f->precision == RHY_MIXED) {
       if (nd == 1) RHY_REFLUX(1, float);
       else if (nd == 2) RHY_REFLUX(2, float);
       else RHY_REFLUX(3, float);
@@ -1788,6 +1819,7 @@ int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double 
       else if (nd == 2) RHY_REFLUX(2, double);
       else RHY_REFLUX(3, double);
     }
+#undef RHY_REFLUX_S
 #undef RHY_REFLUX
   }
   BHIP_TRY(hipGetLastError());

@@ -117,7 +117,10 @@ int64_t defer_cap_setting();
 // the box's stream).
 struct BoxView {
   void* cells;
-  void* flux;  // flux planes of the last rhy_mh_solve
+  void* flux;  // flux planes of the last rhy_mh_solve (the x planes after rhy_amr_coarse_step)
+  void* prev;  // after rhy_amr_coarse_step: the state it stepped from, else nullptr
+  Consts k;    // that step's constants, dt and dx (with prev)
+  double step_dt, step_dx[3];
   Geom geom;
   int32_t ndim, ncmp, precision;
 };
@@ -1167,6 +1170,19 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
 
 [[maybe_unused]] static __global__ void set_scalar_kernel(double* dst, double v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+// A given dt and the step constants the walks read, as dt_kernel forms them.
+[[maybe_unused]] static __global__ void set_step_kernel(double* dt_out, double* stepc, double dt,
+                                                        double dx0, double dx1, double dx2) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double dx[3] = {dx0, dx1, dx2};
+  *dt_out = dt;
+  stepc[0] = dt;
+  for (int a = 0; a < 3; ++a) {
+    stepc[1 + a] = dt / dx[a];
+    stepc[4 + a] = .5 * dt / dx[a];
+  }
 }
 
 [[maybe_unused]] static __global__ void set_u32_kernel(uint32_t* dst, uint32_t v) {

@@ -279,3 +279,61 @@ def test_refluxing_conserves_on_device(ndim, precision):
     say("  drift with / without refluxing", drift[True], drift[False])
     assert drift[True] <= (1e-14 if precision == hydro.FP64 else 1e-6)
     assert drift[False] > 10 * drift[True]
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("case", ["2d", "3d", "2d_adjacent", "3d_adjacent"])
+def test_coarse_walks_equal_flux_plane_coarse_step(case, precision):
+    """The coarse step through rhy_amr_coarse_step (accumulating walks; the
+    reflux forms the coarse y/z interface fluxes from the pre-step state)
+    gives the hierarchy the flux-plane coarse step (ghost fill +
+    rhy_mh_solve, fluxes from the planes) gives, bit for bit: dts, coarse
+    and fine interiors, with refluxing on."""
+    ndim, cdims, fdims, lo, lim, bc, nsteps = CASES[case]
+    say("coarse walks", case, "precision", precision)
+    coarse = ics.random_smooth(ndim, cdims, seed=17 + ndim, vamp=0.6)
+    fines = [inject(ndim, coarse, l, d, 200 + i) for i, (l, d) in enumerate(zip(lo, fdims))]
+    p = params(ndim, lim, bc)
+    dx0 = [1.0 / cdims[0]] * ndim
+    out = {}
+    for walks in (True, False):
+        h = amr.AmrHierarchy(ndim, cdims, dx0, lo, fdims, precision=precision,
+                             coarse_walks=walks)
+        h.upload(coarse, fines)
+        dts = h.run(p, nsteps)
+        dc, dfs = h.download()
+        h.close()
+        out[walks] = (dts, dc, dfs)
+    inner = (slice(None),) + (slice(2, -2),) * ndim
+    np.testing.assert_array_equal(out[True][0], out[False][0])
+    np.testing.assert_array_equal(out[True][1][inner], out[False][1][inner])
+    for a, b in zip(out[True][2], out[False][2]):
+        np.testing.assert_array_equal(a[inner], b[inner])
+
+
+def test_coarse_step_equals_fill_and_solve():
+    """rhy_amr_coarse_step on a lone box equals rhy_bc_set_boundaries +
+    rhy_mh_solve bit for bit (interior), both solver types."""
+    for solver in (hydro.MEMORY_INTENSIVE, hydro.CPU_INTENSIVE):
+        dims = [18, 14, 12]
+        cells = ics.random_smooth(3, dims, seed=5, vamp=0.5)
+        p = hydro.make_params(
+            3, rp=hydro.RiemannProblem(irs=hydro.Irs(n_iteration=100, tolerance=1e-6)),
+            sl=hydro.SlopeLimiter(type=hydro.VAN_LEER), mh=hydro.MusclHancock(solver_type=solver),
+            cfl=hydro.Cfl(0.3), bc=hydro.SamrBc(types=[hydro.REFLECTIVE] * 6))
+        dx = [1.0 / 18] * 3
+        a = hydro.SamrBox(3, dims, cells=cells.copy())
+        a.upload()
+        dt = hydro.rhyme_cfl_time_step(p, a, dx)
+        rc = _L().rhy_amr_coarse_step(a.handle, C.byref(p), (C.c_double * 3)(*dx), dt)
+        assert rc == 0
+        ua = a.download().copy()
+        a.close()
+        b = hydro.SamrBox(3, dims, cells=cells.copy())
+        b.upload()
+        hydro.rhyme_samr_bc_set_boundaries(p, b)
+        hydro.rhyme_muscl_hancock_solve(p, b, dx, dt)
+        ub = b.download().copy()
+        b.close()
+        inner = (slice(None),) + (slice(2, -2),) * 3
+        np.testing.assert_array_equal(ua[inner], ub[inner])
