@@ -496,6 +496,18 @@ int rhy_amr_flux_register(rhy_batch_t *fine, const int32_t *lo, int32_t reset);
 int rhy_amr_reflux(rhy_box_t *coarse, rhy_batch_t *fine, const int32_t *lo, double dt,
                    const double dx[3]);
 int rhy_amr_coarse_step(rhy_box_t *coarse, const rhy_params_t *p, const double dx[3], double dt);
+/* rhy_amr_regrid: no counterpart in the reference (it has no coupled
+ * levels); for callers of the hierarchy that move the fine level.  Fills
+ * the fine batch `to` (its boxes placed at to_lo, same rules as lo above)
+ * from the current coarse box and the old fine batch `from` (at from_lo):
+ * every cell of every new box, both ghost layers included, takes the old
+ * fine cell at the same fine-level position when an old box's interior
+ * holds it, else its coarse parent's value (the hydro components).  Call
+ * it after a restriction, so that coarse cells under old boxes hold their
+ * children's mean and a newly refined cell's children restrict back to it.
+ * Waits for both batches' and the coarse box's streams. */
+int rhy_amr_regrid(rhy_box_t *coarse, rhy_batch_t *from, const int32_t *from_lo,
+                   rhy_batch_t *to, const int32_t *to_lo);
 
 /* Snapshot payload of a whole level (a batch or pool): the active boxes in
  * slot order, each laid out as rhy_box_snapshot_pack lays out one box; the

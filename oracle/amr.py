@@ -142,6 +142,39 @@ def restrict(ndim, fine, lo, coarse):
         coarse[c][csel] = s * w
 
 
+def regrid(ndim, coarse, fines, lo, new_dims, new_lo):
+    """The fine level moved to new boxes (new_dims[i] cells at new_lo[i]):
+    every cell of a new box, both ghost layers included, takes the old fine
+    cell at the same fine-level position (cell f of a box at lo lies at
+    2 (lo - 1) + f) when an old box's interior holds it, else the value of
+    its coarse parent (cell lo + (f + 1) // 2 - 1), hydro components only;
+    the others are 0.  Returns the new fine arrays."""
+    nv = ndim + 2
+    out = []
+    for n_f, l in zip(new_dims, new_lo):
+        shape = (coarse.shape[0],) + tuple(n_f[d] + 4 for d in reversed(range(ndim)))
+        new = np.zeros(shape)
+        f = [np.arange(-1, n_f[d] + 3) for d in range(ndim)]
+        parent = [l[d] + (f[d] + 1) // 2 - 1 + 1 for d in range(ndim)]  # array indices
+        new[:nv] = coarse[(slice(0, nv),) + np.ix_(*[parent[d] for d in reversed(range(ndim))])]
+        gf = [2 * (l[d] - 1) + f[d] for d in range(ndim)]
+        for old, lb in zip(fines, lo):
+            m = _fine_dims(old, ndim)
+            sel_new, sel_old = [], []
+            for d in range(ndim):
+                sc = gf[d] - 2 * (lb[d] - 1)            # the old box's cell index
+                hit = np.nonzero((sc >= 1) & (sc <= m[d]))[0]
+                if hit.size == 0:
+                    break
+                sel_new.append(slice(hit[0], hit[-1] + 1))
+                sel_old.append(slice(sc[hit[0]] + 1, sc[hit[-1]] + 2))
+            else:
+                new[(slice(0, nv),) + tuple(reversed(sel_new))] = \
+                    old[(slice(0, nv),) + tuple(reversed(sel_old))]
+        out.append(new)
+    return out
+
+
 def _transverse(ndim, a):
     return [d for d in range(ndim) if d != a]
 

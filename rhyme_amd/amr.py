@@ -76,6 +76,7 @@ class AmrHierarchy:
         self.fine_lo = [list(l)[:ndim] for l in fine_lo]
         self._check_placement()
         self.ncmp = ncmp if ncmp is not None else hydro.ncmp_for(ndim)
+        self.precision = precision
         self.coarse = hydro.SamrBox(ndim, self.base_dims, self.ncmp, precision)
         self.fine = SamrBatch(ndim, self.fine_dims, [self.dx1] * len(self.fine_dims),
                               ncmp=self.ncmp, precision=precision)
@@ -84,16 +85,18 @@ class AmrHierarchy:
             lo[i, :ndim] = l
         self._lo = np.ascontiguousarray(lo.ravel())
 
-    def _check_placement(self):
+    def _check_placement(self, fine_dims=None, fine_lo=None):
         nd = self.ndim
-        for n_f, l in zip(self.fine_dims, self.fine_lo):
+        fine_dims = self.fine_dims if fine_dims is None else fine_dims
+        fine_lo = self.fine_lo if fine_lo is None else fine_lo
+        for n_f, l in zip(fine_dims, fine_lo):
             for d in range(nd):
                 if n_f[d] % 2:
                     raise ValueError("fine extents must be even (ratio 2)")
                 if l[d] < 2 or l[d] + n_f[d] // 2 > self.base_dims[d]:
                     raise ValueError("a fine box or its ghosts leave the coarse interior")
         cover = [[(l[d], l[d] + n_f[d] // 2 - 1) for d in range(nd)]
-                 for n_f, l in zip(self.fine_dims, self.fine_lo)]
+                 for n_f, l in zip(fine_dims, fine_lo)]
         for a in range(len(cover)):
             for b in range(a + 1, len(cover)):
                 if all(cover[a][d][0] <= cover[b][d][1] and cover[b][d][0] <= cover[a][d][1]
@@ -103,6 +106,34 @@ class AmrHierarchy:
     def close(self):
         self.fine.close()
         self.coarse.close()
+
+    def regrid(self, fine_lo, fine_dims):
+        """Move the fine level to new boxes (fine_dims[i] cells at
+        fine_lo[i]): a new fine cell takes the old fine cell at its position
+        when an old box held it, else its coarse parent's value
+        (rhy_amr_regrid; oracle/amr.py regrid).  Call it between coarse
+        steps, when the coarse cells under the old boxes hold their
+        children's mean (a coarse step ends with that restriction)."""
+        nd = self.ndim
+        fine_dims = [list(d)[:nd] for d in fine_dims]
+        fine_lo = [list(l)[:nd] for l in fine_lo]
+        self._check_placement(fine_dims, fine_lo)
+        new = SamrBatch(nd, fine_dims, [self.dx1] * len(fine_dims), ncmp=self.ncmp,
+                        precision=self.precision)
+        lo = np.ones((len(fine_dims), 3), dtype=np.int32)
+        for i, l in enumerate(fine_lo):
+            lo[i, :nd] = l
+        lo = np.ascontiguousarray(lo.ravel())
+        try:
+            _check(_L().rhy_amr_regrid(self.coarse.handle, self.fine._h,
+                                       self._lo.ctypes.data_as(C.POINTER(C.c_int32)), new._h,
+                                       lo.ctypes.data_as(C.POINTER(C.c_int32))),
+                   "rhy_amr_regrid")
+        except Exception:
+            new.close()
+            raise
+        self.fine.close()
+        self.fine, self.fine_dims, self.fine_lo, self._lo = new, fine_dims, fine_lo, lo
 
     def upload(self, coarse_cells, fine_cells):
         self.coarse.cells[...] = coarse_cells

@@ -392,6 +392,60 @@ __global__ __launch_bounds__(256) void amr_prolong_kernel(const AmrItem* __restr
   }
 }
 
+// Regridding: every cell (interior and both ghost layers) of every new
+// fine box (items [0, nto), threads over their ghosted cells) takes the old
+// fine cell at the same fine-level position when an old box's interior
+// holds it (items [nto, nto + nfrom): the old boxes, in the old arena),
+// else its coarse parent's value; the ND + 2 hydro components.  In
+// fine-level indices a box's cell f lies at 2 (lo - 1) + f.
+template <int ND, typename R>
+__global__ __launch_bounds__(256) void amr_regrid_kernel(const AmrItem* __restrict__ items,
+                                                         int nto, int nfrom, int64_t total,
+                                                         R* __restrict__ to_arena,
+                                                         const R* __restrict__ from_arena,
+                                                         Geom GC, const R* __restrict__ coarse) {
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= total) return;
+  const AmrItem& it = items[amr_item_of(items, nto, t0)];
+  const Geom& GT = it.GD;
+  int64_t q = t0 - it.first;
+  int idx[3] = {1, 1, 1};
+  for (int d = 0; d < ND; ++d) {
+    const int e = GT.n[d] + 4;
+    idx[d] = (int)(q % e) - 1;
+    q /= e;
+  }
+  int gf[3] = {0, 0, 0};
+  for (int d = 0; d < ND; ++d) gf[d] = 2 * (it.lo[d] - 1) + idx[d];
+  const int64_t ot = goff(GT, idx[0], idx[1], idx[2]);
+  const R* src = nullptr;
+  int64_t os = 0, splane = 0;
+  for (int j = 0; j < nfrom && !src; ++j) {
+    const AmrItem& fj = items[nto + j];
+    int sc[3] = {1, 1, 1};
+    bool in = true;
+    for (int d = 0; d < ND; ++d) {
+      sc[d] = gf[d] - 2 * (fj.lo[d] - 1);
+      in = in && sc[d] >= 1 && sc[d] <= fj.GD.n[d];
+    }
+    if (in) {
+      src = from_arena + fj.dst_off;
+      os = goff(fj.GD, sc[0], sc[1], sc[2]);
+      splane = fj.GD.plane;
+    }
+  }
+  if (!src) {
+    int pc[3] = {1, 1, 1};
+    for (int d = 0; d < ND; ++d) pc[d] = coarse_parent(it.lo[d], idx[d]);
+    src = coarse;
+    os = goff(GC, pc[0], pc[1], pc[2]);
+    splane = GC.plane;
+  }
+  R* dst = to_arena + it.dst_off;
+#pragma unroll
+  for (int c = 0; c < ND + 2; ++c) dst[ot + c * GT.plane] = src[os + c * splane];
+}
+
 // Regions of fine ghosts that lie in a sibling's interior, copied from it:
 // ext[d] cells from src cell slo[d] to dst cell lo[d], the ND + 2 hydro
 // components.
@@ -1640,6 +1694,62 @@ int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
 #undef RHY_RESTRICT
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));
+  return RHY_OK;
+}
+
+int rhy_amr_regrid(rhy_box_t* coarse, rhy_batch_t* from, const int32_t* from_lo,
+                   rhy_batch_t* to, const int32_t* to_lo) {
+  BoxView vc;
+  int rc;
+  if (!coarse || !from || !to || !from_lo || !to_lo) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (from == to) return bfail(RHY_ERR_ARG, "the old and new fine levels are one batch");
+  if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if ((rc = amr_check(from, vc, from_lo))) return rc;
+  if ((rc = amr_check(to, vc, to_lo))) return rc;
+  BHIP_TRY(hipStreamSynchronize(from->stream));
+  const int nd = to->ndim, nto = (int)to->slots.size(), nfrom = (int)from->slots.size();
+  to->amr_host.clear();
+  int64_t total = 0;
+  for (int i = 0; i < nto; ++i) {
+    const BBox& B = to->slots[i].B;
+    AmrItem it{};
+    it.first = total;
+    it.dst_off = B.cell_off;
+    it.GD = B.G;
+    for (int d = 0; d < 3; ++d) it.lo[d] = to_lo[3 * i + d];
+    to->amr_host.push_back(it);
+    int64_t m = 1;
+    for (int d = 0; d < nd; ++d) m *= B.G.n[d] + 4;
+    total += m;
+  }
+  for (int j = 0; j < nfrom; ++j) {
+    const BBox& B = from->slots[j].B;
+    AmrItem it{};
+    it.first = total;
+    it.dst_off = B.cell_off;
+    it.GD = B.G;
+    for (int d = 0; d < 3; ++d) it.lo[d] = from_lo[3 * j + d];
+    to->amr_host.push_back(it);
+  }
+  if ((rc = amr_upload(to))) return rc;
+  if (total > 0) {
+#define RHY_REGRID(ND, R)                                                                       \
+  hipLaunchKernelGGL((amr_regrid_kernel<ND, R>), dim3(cdiv(total, 256)), dim3(256), 0,         \
+                     to->stream, to->amr_items, nto, nfrom, total, static_cast<R*>(to->arena), \
+                     static_cast<const R*>(from->arena), vc.geom, static_cast<const R*>(vc.cells))
+    if (to->precision == RHY_MIXED) {
+      if (nd == 1) RHY_REGRID(1, float);
+      else if (nd == 2) RHY_REGRID(2, float);
+      else RHY_REGRID(3, float);
+    } else {
+      if (nd == 1) RHY_REGRID(1, double);
+      else if (nd == 2) RHY_REGRID(2, double);
+      else RHY_REGRID(3, double);
+    }
+#undef RHY_REGRID
+  }
+  BHIP_TRY(hipGetLastError());
+  BHIP_TRY(hipStreamSynchronize(to->stream));
   return RHY_OK;
 }
 

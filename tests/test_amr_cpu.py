@@ -294,3 +294,89 @@ def test_register_and_reflux_against_loops():
                             z, y, x = out[2] + 1, out[1] + 1, out[0] + 1
                             want[c, z, y, x] = want[c, z, y, x] + dd
     np.testing.assert_array_equal(coarse, want)
+
+
+# ------------------------------------------------------------ regrid ----
+
+def _regrid_loops(ndim, coarse, fines, lo, new_dims, new_lo):
+    """The regrid rule cell by cell with the indices written out."""
+    nv = ndim + 2
+    out = []
+    for n_f, l in zip(new_dims, new_lo):
+        new = np.zeros((coarse.shape[0],) + tuple(n_f[d] + 4 for d in reversed(range(ndim))))
+        ranges = [range(-1, n_f[d] + 3) for d in range(ndim)] + [range(1, 2)] * (3 - ndim)
+        for k in ranges[2]:
+            for j in ranges[1]:
+                for i in ranges[0]:
+                    f = (i, j, k)[:ndim]
+                    gf = [2 * (l[d] - 1) + f[d] for d in range(ndim)]
+                    src = None
+                    for old, lb in zip(fines, lo):
+                        m = [old.shape[ndim - d] - 4 for d in range(ndim)]
+                        sc = [gf[d] - 2 * (lb[d] - 1) for d in range(ndim)]
+                        if all(1 <= sc[d] <= m[d] for d in range(ndim)):
+                            src = old[(slice(0, nv),) + tuple(_fidx(sc[d])
+                                                              for d in reversed(range(ndim)))]
+                    if src is None:
+                        pc = [l[d] + (f[d] + 1) // 2 - 1 for d in range(ndim)]
+                        src = coarse[(slice(0, nv),) + tuple(_fidx(pc[d])
+                                                             for d in reversed(range(ndim)))]
+                    new[(slice(0, nv),) + tuple(_fidx(f[d]) for d in reversed(range(ndim)))] = src
+        out.append(new)
+    return out
+
+
+REGRID = {
+    2: ([24, 20], [[8, 6], [6, 4]], [[3, 3], [12, 10]], [[10, 8], [4, 6], [6, 4]],
+        [[4, 4], [10, 9], [17, 14]]),
+    3: ([14, 12, 12], [[6, 6, 4], [4, 4, 4]], [[3, 3, 3], [8, 7, 7]], [[8, 4, 6], [4, 6, 4]],
+        [[4, 3, 4], [9, 6, 8]]),
+}
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_regrid_against_loops(ndim):
+    """Shifted, grown and new boxes over two old ones: the oracle's regrid
+    equals the rule applied cell by cell; a new box equal to an old one
+    keeps its interior bit for bit; boxes over no old box are the coarse
+    parents' values (fine_from_coarse)."""
+    from rhyme_amd import amr
+    cdims, odims, olo, ndims, nlo = REGRID[ndim]
+    coarse = ics.random_smooth(ndim, cdims, seed=60 + ndim)
+    g = np.random.default_rng(ndim)
+    fines = [amr.fine_from_coarse(ndim, coarse, l, d) * (1 + 0.02 * g.standard_normal())
+             for l, d in zip(olo, odims)]
+    got = oamr.regrid(ndim, coarse, fines, olo, ndims, nlo)
+    want = _regrid_loops(ndim, coarse, fines, olo, ndims, nlo)
+    for a, b in zip(got, want):
+        assert np.array_equal(a, b)
+    same = oamr.regrid(ndim, coarse, fines, olo, odims, olo)
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    for a, b in zip(same, fines):
+        assert np.array_equal(a[inner], b[inner])
+    far = oamr.regrid(ndim, coarse, [], [], ndims, nlo)
+    for a, d, l in zip(far, ndims, nlo):
+        assert np.array_equal(a[:ndim + 2], amr.fine_from_coarse(ndim, coarse, l, d)[:ndim + 2])
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_regrid_conserves(ndim):
+    """After a restriction, a regrid and a restriction of the new level keep
+    the coarse totals of every conserved variable to rounding (old fine cells
+    are kept, new children copy their parent)."""
+    from rhyme_amd import amr
+    cdims, odims, olo, ndims, nlo = REGRID[ndim]
+    coarse = ics.random_smooth(ndim, cdims, seed=70 + ndim)
+    g = np.random.default_rng(7 + ndim)
+    fines = [amr.fine_from_coarse(ndim, coarse, l, d) * (1 + 0.02 * g.standard_normal())
+             for l, d in zip(olo, odims)]
+    for f, l in zip(fines, olo):
+        oamr.restrict(ndim, f, l, coarse)
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    before = coarse[inner].reshape(ndim + 2, -1).sum(axis=1)
+    new = oamr.regrid(ndim, coarse, fines, olo, ndims, nlo)
+    after_c = coarse.copy()
+    for f, l in zip(new, nlo):
+        oamr.restrict(ndim, f, l, after_c)
+    after = after_c[inner].reshape(ndim + 2, -1).sum(axis=1)
+    assert np.all(np.abs(after - before) <= 1e-13 * np.abs(before))

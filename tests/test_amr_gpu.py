@@ -337,3 +337,49 @@ def test_coarse_step_equals_fill_and_solve():
         b.close()
         inner = (slice(None),) + (slice(2, -2),) * 3
         np.testing.assert_array_equal(ua[inner], ub[inner])
+
+
+REGRID = {
+    2: ([24, 20], [[8, 6], [6, 4]], [[3, 3], [12, 10]], [[10, 8], [4, 6], [6, 4]],
+        [[4, 4], [10, 9], [17, 14]]),
+    3: ([14, 12, 12], [[6, 6, 4], [4, 4, 4]], [[3, 3, 3], [8, 7, 7]], [[8, 4, 6], [4, 6, 4]],
+        [[4, 3, 4], [9, 6, 8]]),
+}
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_regrid_matches_oracle(ndim, precision):
+    """Two coarse steps, a regrid to shifted, grown and new fine boxes, two
+    more steps: the device's regrid equals oracle/amr.py's regrid of the
+    device's own arrays bit for bit (every cell of the new boxes, ghosts
+    included; the coarse box untouched), and the steps after it match the
+    oracle hierarchy started from those arrays."""
+    cdims, odims, olo, ndims, nlo = REGRID[ndim]
+    say("regrid", ndim, "precision", precision, "old", odims, olo, "new", ndims, nlo)
+    coarse = ics.random_smooth(ndim, cdims, seed=80 + ndim, vamp=0.6)
+    fines = [inject(ndim, coarse, l, d, 300 + i) for i, (l, d) in enumerate(zip(olo, odims))]
+    p = params(ndim, hydro.MINMOD, hydro.OUTFLOW)
+    dx0 = [1.0 / cdims[0]] * ndim
+    h = amr.AmrHierarchy(ndim, cdims, dx0, olo, odims, precision=precision)
+    h.upload(coarse, fines)
+    h.run(p, 2)
+    dc, dfs = h.download()
+    h.regrid(nlo, ndims)
+    dc2, dfs2 = h.download()
+    assert np.array_equal(dc2, dc)
+    want = oamr.regrid(ndim, dc, dfs, olo, ndims, nlo)
+    for a, b in zip(dfs2, want):
+        assert np.array_equal(a, b)
+    dts = h.run(p, 2)
+    ec, efs = h.download()
+    h.close()
+    pc = oracle.params_from(p)
+    pf = oamr.fine_params(pc)
+    rc, rfs = dc2.copy(), [w.copy() for w in want]
+    rdts = np.array([oamr.step(pc, pf, cdims, rc, dx0, rfs, nlo) for _ in range(2)])
+    tol = 1e-12 if precision == hydro.FP64 else 1e-5
+    assert np.max(np.abs(dts - rdts) / rdts) <= tol
+    assert max(rel_l1(ec, rc, ndim)) <= tol
+    for a, b in zip(efs, rfs):
+        assert max(rel_l1(a, b, ndim)) <= tol
