@@ -17,6 +17,9 @@ size, plus a short oracle prefix where that fits:
   box bit-identical to the same box run alone.
 * Served: the 1000-job stream at a saturating arrival rate, sampled jobs
   bit-identical (state and dt sequence) to the same job run alone.
+* Hierarchy: the 256^3 blast with 2x2x2 fine boxes of 64^3 (the bench's
+  workload), refluxing on, stepped with the accumulating coarse step and
+  with the flux-plane coarse step: bit-identical levels and dts.
 """
 import math
 
@@ -213,3 +216,30 @@ def test_served_full_stream_identical_to_alone(precision):
         np.testing.assert_array_equal(np.array(j.dts), rdts)
         np.testing.assert_array_equal(j.result, ref)
     srv.close()
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+def test_hierarchy256_coarse_walks_identical(precision):
+    from rhyme_amd import amr
+    n, m, steps = 256, 64, 3
+    c0 = n // 2 - m // 2 + 1
+    lo = [[c0 + m // 2 * a, c0 + m // 2 * b, c0 + m // 2 * c]
+          for c in (0, 1) for b in (0, 1) for a in (0, 1)]
+    fdims = [[m, m, m]] * 8
+    cells = ics.blast_3d(n=n, seed=1)
+    fines = [amr.fine_from_coarse(3, cells, l, d) for l, d in zip(lo, fdims)]
+    p = params(3, hydro.SUPERBEE, hydro.REFLECTIVE)
+    out = {}
+    for walks in (True, False):
+        say("hierarchy 256^3 + 8 x 64^3, precision", precision, "coarse walks", walks)
+        h = amr.AmrHierarchy(3, [n] * 3, [1.0 / n] * 3, lo, fdims, precision=precision,
+                             coarse_walks=walks)
+        h.upload(cells, fines)
+        dts = h.run(p, steps)
+        dc, dfs = h.download()
+        h.close()
+        out[walks] = (dts, interior(3, dc).copy(), [interior(3, f).copy() for f in dfs])
+    assert np.array_equal(out[True][0], out[False][0])
+    assert np.array_equal(out[True][1], out[False][1])
+    for a, b in zip(out[True][2], out[False][2]):
+        assert np.array_equal(a, b)
