@@ -508,6 +508,20 @@ int rhy_amr_coarse_step(rhy_box_t *coarse, const rhy_params_t *p, const double d
  * Waits for both batches' and the coarse box's streams. */
 int rhy_amr_regrid(rhy_box_t *coarse, rhy_batch_t *from, const int32_t *from_lo,
                    rhy_batch_t *to, const int32_t *to_lo);
+/* The same calls with a box of a fixed batch as the coarse side (`slot` of
+ * `coarse`): a level between the coarsest box and the finest batch, whose
+ * fine boxes all lie in that one box (lo relative to it).  No counterpart in
+ * the reference; for hierarchies of more than two levels.  The coarse
+ * fluxes of rhy_amr_reflux_slot are that box's flux planes from its batch's
+ * last step. */
+int rhy_amr_prolong_slot(rhy_batch_t *coarse, int32_t slot, rhy_batch_t *fine,
+                         const int32_t *lo, int32_t half);
+int rhy_amr_restrict_slot(rhy_batch_t *coarse, int32_t slot, rhy_batch_t *fine,
+                          const int32_t *lo);
+int rhy_amr_reflux_slot(rhy_batch_t *coarse, int32_t slot, rhy_batch_t *fine, const int32_t *lo,
+                        double dt, const double dx[3]);
+int rhy_amr_regrid_slot(rhy_batch_t *coarse, int32_t slot, rhy_batch_t *from,
+                        const int32_t *from_lo, rhy_batch_t *to, const int32_t *to_lo);
 
 /* Snapshot payload of a whole level (a batch or pool): the active boxes in
  * slot order, each laid out as rhy_box_snapshot_pack lays out one box; the

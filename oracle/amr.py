@@ -275,3 +275,61 @@ def step(pc, pf, coarse_dims, coarse, dx0, fines, lo, refluxing=True):
     for f, l in zip(fines, lo):
         restrict(ndim, f, l, coarse)
     return dt0
+
+
+def _kids_cycle(ndim, pf, parent, parent_old, parent_flux, dt, dx, dxs, lev, levels, kids,
+                refluxing):
+    """Two substeps of dt / 2 of the boxes `kids` of levels[lev] (children of
+    `parent`, whose step of dt from parent_old gave parent_flux), each followed
+    by its own children's cycle; then their reflux and restriction into the
+    parent.  Siblings are the children of one parent."""
+    fines = [levels[lev]["boxes"][i] for i in kids]
+    lo = [levels[lev]["lo"][i] for i in kids]
+    fdims = [_fine_dims(f, ndim) for f in fines]
+    reg = {}
+    for half in (0, 1):
+        for f, l in zip(fines, lo):
+            prolong(ndim, f, l, parent, parent_old, half)
+        fill_siblings(ndim, fines, lo)
+        for j, (fd, f) in enumerate(zip(fdims, fines)):
+            f_old = f.copy()
+            fl = mh_solve_fluxes(pf, fd, f, dxs[lev + 1], 0.5 * dt)
+            if refluxing:
+                register(ndim, fl, lo[j], reg, j, half == 0)
+            if lev + 1 < len(levels):
+                grand = [g for g, par in enumerate(levels[lev + 1]["parent"]) if par == kids[j]]
+                if grand:
+                    _kids_cycle(ndim, pf, f, f_old, fl, 0.5 * dt, dxs[lev + 1], dxs, lev + 1,
+                                levels, grand, refluxing)
+    if refluxing:
+        reflux(ndim, parent, parent_flux, fdims, lo, reg, dt, dx)
+    for f, l in zip(fines, lo):
+        restrict(ndim, f, l, parent)
+
+
+def step_levels(pc, pf, coarse_dims, coarse, dx0, levels, refluxing=True):
+    """One coarse step of a hierarchy of 1 + len(levels) levels, in place;
+    returns dt0.  levels[k] = {"boxes": arrays, "lo": placements, "parent":
+    index of each box's parent in levels[k - 1] (level 1: the coarse box,
+    None)}, level k + 1's boxes at dx0 / 2^(k + 1), each inside one parent
+    box (lo relative to it, the placement rules above).  dt0 = min over
+    levels of 2^k times the level's CFL dt; the coarse box takes its ghost
+    fill and one step of dt0; every box of level k + 1 then takes two
+    substeps of half its parent's dt, each after the prolongation from its
+    parent (U_old, then 0.5 U_old + 0.5 U_new) and the sibling fill among
+    the parent's children, and followed by its own children's cycle; then
+    the children's reflux into the parent (from the parent step's fluxes)
+    and restriction.  With one fine level this is step() bit for bit."""
+    ndim = len(coarse_dims)
+    dxs = [list(dx0)] + [[x / 2.0 ** (k + 1) for x in dx0] for k in range(len(levels))]
+    dt0 = cfl_time_step(pc, coarse_dims, coarse, dx0)
+    for k, lev in enumerate(levels):
+        for f in lev["boxes"]:
+            dt0 = min(dt0, 2.0 ** (k + 1) * cfl_time_step(pf, _fine_dims(f, ndim), f, dxs[k + 1]))
+    old = coarse.copy()
+    bc_set_boundaries(pc, coarse_dims, coarse)
+    cflux = mh_solve_fluxes(pc, coarse_dims, coarse, dx0, dt0)
+    if levels and levels[0]["boxes"]:
+        _kids_cycle(ndim, pf, coarse, old, cflux, dt0, dxs[0], dxs, 0, levels,
+                    list(range(len(levels[0]["boxes"]))), refluxing)
+    return dt0

@@ -43,6 +43,8 @@ EXPORTS = [
     "rhy_riemann_sampling_batch", "rhy_riemann_region_batch", "rhy_irs_exec_batch",
     "rhy_irs_guess_batch", "rhy_pow_batch", "rhy_mh_solve_noalloc", "rhy_box_draw",
     "rhy_amr_flux_register", "rhy_amr_reflux", "rhy_amr_coarse_step", "rhy_amr_regrid",
+    "rhy_amr_prolong_slot", "rhy_amr_restrict_slot", "rhy_amr_reflux_slot",
+    "rhy_amr_regrid_slot",
 ]
 # size of the static device scratch of rhy_mh_solve_noalloc
 NOALLOC_SCRATCH_BYTES = 64 << 20

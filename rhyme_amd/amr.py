@@ -63,7 +63,7 @@ class AmrHierarchy:
     holds fine cells 1 and 2 along each axis."""
 
     def __init__(self, ndim, base_dims, base_dx, fine_lo, fine_dims, precision=hydro.FP64,
-                 ncmp=None, refluxing=True, coarse_walks=True):
+                 ncmp=None, refluxing=True, coarse_walks=True, level2=()):
         self.ndim = ndim
         self.refluxing = refluxing
         # the coarse step through rhy_amr_coarse_step (accumulating walks), or
@@ -84,16 +84,33 @@ class AmrHierarchy:
         for i, l in enumerate(self.fine_lo):
             lo[i, :ndim] = l
         self._lo = np.ascontiguousarray(lo.ravel())
+        # a third level: per level-1 parent box, a batch of its children
+        # (dx0 / 4), lo relative to the parent; (parent, lo, dims, batch, lo3)
+        self.kids = []
+        self.dx2 = [x / 4.0 for x in self.dx0]
+        for spec in level2:
+            par = int(spec["parent"])
+            kl = [list(l)[:ndim] for l in spec["lo"]]
+            kd = [list(d)[:ndim] for d in spec["dims"]]
+            if not 0 <= par < len(self.fine_dims):
+                raise ValueError("no such level-1 parent box")
+            self._check_placement(kd, kl, self.fine_dims[par])
+            b = SamrBatch(ndim, kd, [self.dx2] * len(kd), ncmp=self.ncmp, precision=precision)
+            l3 = np.ones((len(kd), 3), dtype=np.int32)
+            for i, l in enumerate(kl):
+                l3[i, :ndim] = l
+            self.kids.append((par, kl, kd, b, np.ascontiguousarray(l3.ravel())))
 
-    def _check_placement(self, fine_dims=None, fine_lo=None):
+    def _check_placement(self, fine_dims=None, fine_lo=None, parent_dims=None):
         nd = self.ndim
         fine_dims = self.fine_dims if fine_dims is None else fine_dims
         fine_lo = self.fine_lo if fine_lo is None else fine_lo
+        parent_dims = self.base_dims if parent_dims is None else parent_dims
         for n_f, l in zip(fine_dims, fine_lo):
             for d in range(nd):
                 if n_f[d] % 2:
                     raise ValueError("fine extents must be even (ratio 2)")
-                if l[d] < 2 or l[d] + n_f[d] // 2 > self.base_dims[d]:
+                if l[d] < 2 or l[d] + n_f[d] // 2 > parent_dims[d]:
                     raise ValueError("a fine box or its ghosts leave the coarse interior")
         cover = [[(l[d], l[d] + n_f[d] // 2 - 1) for d in range(nd)]
                  for n_f, l in zip(fine_dims, fine_lo)]
@@ -104,8 +121,20 @@ class AmrHierarchy:
                     raise ValueError("fine boxes %d and %d cover the same coarse cells" % (a, b))
 
     def close(self):
+        for k in self.kids:
+            k[3].close()
         self.fine.close()
         self.coarse.close()
+
+    def upload_kids(self, cells):
+        """Host arrays of the level-2 boxes, cells[j][i] for box i of the
+        j-th level2 entry."""
+        for (_, _, _, b, _), cs in zip(self.kids, cells):
+            for i, c in enumerate(cs):
+                b.upload(i, c)
+
+    def download_kids(self):
+        return [[b.download(i).copy() for i in range(b.n)] for (_, _, _, b, _) in self.kids]
 
     def regrid(self, fine_lo, fine_dims):
         """Move the fine level to new boxes (fine_dims[i] cells at
@@ -114,6 +143,8 @@ class AmrHierarchy:
         (rhy_amr_regrid; oracle/amr.py regrid).  Call it between coarse
         steps, when the coarse cells under the old boxes hold their
         children's mean (a coarse step ends with that restriction)."""
+        if self.kids:
+            raise ValueError("regrid moves the finest level; this hierarchy has level-2 boxes")
         nd = self.ndim
         fine_dims = [list(d)[:nd] for d in fine_dims]
         fine_lo = [list(l)[:nd] for l in fine_lo]
@@ -147,8 +178,9 @@ class AmrHierarchy:
 
     def zones_per_step(self):
         """Zone-updates of one coarse step: the coarse cells once, the fine
-        cells twice (two substeps)."""
-        return int(np.prod(self.base_dims)) + 2 * sum(int(np.prod(d)) for d in self.fine_dims)
+        cells twice (two substeps), level-2 cells four times."""
+        return (int(np.prod(self.base_dims)) + 2 * sum(int(np.prod(d)) for d in self.fine_dims)
+                + 4 * sum(int(np.prod(d)) for k in self.kids for d in k[2]))
 
     def _prolong(self, half):
         _check(_L().rhy_amr_prolong(self.coarse.handle, self.fine._h,
@@ -171,12 +203,48 @@ class AmrHierarchy:
                                    self._lo.ctypes.data_as(C.POINTER(C.c_int32)), dt0, dx),
                "rhy_amr_reflux")
 
+    def _kids_prolong0(self):
+        """Level-2 face ghosts from their parents' U_old: before the level-1
+        substep (and its sibling fill)."""
+        p32 = C.POINTER(C.c_int32)
+        for par, _, _, b, l3 in self.kids:
+            _check(_L().rhy_amr_prolong_slot(self.fine._h, par, b._h, l3.ctypes.data_as(p32), 0),
+                   "rhy_amr_prolong_slot")
+
+    def _kids_cycle(self, fp, dt1):
+        """Level 2 after a level-1 substep of dt1 (its ghosts set from U_old by
+        _kids_prolong0 before that substep): for each parent, two substeps of
+        its children, the second after the time-interpolated prolongation,
+        then their reflux and restriction into the parent (oracle/amr.py
+        step_levels)."""
+        p32 = C.POINTER(C.c_int32)
+        dx1 = (C.c_double * 3)(*(self.dx1 + [1.0] * (3 - self.ndim)))
+        for par, _, _, b, l3 in self.kids:
+            lo = l3.ctypes.data_as(p32)
+            for half in (0, 1):
+                if half:
+                    _check(_L().rhy_amr_prolong_slot(self.fine._h, par, b._h, lo, 1),
+                           "rhy_amr_prolong_slot")
+                b.run_dt_async(fp, 0.5 * dt1, 1)
+                if self.refluxing:
+                    _check(_L().rhy_amr_flux_register(b._h, lo, 1 if half == 0 else 0),
+                           "rhy_amr_flux_register")
+            if self.refluxing:
+                _check(_L().rhy_amr_reflux_slot(self.fine._h, par, b._h, lo, dt1, dx1),
+                       "rhy_amr_reflux_slot")
+            _check(_L().rhy_amr_restrict_slot(self.fine._h, par, b._h, lo),
+                   "rhy_amr_restrict_slot")
+
     def step(self, params):
         """One coarse step; returns dt0."""
         fp = fine_params(params)
         dt0 = hydro.rhyme_cfl_time_step(params, self.coarse, self.dx0)
         dt0 = min(dt0, 2.0 * float(self.fine.cfl_dts(fp).min()))
+        for k in self.kids:
+            dt0 = min(dt0, 4.0 * float(k[3].cfl_dts(fp).min()))
         self._prolong(0)                              # U_old: before the coarse step
+        if self.kids:
+            self._kids_prolong0()                     # level 2 from level-1 U_old
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)      # overlaps the coarse step
         if self.refluxing:
             self._register(True)                      # the batch's stream, after substep 1
@@ -187,10 +255,17 @@ class AmrHierarchy:
         else:
             hydro.rhyme_samr_bc_set_boundaries(params, self.coarse)
             hydro.rhyme_muscl_hancock_solve(params, self.coarse, self.dx0, dt0)
+        if self.kids:
+            self._kids_cycle(fp, 0.5 * dt0)           # level 2 under substep 1
         self._prolong(1)                              # after substep 1 (stream order)
+        if self.kids:
+            self._kids_prolong0()
         self.fine.run_dt_async(fp, 0.5 * dt0, 1)
         if self.refluxing:
             self._register(False)
+        if self.kids:
+            self._kids_cycle(fp, 0.5 * dt0)           # level 2 under substep 2
+        if self.refluxing:
             self._reflux(dt0)                         # waits for the coarse box's stream
         self._restrict()                              # waits for the batch's stream
         return dt0

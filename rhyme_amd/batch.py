@@ -43,6 +43,14 @@ _BATCH_SIGS = {
                                       C.POINTER(C.c_double), C.c_double]),
     "rhy_amr_regrid": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_int32), C.c_void_p,
                                  C.POINTER(C.c_int32)]),
+    "rhy_amr_prolong_slot": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p,
+                                       C.POINTER(C.c_int32), C.c_int32]),
+    "rhy_amr_restrict_slot": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p,
+                                        C.POINTER(C.c_int32)]),
+    "rhy_amr_reflux_slot": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
+                                      C.c_double, C.POINTER(C.c_double)]),
+    "rhy_amr_regrid_slot": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.POINTER(C.c_int32),
+                                      C.c_void_p, C.POINTER(C.c_int32)]),
     "rhy_batch_snapshot_length": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
     "rhy_batch_snapshot_pack": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     "rhy_batch_snapshot_unpack": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),

@@ -1511,6 +1511,29 @@ int rhy_batch_cfl_dts(rhy_batch_t* b, const rhy_params_t* p, double* dts) {
 
 // Coarse parents of every fine box (ghosts included) must be interior
 // coarse cells: 2 <= lo[d] and lo[d] + n_f[d] / 2 <= n_c[d], n_f[d] even.
+// A box of a fixed batch as the coarse side of a hierarchy call (a level
+// between the coarsest and the finest): its state and flux planes in the
+// batch's arena, after the batch's queued work.
+static int slot_view(rhy_batch_t* b, int32_t slot, BoxView* v) {
+  if (!b) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (b->pool) return bfail(RHY_ERR_ARG, "the coarse level is a fixed batch, not a pool");
+  if (slot < 0 || slot >= (int32_t)b->slots.size() || !b->slots[slot].active)
+    return bfail(RHY_ERR_ARG, "no such coarse box");
+  BHIP_TRY(hipStreamSynchronize(b->stream));
+  const BBox& B = b->slots[slot].B;
+  v->cells = static_cast<char*>(b->arena) + b->esz * (size_t)B.cell_off;
+  v->flux = static_cast<char*>(b->arena) + b->esz * (size_t)B.flux_off;
+  v->prev = nullptr;
+  v->k = Consts{};
+  v->step_dt = 0.0;
+  v->step_dx[0] = v->step_dx[1] = v->step_dx[2] = 1.0;
+  v->geom = B.G;
+  v->ndim = b->ndim;
+  v->ncmp = b->ncmp;
+  v->precision = b->precision;
+  return RHY_OK;
+}
+
 static int amr_check(rhy_batch_t* f, const BoxView& c, const int32_t* lo) {
   if (!f || !lo) return bfail(RHY_ERR_ARG, "NULL argument");
   if (f->pool) return bfail(RHY_ERR_ARG, "the fine level is a fixed batch, not a pool");
@@ -1549,12 +1572,9 @@ static int amr_upload(rhy_batch_t* f) {
   return RHY_OK;
 }
 
-int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_t half) {
-  BoxView vn;
+static int amr_prolong_v(const BoxView& vc, rhy_batch_t* f, const int32_t* lo, int32_t half) {
   int rc;
-  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
-  if ((rc = box_view(coarse, &vn))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
-  if ((rc = amr_check(f, vn, lo))) return rc;
+  if ((rc = amr_check(f, vc, lo))) return rc;
   const int nd = f->ndim, nb = (int)f->slots.size();
   f->amr_host.clear();
   // items [0, nb): one per fine box, its face ghost cells
@@ -1631,8 +1651,8 @@ int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_
   do {                                                                                         \
     R* arena = static_cast<R*>(f->arena);                                                      \
     hipLaunchKernelGGL((amr_prolong_kernel<ND, R>), dim3(cdiv(nprol, 256)), dim3(256), 0,     \
-                       f->stream, items, nb, nprol, arena, vn.geom,                            \
-                       static_cast<const R*>(vn.cells), half ? 1 : 0);                         \
+                       f->stream, items, nb, nprol, arena, vc.geom,                            \
+                       static_cast<const R*>(vc.cells), half ? 1 : 0);                         \
     if (ncopy)                                                                                 \
       hipLaunchKernelGGL((amr_copy_kernel<ND, R>), dim3(cdiv(ncopy, 256)), dim3(256), 0,      \
                          f->stream, items + nb, ncitems, ncopy, arena);                        \
@@ -1654,11 +1674,26 @@ int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_
   return RHY_OK;
 }
 
-int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
+int rhy_amr_prolong(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, int32_t half) {
   BoxView vc;
   int rc;
   if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
-  if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if ((rc = box_view(coarse, &vc)))
+    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  return amr_prolong_v(vc, f, lo, half);
+}
+
+int rhy_amr_prolong_slot(rhy_batch_t* coarse, int32_t slot, rhy_batch_t* f, const int32_t* lo,
+                         int32_t half) {
+  BoxView vc;
+  int rc;
+  if (coarse == f) return bfail(RHY_ERR_ARG, "the coarse and fine levels are one batch");
+  if ((rc = slot_view(coarse, slot, &vc))) return rc;
+  return amr_prolong_v(vc, f, lo, half);
+}
+
+static int amr_restrict_v(const BoxView& vc, rhy_batch_t* f, const int32_t* lo) {
+  int rc;
   if ((rc = amr_check(f, vc, lo))) return rc;
   const int nd = f->ndim, nb = (int)f->slots.size();
   f->amr_host.clear();
@@ -1697,13 +1732,28 @@ int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
   return RHY_OK;
 }
 
-int rhy_amr_regrid(rhy_box_t* coarse, rhy_batch_t* from, const int32_t* from_lo,
-                   rhy_batch_t* to, const int32_t* to_lo) {
+int rhy_amr_restrict(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo) {
   BoxView vc;
   int rc;
-  if (!coarse || !from || !to || !from_lo || !to_lo) return bfail(RHY_ERR_ARG, "NULL argument");
+  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vc)))
+    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  return amr_restrict_v(vc, f, lo);
+}
+
+int rhy_amr_restrict_slot(rhy_batch_t* coarse, int32_t slot, rhy_batch_t* f, const int32_t* lo) {
+  BoxView vc;
+  int rc;
+  if (coarse == f) return bfail(RHY_ERR_ARG, "the coarse and fine levels are one batch");
+  if ((rc = slot_view(coarse, slot, &vc))) return rc;
+  return amr_restrict_v(vc, f, lo);
+}
+
+static int amr_regrid_v(const BoxView& vc, rhy_batch_t* from, const int32_t* from_lo,
+                        rhy_batch_t* to, const int32_t* to_lo) {
+  int rc;
+  if (!from || !to || !from_lo || !to_lo) return bfail(RHY_ERR_ARG, "NULL argument");
   if (from == to) return bfail(RHY_ERR_ARG, "the old and new fine levels are one batch");
-  if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
   if ((rc = amr_check(from, vc, from_lo))) return rc;
   if ((rc = amr_check(to, vc, to_lo))) return rc;
   BHIP_TRY(hipStreamSynchronize(from->stream));
@@ -1751,6 +1801,25 @@ int rhy_amr_regrid(rhy_box_t* coarse, rhy_batch_t* from, const int32_t* from_lo,
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(to->stream));
   return RHY_OK;
+}
+
+int rhy_amr_regrid(rhy_box_t* coarse, rhy_batch_t* from, const int32_t* from_lo,
+                   rhy_batch_t* to, const int32_t* to_lo) {
+  BoxView vc;
+  int rc;
+  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vc))) return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  return amr_regrid_v(vc, from, from_lo, to, to_lo);
+}
+
+int rhy_amr_regrid_slot(rhy_batch_t* coarse, int32_t slot, rhy_batch_t* from,
+                        const int32_t* from_lo, rhy_batch_t* to, const int32_t* to_lo) {
+  BoxView vc;
+  int rc;
+  if (coarse == from || coarse == to)
+    return bfail(RHY_ERR_ARG, "the coarse level and a fine level are one batch");
+  if ((rc = slot_view(coarse, slot, &vc))) return rc;
+  return amr_regrid_v(vc, from, from_lo, to, to_lo);
 }
 
 // The register items of every fine box, grouped by (axis, side) pass in
@@ -1878,13 +1947,10 @@ int rhy_amr_flux_register(rhy_batch_t* f, const int32_t* lo, int32_t reset) {
   return RHY_OK;
 }
 
-int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double dt,
-                   const double dx[3]) {
-  BoxView vc;
+static int amr_reflux_v(const BoxView& vc, rhy_batch_t* f, const int32_t* lo, double dt,
+                        const double dx[3]) {
   int rc;
-  if (!coarse || !dx) return bfail(RHY_ERR_ARG, "NULL argument");
-  if ((rc = box_view(coarse, &vc)))
-    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  if (!dx) return bfail(RHY_ERR_ARG, "NULL argument");
   if ((rc = amr_check(f, vc, lo))) return rc;
   const std::vector<int64_t> before = f->reg_key;
   int64_t len = 0;
@@ -1935,6 +2001,25 @@ int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double 
   BHIP_TRY(hipGetLastError());
   BHIP_TRY(hipStreamSynchronize(f->stream));
   return RHY_OK;
+}
+
+int rhy_amr_reflux(rhy_box_t* coarse, rhy_batch_t* f, const int32_t* lo, double dt,
+                   const double dx[3]) {
+  BoxView vc;
+  int rc;
+  if (!coarse) return bfail(RHY_ERR_ARG, "NULL argument");
+  if ((rc = box_view(coarse, &vc)))
+    return bfail(rc, "coarse box: " + std::string(rhy_last_error()));
+  return amr_reflux_v(vc, f, lo, dt, dx);
+}
+
+int rhy_amr_reflux_slot(rhy_batch_t* coarse, int32_t slot, rhy_batch_t* f, const int32_t* lo,
+                        double dt, const double dx[3]) {
+  BoxView vc;
+  int rc;
+  if (coarse == f) return bfail(RHY_ERR_ARG, "the coarse and fine levels are one batch");
+  if ((rc = slot_view(coarse, slot, &vc))) return rc;
+  return amr_reflux_v(vc, f, lo, dt, dx);
 }
 
 int rhy_batch_sync(rhy_batch_t* b) {

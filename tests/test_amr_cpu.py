@@ -380,3 +380,79 @@ def test_regrid_conserves(ndim):
         oamr.restrict(ndim, f, l, after_c)
     after = after_c[inner].reshape(ndim + 2, -1).sum(axis=1)
     assert np.all(np.abs(after - before) <= 1e-13 * np.abs(before))
+
+
+# ----------------------------------------------------- three levels ----
+
+LEVELS3 = {
+    2: ([24, 20], [[12, 10], [6, 6]], [[4, 4], [15, 10]], [[8, 6], [4, 4]], [[2, 2], [7, 6]]),
+    3: ([14, 12, 12], [[12, 8, 8]], [[3, 3, 3]], [[8, 6, 4]], [[2, 2, 3]]),
+}
+
+
+def _levels3(ndim, seed, scale=0.0):
+    from rhyme_amd import amr
+    cdims, d1, lo1, d2, lo2 = LEVELS3[ndim]
+    coarse = ics.random_smooth(ndim, cdims, seed=seed, vamp=0.6)
+    f1 = [amr.fine_from_coarse(ndim, coarse, l, d) for l, d in zip(lo1, d1)]
+    f2 = [amr.fine_from_coarse(ndim, f1[0], l, d) for l, d in zip(lo2, d2)]
+    g = np.random.default_rng(seed)
+    for f in f1 + f2:
+        f[:ndim + 2] *= 1.0 + scale * g.uniform(-1, 1, size=f.shape[1:])
+    for f, l in zip(f2, lo2):
+        oamr.restrict(ndim, f, l, f1[0])
+    for f, l in zip(f1, lo1):
+        oamr.restrict(ndim, f, l, coarse)
+    levels = [{"boxes": f1, "lo": lo1, "parent": [None] * len(f1)},
+              {"boxes": f2, "lo": lo2, "parent": [0] * len(f2)}]
+    return cdims, coarse, levels
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_step_levels_reduces_to_step(ndim):
+    """With one fine level the recursive step is the two-level step bit for
+    bit (states and dts)."""
+    cdims, coarse, levels = _levels3(ndim, seed=90 + ndim, scale=0.02)
+    p = mk(ndim)
+    pf = oamr.fine_params(p)
+    dx0 = [1.0 / cdims[0]] * ndim
+    c1, f1 = coarse.copy(), [f.copy() for f in levels[0]["boxes"]]
+    c2, f2 = coarse.copy(), [f.copy() for f in levels[0]["boxes"]]
+    lo = levels[0]["lo"]
+    d1 = [oamr.step(p, pf, cdims, c1, dx0, f1, lo) for _ in range(2)]
+    one = [{"boxes": f2, "lo": lo, "parent": [None] * len(f2)}]
+    d2 = [oamr.step_levels(p, pf, cdims, c2, dx0, one) for _ in range(2)]
+    assert d1 == d2 and np.array_equal(c1, c2)
+    assert all(np.array_equal(a, b) for a, b in zip(f1, f2))
+
+
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_three_levels_uniform_and_conserving(ndim):
+    """Three levels: a uniform state stays uniform bit for bit on every
+    level, and with a periodic coarse box and refluxing at both interfaces
+    mass and energy (the coarse level after restriction) are kept to
+    rounding."""
+    cdims, coarse, levels = _levels3(ndim, seed=95 + ndim)
+    p = mk(ndim)
+    pf = oamr.fine_params(p)
+    dx0 = [1.0 / cdims[0]] * ndim
+    u = np.zeros_like(coarse)
+    u[0], u[ndim + 1] = 1.3, 2.1
+    uni = [{"boxes": [np.broadcast_to(u[:, :1, :1] if ndim == 2 else u[:, :1, :1, :1],
+                                      f.shape).copy() for f in lev["boxes"]],
+            "lo": lev["lo"], "parent": lev["parent"]} for lev in levels]
+    uc = u.copy()
+    for _ in range(2):
+        oamr.step_levels(p, pf, cdims, uc, dx0, uni)
+    inner = (slice(0, ndim + 2),) + (slice(2, -2),) * ndim
+    assert np.array_equal(uc[inner], u[inner])
+    for lev in uni:
+        for f in lev["boxes"]:
+            assert np.all(f[inner] == f[inner].reshape(ndim + 2, -1)[:, :1].reshape(
+                (ndim + 2,) + (1,) * ndim))
+    before = coarse[inner].reshape(ndim + 2, -1).sum(axis=1)
+    for _ in range(3):
+        oamr.step_levels(p, pf, cdims, coarse, dx0, levels)
+    after = coarse[inner].reshape(ndim + 2, -1).sum(axis=1)
+    for c in (0, ndim + 1):
+        assert abs(after[c] - before[c]) <= 1e-14 * abs(before[c])

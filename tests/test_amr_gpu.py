@@ -383,3 +383,48 @@ def test_regrid_matches_oracle(ndim, precision):
     assert max(rel_l1(ec, rc, ndim)) <= tol
     for a, b in zip(efs, rfs):
         assert max(rel_l1(a, b, ndim)) <= tol
+
+
+LEVELS3 = {
+    2: ([24, 20], [[12, 10], [6, 6]], [[4, 4], [15, 10]], [[8, 6], [4, 4]], [[2, 2], [7, 6]]),
+    3: ([14, 12, 12], [[12, 8, 8]], [[3, 3, 3]], [[8, 6, 4]], [[2, 2, 3]]),
+}
+
+
+@pytest.mark.parametrize("precision", [hydro.FP64, hydro.MIXED])
+@pytest.mark.parametrize("ndim", [2, 3])
+def test_three_levels_match_oracle(ndim, precision):
+    """A third level (boxes inside level-1 box 0, dx0 / 4, four substeps
+    per coarse step) through the slot calls (rhy_amr_*_slot) against
+    oracle/amr.py step_levels: dts, coarse, level-1 and level-2 interiors,
+    refluxing at both interfaces."""
+    cdims, d1, lo1, d2, lo2 = LEVELS3[ndim]
+    say("three levels", ndim, "precision", precision)
+    coarse = ics.random_smooth(ndim, cdims, seed=110 + ndim, vamp=0.6)
+    f1 = [inject(ndim, coarse, l, d, 400 + i) for i, (l, d) in enumerate(zip(lo1, d1))]
+    f2 = [inject(ndim, f1[0], l, d, 500 + i) for i, (l, d) in enumerate(zip(lo2, d2))]
+    p = params(ndim, hydro.VAN_LEER, hydro.OUTFLOW)
+    dx0 = [1.0 / cdims[0]] * ndim
+    h = amr.AmrHierarchy(ndim, cdims, dx0, lo1, d1, precision=precision,
+                         level2=[{"parent": 0, "lo": lo2, "dims": d2}])
+    h.upload(coarse, f1)
+    h.upload_kids([f2])
+    dts = h.run(p, 3)
+    dc, dfs = h.download()
+    (dks,) = h.download_kids()
+    h.close()
+    pc = oracle.params_from(p)
+    rc, r1, r2 = coarse.copy(), [f.copy() for f in f1], [f.copy() for f in f2]
+    levels = [{"boxes": r1, "lo": lo1, "parent": [None] * len(r1)},
+              {"boxes": r2, "lo": lo2, "parent": [0] * len(r2)}]
+    rdts = np.array([oamr.step_levels(pc, oamr.fine_params(pc), cdims, rc, dx0, levels)
+                     for _ in range(3)])
+    tol = 1e-12 if precision == hydro.FP64 else 1e-5
+    e_dt = float(np.max(np.abs(dts - rdts) / rdts))
+    say("  dt rel err", e_dt)
+    assert e_dt <= tol
+    assert max(rel_l1(dc, rc, ndim)) <= tol
+    for a, b in zip(dfs + dks, r1 + r2):
+        e = rel_l1(a, b, ndim)
+        say("  box rel L1", e)
+        assert max(e) <= tol
