@@ -83,6 +83,7 @@ struct rhy_box {
   double* partial = nullptr;      // per-block CFL maxima of the last reduction
   int32_t partial_n = 0;          // partials of the stand-alone CFL pass
   int32_t acc_blocks = 0;         // partials of the last accumulating walk
+  int32_t nf_x = 8, acc_L = 16;   // walk lengths for this shape (choose_walks)
   int32_t partial_cap = 0;
   double* scalars = nullptr;      // [0] max_u, [16..16+6*RHY_MAX_CMP) inflow states
   double inflow_copied[6 * RHY_MAX_CMP] = {};  // what the device inflow slots hold
@@ -224,6 +225,15 @@ constexpr int kNfYZ = RHY_NF_YZ;
 #endif
 constexpr int kAccL = RHY_ACC_L;
 
+// The walk lengths of a box: kNfX steps per x wave and kAccL cells per y/z
+// chunk, halved (down to 2 and 4) while a walk would have fewer waves than
+// fill the chip a few times over (256 CUs x 4 SIMDs x 3 waves = 3072 at a
+// time).  Longer walks repeat less work (a warm-up half step per x wave,
+// one extra face per chunk); more waves hide more latency.  Measured on
+// the 2D 1024^2 spheres: 8 / 16 -> 2 / 4 took 3.2e9 to 4.9e9 zone-updates/s
+// (FP64); at 3D 256^3 the longest lengths already give 33k and 16k waves.
+constexpr int64_t kMinWavesX = 8192, kMinWavesAcc = 4096;
+
 // Faces of axis a a step solves (faces 0..n_a along a, interior across).
 int64_t axis_faces(const Geom& G, int a) {
   int64_t n = (int64_t)G.n[a] + 1;
@@ -245,7 +255,7 @@ DeferList defer_list(rhy_box* b) {
 template <int ND, int A, int LIM, typename R, class PW>
 void launch_flux_axis(rhy_box* b, const Consts& k, const double* dx, const DeferList& dl) {
   const Geom& G = b->geom;
-  const int nf = A == 0 ? kNfX : kNfYZ;
+  const int nf = A == 0 ? b->nf_x : kNfYZ;
   const int64_t nthreads = flux_threads(G, A, nf);
   R* F = static_cast<R*>(b->flux) + (int64_t)A * (ND + 2) * G.plane;
   TimedLaunch tl;
@@ -441,8 +451,8 @@ void launch_acc_axis(rhy_box* b, const Consts& k, const double* dx,
   TimedLaunch tl;
   timed_begin(b, 0, tl);
   hipLaunchKernelGGL((acc_walk_kernel<ND, A, LIM, R, PW, MODE>),
-                     dim3(grid_for(acc_threads(G, A, kAccL))), dim3(kThreads), 0, b->stream, k, G,
-                     cur, fx, acc, out, step_consts(b), kAccL, order, dl, b->partial);
+                     dim3(grid_for(acc_threads(G, A, b->acc_L))), dim3(kThreads), 0, b->stream, k,
+                     G, cur, fx, acc, out, step_consts(b), b->acc_L, order, dl, b->partial);
   timed_end(b, tl);
   if constexpr (PW::kDefer) {
     TimedLaunch tf;
@@ -595,8 +605,17 @@ int alloc_box_memory(rhy_box* b) {
   }
   const int64_t ncell = (int64_t)b->geom.n[0] * b->geom.n[1] * b->geom.n[2];
   b->partial_n = (int32_t)grid_for(ncell);
+  b->nf_x = kNfX;
+  while (b->nf_x > 2 && flux_threads(b->geom, 0, b->nf_x) / 64 < kMinWavesX) b->nf_x /= 2;
+  b->acc_L = kAccL;
+  auto acc_waves = [&](int L) {
+    int64_t w = (int64_t)1 << 62;
+    for (int a = 1; a < b->ndim; ++a) w = std::min(w, acc_threads(b->geom, a, L) / 64);
+    return w;
+  };
+  while (b->ndim > 1 && b->acc_L > 4 && acc_waves(b->acc_L) < kMinWavesAcc) b->acc_L /= 2;
   b->acc_blocks =
-      b->ndim > 1 ? (int32_t)grid_for(acc_threads(b->geom, b->ndim - 1, kAccL)) : 0;
+      b->ndim > 1 ? (int32_t)grid_for(acc_threads(b->geom, b->ndim - 1, b->acc_L)) : 0;
   // one more slot: the CFL speeds of cells the fixup recomputed
   b->partial_cap = std::max(b->partial_n, b->acc_blocks) + 1;
   if (dev_alloc(&b->partial, sizeof(double) * (size_t)b->partial_cap) != hipSuccess) {
@@ -756,6 +775,8 @@ int rhy_box_resize(rhy_box_t* b, const int32_t dims[3]) {
   b->partial = t.partial;
   b->partial_n = t.partial_n;
   b->acc_blocks = t.acc_blocks;
+  b->nf_x = t.nf_x;
+  b->acc_L = t.acc_L;
   b->partial_cap = t.partial_cap;
   b->defer_count = t.defer_count;
   b->defer_items = t.defer_items;
