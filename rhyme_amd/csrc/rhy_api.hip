@@ -471,11 +471,8 @@ template <int ND, int LIM, typename R, class PW, class P>
 void launch_acc_step_lim(rhy_box* b, const Consts& k, const double* dx,
                          int order) {
   static_assert(ND > 1, "1D boxes step through the flux planes");
-  TimedLaunch tr;
-  timed_begin(b, 5, tr);
-  hipLaunchKernelGGL(step_reset_kernel, dim3(1), dim3(64), 0, b->stream, b->defer_count, 4,
-                     b->partial + b->acc_blocks);
-  timed_end(b, tr);
+  // the deferred-face counters and the fixup's partial slot were reset by
+  // the step's dt_kernel (rhy_run) or set_step_kernel (rhy_amr_coarse_step)
   const DeferList dlx = defer_list_axis(b, 0);
   launch_flux_axis<ND, 0, LIM, R, PW>(b, k, dx, dlx);
   if constexpr (PW::kDefer) {
@@ -544,7 +541,9 @@ int run_steps(rhy_box* b, const rhy_params_t* p, const double* dx, int32_t nstep
     TimedLaunch tl;
     timed_begin(b, 4, tl);
     hipLaunchKernelGGL(dt_kernel, dim3(1), dim3(1024), 0, b->stream, b->partial, np, cmdx, b->dt,
-                       b->dt_log + s, (double*)nullptr, dx[0], dx[1], dx[2], step_consts(b));
+                       b->dt_log + s, (double*)nullptr, dx[0], dx[1], dx[2], step_consts(b),
+                       ND > 1 ? b->defer_count : (uint32_t*)nullptr,
+                       ND > 1 ? b->partial + b->acc_blocks : (double*)nullptr);
     timed_end(b, tl);
     if constexpr (ND > 1) {
       if ((rc = launch_acc_step<ND>(b, k, dx, p->solver_type))) return rc;
@@ -1216,7 +1215,8 @@ int rhy_amr_coarse_step(rhy_box_t* b, const rhy_params_t* p, const double dx[3],
   const Consts k = make_consts(p);
   if ((rc = b->ndim == 2 ? launch_bc<2>(b, p, -1) : launch_bc<3>(b, p, -1))) return rc;
   hipLaunchKernelGGL(set_step_kernel, dim3(1), dim3(64), 0, b->stream, b->dt, step_consts(b),
-                     dt, dx[0], b->ndim > 1 ? dx[1] : 1.0, b->ndim > 2 ? dx[2] : 1.0);
+                     dt, dx[0], b->ndim > 1 ? dx[1] : 1.0, b->ndim > 2 ? dx[2] : 1.0,
+                     b->defer_count, b->partial + b->acc_blocks);
   rc = b->ndim == 2 ? launch_acc_step<2>(b, k, dx, p->solver_type)
                     : launch_acc_step<3>(b, k, dx, p->solver_type);
   if (rc) return rc;

@@ -1134,10 +1134,13 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
 // dt = courant*minval(dx)/max_u (rhyme_cfl_time_step.f90:49); the product
 // courant*minval(dx) is formed on the host.  Writes dt (and logs it) and the
 // raw max_u bits to maxu_out when given.
+// reset (with stepc): after the reduction has read every partial, also the
+// step's deferred-face counters (4) and the fixup's partial slot, as
+// step_reset_kernel would before the step's walks.
 [[maybe_unused]] static __global__ __launch_bounds__(1024) void dt_kernel(
     const double* __restrict__ partial, int32_t n, double cmdx, double* dt_out, double* dt_log,
     double* maxu_out, double dx0 = 1.0, double dx1 = 1.0, double dx2 = 1.0,
-    double* stepc = nullptr) {
+    double* stepc = nullptr, uint32_t* reset_counts = nullptr, double* reset_slot = nullptr) {
   __shared__ double wave_max[16];
   double v = strided_max(partial, 0, n, (int)threadIdx.x, (int)blockDim.x);
 #pragma unroll
@@ -1165,6 +1168,9 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
         }
       }
     }
+    if (reset_counts)
+      for (int i = 0; i < 4; ++i) reset_counts[i] = 0u;
+    if (reset_slot) *reset_slot = 0.0;
   }
 }
 
@@ -1174,8 +1180,12 @@ __device__ __forceinline__ double strided_max(const double* __restrict__ p, int6
 
 // A given dt and the step constants the walks read, as dt_kernel forms them.
 [[maybe_unused]] static __global__ void set_step_kernel(double* dt_out, double* stepc, double dt,
-                                                        double dx0, double dx1, double dx2) {
+                                                        double dx0, double dx1, double dx2,
+                                                        uint32_t* reset_counts,
+                                                        double* reset_slot) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int i = 0; i < 4; ++i) reset_counts[i] = 0u;  // as step_reset_kernel
+  *reset_slot = 0.0;
   const double dx[3] = {dx0, dx1, dx2};
   *dt_out = dt;
   stepc[0] = dt;
